@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Throughput of the fused HIP Tsit5 forward + adjoint on one MI355X.
+
+Workload: Lorenz ensemble (sigma=10, beta=8/3), N = 1,048,576 trajectories, rho ~ U(20, 30),
+y0 = [1, 0, 0] + 0.1 N(0, 1), t in [0, 1], Tsit5 adaptive at rtol = atol = 1e-8, FP64.
+One step = the fused forward with the accepted-h tape and sqrt(n)-spaced checkpoints, then
+the hand-written adjoint kernel for the gradient of sum(y_final^2) w.r.t. rho and y0.
+Inputs are resident in HBM before the timed region; buffers are preallocated.
+
+Prints one JSON line (rank 0). Extra objects: "roofline" for the dominant kernel (HIP
+events on the launch stream), "cpu_baseline" (the oracle on a bounded sample on host
+cores), and "plain" (the same forward with the gradient by autograd through a step-by-step
+replay, the reference's gradient strategy).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "1M-traj Lorenz Tsit5 fwd+adjoint traj/s at rtol 1e-8: fused HIP vs plain"
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--ckpt", type=int, default=0, help="checkpoint interval (0 = autotune)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 17)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--strict", action="store_true",
+                    help="contraction-off forward (rounds like the IEEE-sequential oracle)")
+    return ap.parse_args()
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = _parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the ensemble on a single MI355X "
+              "(trajectories shard by contiguous ranges with no exchange; see DESIGN.md). "
+              f"--gpus {args.gpus} / WORLD_SIZE {world} is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gradsolve_amd import _native as N
+    from gradsolve_amd import hip_tsit5 as H
+    from gradsolve_amd import workcounts as WC
+    from gradsolve_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    ens = W.lorenz_1m(seed=0, n=args.n)
+    n = ens.y0.shape[0]
+    y0 = torch.as_tensor(np.ascontiguousarray(ens.y0.T), device=dev)
+    p = torch.as_tensor(np.ascontiguousarray(ens.params.T), device=dev)
+    kw = dict(t0=ens.t0, t1=ens.t1, rtol=ens.rtol, atol=ens.atol, max_steps=100000)
+    _log(f"workload lorenz n={n}")
+
+    # Pilot record: sizes the tape (capacity = longest trajectory + margin).
+    rec = H.record(N.FIELD_LORENZ, y0, p, tape_cap=256, ckpt_interval=8, **kw)
+    torch.cuda.synchronize()
+    s_max = rec.max_steps_taken()
+    cap = int(s_max + 16)
+    del rec
+    _log(f"pilot: max accepted steps {s_max}, tape cap {cap}")
+
+    def run_once(ws):
+        ws.forward(y0, p, **kw)
+        torch.mul(ws.y_final, 2.0, out=ws.ybar)
+        ws.adjoint(p, ws.ybar, t0=ens.t0)
+
+    # Checkpoint-interval autotune (measured; the candidates bracket sqrt(S)).
+    if args.ckpt:
+        C = args.ckpt
+    else:
+        best = None
+        for cand in (1, 4, 8, 11, 16):
+            ws = H.Workspace(N.FIELD_LORENZ, 3, 1, n, cap, cand, dev, strict=args.strict)
+            run_once(ws)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(2):
+                run_once(ws)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t) / 2
+            _log(f"autotune ckpt_interval={cand}: {dt * 1e3:.2f} ms")
+            if best is None or dt < best[0]:
+                best = (dt, cand)
+            del ws
+        C = best[1]
+    ws = H.Workspace(N.FIELD_LORENZ, 3, 1, n, cap, C, dev, strict=args.strict)
+
+    for _ in range(args.warmup):
+        run_once(ws)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        ws.forward(y0, p, **kw)
+        ev[k][1].record(stream)
+        torch.mul(ws.y_final, 2.0, out=ws.ybar)
+        ws.adjoint(p, ws.ybar, t0=ens.t0)
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    _log(f"timed {args.steps} steps in {t_total:.4f} s")
+
+    status = ws.status
+    bad = int((status != 0).sum().item())
+    if bad:
+        raise RuntimeError(f"{bad} trajectories did not reach t1 (status != 0)")
+    n_acc = ws.n_acc.to(torch.int64)
+    n_rej = ws.n_rej.to(torch.int64)
+    acc_total = int(n_acc.sum().item())
+    trials_total = acc_total + int(n_rej.sum().item())
+
+    fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    adj_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    ms_per_step = t_total * 1e3 / args.steps
+    value = n * args.steps / t_total
+
+    fwd_flops = WC.LORENZ_TSIT5_TRIAL_FLOPS * trials_total
+    adj_flops = WC.LORENZ_TSIT5_ADJ_STEP_FLOPS * acc_total
+    kernels = {
+        "tsit5_fwd_kernel": {"ms": fwd_ms, "flops": fwd_flops,
+                             "tflops": fwd_flops / (fwd_ms * 1e-3) / 1e12},
+        "tsit5_adj_kernel": {"ms": adj_ms, "flops": adj_flops,
+                             "tflops": adj_flops / (adj_ms * 1e-3) / 1e12},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    achieved = kernels[dom]["tflops"]
+    roofline = {
+        "bound": "mfma",
+        "compute": "fp64 (VALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+        "kernel": dom,
+        "achieved": round(achieved, 3),
+        "peak": WC.PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / WC.PEAK_FP64_TFLOPS, 4),
+        "traffic": None,
+        "per_kernel": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                           for kk, vv in v.items()} for k, v in kernels.items()},
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "traj/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "lorenz_tsit5_fwd_adjoint", "n": n, "t1": ens.t1,
+                   "rtol": ens.rtol, "atol": ens.atol, "rho": "U(20,30)",
+                   "y0_noise": 0.1, "ckpt_interval": C, "tape_cap": cap,
+                   "arithmetic": "strict" if args.strict else "fma"},
+        "accepted_steps_per_s": round(acc_total * args.steps / t_total, 1),
+        "mean_accepted_steps": round(acc_total / n, 3),
+        "mean_rejected_steps": round((trials_total - acc_total) / n, 3),
+        "roofline": roofline,
+    }
+
+    if not args.no_plain:
+        out["plain"] = _plain_arm(args, ens, y0, p, kw, dev)
+        if out["plain"].get("value"):
+            out["vs_plain"] = round(value / out["plain"]["value"], 3)
+    if not args.no_cpu:
+        out["cpu_baseline"] = _cpu_baseline(args, ens)
+    print(json.dumps(out), flush=True)
+
+
+def _plain_arm(args, ens, y0, p, kw, dev):
+    """HIP forward with the tape (no checkpoints), then the gradient by autograd through a
+    step-by-step replay of the frozen tape with every step rematerialised."""
+    import torch
+
+    from gradsolve_amd import _native as N
+    from gradsolve_amd import hip_tsit5 as H
+    from gradsolve_amd import plain as PL
+
+    n = y0.shape[1]
+    ws = H.Workspace(N.FIELD_LORENZ, 3, 1, n, 512, 1, dev, checkpoints=False)
+
+    def step():
+        ws.forward(y0, p, checkpoints=False, **kw)
+        s = int(ws.n_acc.max().item())
+        rows = torch.arange(s, device=dev, dtype=torch.int32)[:, None]
+        steps = torch.where(rows < ws.n_acc[None, :], ws.tape[:s], 0.0)
+        yf, gy0, gr = PL.grad_lorenz(y0, p[0], steps, remat=True)
+        return yf, gy0, gr
+
+    try:
+        step()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.plain_steps):
+            t = time.perf_counter()
+            yf, gy0, gr = step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        best = min(times)
+        return {"arm": "hip_fwd+torch_autograd_replay_remat", "value": round(n / best, 1),
+                "unit": "traj/s", "ms_per_step": round(best * 1e3, 3),
+                "protocol": f"1 warm-up, best of {args.plain_steps}"}
+    except Exception as exc:  # report, do not hide
+        return {"arm": "hip_fwd+torch_autograd_replay_remat", "value": None,
+                "error": f"{type(exc).__name__}: {exc}"}
+
+
+def _cpu_baseline(args, ens):
+    """The oracle (FP64 C++ restatement of the record + discrete adjoint, OpenMP over
+    trajectories) on a bounded sample of the same workload, on host cores."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+
+    m = min(args.cpu_sample, ens.y0.shape[0])
+    cores = max(1, min(16, os.cpu_count() or 1))
+    y0, prm = ens.y0[:m], ens.params[:m]
+    O.tsit5_record(O.LORENZ, y0[:64], prm[:64], 0.0, 1.0, rtol=1e-8, atol=1e-8,
+                   nthreads=cores)
+    t = time.perf_counter()
+    yf, tape, na, nr, st = O.tsit5_record(O.LORENZ, y0, prm, 0.0, 1.0, rtol=1e-8, atol=1e-8,
+                                          nthreads=cores)
+    O.tsit5_adjoint(O.LORENZ, y0, prm, 0.0, tape, 2.0 * yf, nthreads=cores)
+    dt = time.perf_counter() - t
+    return {"value": round(m / dt, 1), "unit": "traj/s", "cores": cores, "kind": "port",
+            "sample": f"first {m} trajectories of the workload, record + adjoint, "
+                      f"{dt:.2f} s wall"}
+
+
+if __name__ == "__main__":
+    main()

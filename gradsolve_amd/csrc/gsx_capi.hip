@@ -1,0 +1,217 @@
+// gsx_capi.hip -- the extern "C" boundary (include/gradsolve_amd.h): argument validation,
+// error reporting, and the small layout-conversion kernels for tape interchange.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "gsx_internal.h"
+
+namespace gsx {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(GSX_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  return GSX_OK;
+}
+
+bool field_info(int field, int D, FieldInfo* out) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: *out = {3, 1, true, false}; return D == 3;
+    case GSX_FIELD_LINEAR_DECAY: *out = {D, 1, true, false}; return D >= 1 && D <= 3;
+    case GSX_FIELD_ROBERTSON: *out = {3, 3, true, true}; return D == 3;
+    case GSX_FIELD_KURAMOTO: *out = {32, 32, true, false}; return D == 32;
+    case GSX_FIELD_VDP: *out = {2, 1, true, false}; return D == 2;
+    default: return false;
+  }
+}
+
+namespace {
+
+__global__ void tape_to_padded_kernel(int64_t n, const double* __restrict__ tape,
+                                      const int32_t* __restrict__ n_acc, int64_t S,
+                                      double* __restrict__ out) {
+  // One thread per (s, j); j fastest so that reads of the step-major tape coalesce.
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n * S) return;
+  const int64_t s = idx / n;
+  const int64_t j = idx - s * n;
+  out[j * S + s] = (s < n_acc[j]) ? tape[s * n + j] : 0.0;
+}
+
+__global__ void tape_from_padded_kernel(int64_t n, const double* __restrict__ in, int64_t S,
+                                        double* __restrict__ tape, int64_t cap,
+                                        int32_t* __restrict__ n_acc) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int32_t last = 0;
+  for (int64_t s = 0; s < S; ++s) {
+    const double h = in[j * S + s];
+    tape[s * n + j] = h;
+    if (h != 0.0) last = static_cast<int32_t>(s + 1);
+  }
+  for (int64_t s = S; s < cap; ++s) tape[s * n + j] = 0.0;
+  n_acc[j] = last;
+}
+
+int check_field(int field, int D, int P, FieldInfo* fi) {
+  if (!field_info(field, D, fi))
+    return set_error(GSX_E_INVALID, "field %d is not built for D=%d", field, D);
+  if (P != fi->P)
+    return set_error(GSX_E_INVALID, "field %d expects P=%d parameters, got %d", field, fi->P, P);
+  return GSX_OK;
+}
+
+}  // namespace
+}  // namespace gsx
+
+using namespace gsx;
+
+extern "C" {
+
+int gsx_version(void) { return 100; }
+
+const char* gsx_last_error(void) { return g_err; }
+
+int gsx_tsit5_tableau(double* c, double* a, double* b, double* e) {
+  if (!c || !a || !b || !e) return set_error(GSX_E_INVALID, "tableau: null output pointer");
+  using namespace tab;
+  const double cc[5] = {c2, c3, c4, c5, c6};
+  const double aa[15] = {a21, a31, a32, a41, a42, a43, a51, a52,
+                         a53, a54, a61, a62, a63, a64, a65};
+  const double bb[6] = {b1, b2, b3, b4, b5, b6};
+  const double ee[7] = {e1, e2, e3, e4, e5, e6, e7};
+  std::memcpy(c, cc, sizeof cc);
+  std::memcpy(a, aa, sizeof aa);
+  std::memcpy(b, bb, sizeof bb);
+  std::memcpy(e, ee, sizeof ee);
+  return GSX_OK;
+}
+
+int gsx_ckpt_rows(int field, int D) {
+  FieldInfo fi;
+  if (!field_info(field, D, &fi))
+    return set_error(GSX_E_INVALID, "field %d is not built for D=%d", field, D);
+  return fi.D + (fi.autonomous ? 0 : 1);
+}
+
+int64_t gsx_ckpt_slots(int64_t tape_cap, int32_t ckpt_interval) {
+  if (tape_cap < 0 || ckpt_interval < 1)
+    return set_error(GSX_E_INVALID, "ckpt_slots: tape_cap %lld, interval %d",
+                     static_cast<long long>(tape_cap), ckpt_interval);
+  return tape_cap / ckpt_interval + 1;
+}
+
+int gsx_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                  const double* p, double t0, double t1, const double* t1_per, double rtol,
+                  double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
+                  int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
+                  double* ckpt, int32_t ckpt_interval, int flags, void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
+  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+    return set_error(GSX_E_INVALID, "tsit5_fwd: null input/output pointer");
+  if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (ckpt && !tape) return set_error(GSX_E_INVALID, "checkpoints need a tape");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  FwdArgs a{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1};
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & GSX_STRICT)
+    return gsx_strict::tsit5_fwd_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  return tsit5_fwd_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_tsit5_replay(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                     const double* p, double t0, const double* tape, const int32_t* n_acc,
+                     double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                     void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!y0 || !p || !tape || !n_acc || !y_final))
+    return set_error(GSX_E_INVALID, "tsit5_replay: null pointer");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  ReplayArgs a{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1};
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & GSX_STRICT)
+    return gsx_strict::tsit5_replay_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  return tsit5_replay_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                      const double* p, double t0, const double* tape, int64_t tape_cap,
+                      const int32_t* n_acc, const double* ckpt, int32_t ckpt_interval,
+                      const double* ybar, double* y0bar, double* pbar, int flags,
+                      void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!p || !tape || !n_acc || !ckpt || !ybar || !y0bar || !pbar))
+    return set_error(GSX_E_INVALID, "tsit5_adjoint: null pointer");
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & GSX_STRICT)
+    return gsx_strict::tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  return tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                       int64_t S, double* out, void* stream) {
+  if (n < 0 || S < 0 || S > tape_cap)
+    return set_error(GSX_E_INVALID, "tape_to_padded: n %lld, S %lld, cap %lld",
+                     static_cast<long long>(n), static_cast<long long>(S),
+                     static_cast<long long>(tape_cap));
+  if (n == 0 || S == 0) return GSX_OK;
+  if (!tape || !n_acc || !out) return set_error(GSX_E_INVALID, "tape_to_padded: null pointer");
+  const int64_t total = n * S;
+  const unsigned grid = static_cast<unsigned>((total + 255) / 256);
+  tape_to_padded_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(n, tape, n_acc, S,
+                                                                            out);
+  return check_launch("tape_to_padded_kernel");
+}
+
+int gsx_tape_from_padded(int64_t n, const double* in, int64_t S, double* tape,
+                         int64_t tape_cap, int32_t* n_acc, void* stream) {
+  if (n < 0 || S < 0 || S > tape_cap)
+    return set_error(GSX_E_INVALID, "tape_from_padded: n %lld, S %lld, cap %lld",
+                     static_cast<long long>(n), static_cast<long long>(S),
+                     static_cast<long long>(tape_cap));
+  if (n == 0) return GSX_OK;
+  if ((S > 0 && !in) || !tape || !n_acc)
+    return set_error(GSX_E_INVALID, "tape_from_padded: null pointer");
+  const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+  tape_from_padded_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(n, in, S, tape,
+                                                                              tape_cap, n_acc);
+  return check_launch("tape_from_padded_kernel");
+}
+
+}  // extern "C"

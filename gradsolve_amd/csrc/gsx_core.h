@@ -1,0 +1,471 @@
+// gsx_core.h -- per-trajectory (per-lane) building blocks of the ensemble ODE kernels.
+//
+// Everything here is written once for one trajectory and instantiated by the HIP kernels
+// with one trajectory per lane. The functions are __host__ __device__ so that the same
+// source also compiles as plain C++ for a CPU test harness that checks indexing and
+// arithmetic against the oracle without a GPU; the product library only ever runs them
+// on the device.
+//
+// Memory layout in HBM (structure of arrays, index j = trajectory, coalesced across lanes):
+//   y0, y_final, ybar, y0bar : [D][n]        element (d, j) at d*n + j
+//   p, pbar                  : [P][n]
+//   tape                     : [cap][n]      accepted step s of trajectory j at s*n + j
+//   ckpt                     : [slots][DCK][n] state after m*C accepted steps at slot m
+//                              (DCK = D, plus one row holding t for non-autonomous fields)
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GSX_HD __host__ __device__ __forceinline__
+#define GSX_UNROLL _Pragma("unroll")
+#else
+#define GSX_HD inline
+#define GSX_UNROLL
+#endif
+
+#include "gsx_args.h"
+
+// The computational code lives in GSX_NS. The library compiles it twice: once in namespace
+// gsx with fused multiply-add contraction (the fast variant), and once in namespace
+// gsx_strict with contraction off, whose forward rounds every operation exactly as the
+// IEEE-sequential reference loop does.
+#ifndef GSX_NS
+#define GSX_NS gsx
+#endif
+
+namespace GSX_NS {
+
+using ::gsx::AdjArgs;
+using ::gsx::FwdArgs;
+using ::gsx::ReplayArgs;
+using ::gsx::kExhausted;
+using ::gsx::kReached;
+using ::gsx::kTapeFull;
+
+// ---------------------------------------------------------------------------------------
+// Tsitouras (2011) 5(4) pair (Comput. Math. Appl. 62(2), 770-775). The decimal strings and
+// the binary64 expressions (e_i = b_i - bhat_i) are those of the published table as
+// gradsolve holds it in gradsolve/solvers/tsit5_step.py:31-77.
+// ---------------------------------------------------------------------------------------
+namespace tab {
+constexpr double c2 = 161.0 / 1000.0;
+constexpr double c3 = 327.0 / 1000.0;
+constexpr double c4 = 9.0 / 10.0;
+constexpr double c5 = 0.9800255409045096857298102862870245954942137979563024768854764293221195950761080302604;
+constexpr double c6 = 1.0;
+constexpr double a21 = 161.0 / 1000.0;
+constexpr double a31 = -0.8480655492356988544426874250230774675121177393430391537369234245294192976164141156943e-2;
+constexpr double a32 = 0.3354806554923569885444268742502307746751211773934303915373692342452941929761641411569;
+constexpr double a41 = 2.897153057105493432130432594192938764924887287701866490314866693455023795137503079289;
+constexpr double a42 = -6.359448489975074843148159912383825625952700647415626703305928850207288721235210244366;
+constexpr double a43 = 4.362295432869581411017727318190886861027813359713760212991062156752264926097707165077;
+constexpr double a51 = 5.325864828439256604428877920840511317836476253097040101202360397727981648835607691791;
+constexpr double a52 = -11.74888356406282787774717033978577296188744178259862899288666928009020615663593781589;
+constexpr double a53 = 7.495539342889836208304604784564358155658679161518186721010132816213648793440552049753;
+constexpr double a54 = -0.9249506636175524925650207933207191611349983406029535244034750452930469056411389539635e-1;
+constexpr double a61 = 5.861455442946420028659251486982647890394337666164814434818157239052507339770711679748;
+constexpr double a62 = -12.92096931784710929170611868178335939541780751955743459166312250439928519268343184452;
+constexpr double a63 = 8.159367898576158643180400794539253485181918321135053305748355423955009222648673734986;
+constexpr double a64 = -0.7158497328140099722453054252582973869127213147363544882721139659546372402303777878835e-1;
+constexpr double a65 = -0.2826905039406838290900305721271224146717633626879770007617876201276764571291579142206e-1;
+constexpr double b1 = 0.9646076681806522951816731316512876333711995238157997181903319145764851595234062815396e-1;
+constexpr double b2 = 1.0 / 100.0;
+constexpr double b3 = 0.4798896504144995747752495322905965199130404621990332488332634944254542060153074523509;
+constexpr double b4 = 1.379008574103741893192274821856872770756462643091360525934940067397245698027561293331;
+constexpr double b5 = -3.290069515436080679901047585711363850115683290894936158531296799594813811049925401677;
+constexpr double b6 = 2.324710524099773982415355918398765796109060233222962411944060046314465391054716027841;
+constexpr double e1 = 0.9646076681806522951816731316512876333711995238157997181903319145764851595234062815396e-1 - 0.9468075576583945807478876255758922856117527357724631226139574065785592789071067303271e-1;
+constexpr double e2 = 1.0 / 100.0 - 0.9183565540343253096776363936645313759813746240984095238905939532922955247253608687270e-2;
+constexpr double e3 = 0.4798896504144995747752495322905965199130404621990332488332634944254542060153074523509 - 0.4877705284247615707855642599631228241516691959761363774365216240304071651579571959813;
+constexpr double e4 = 1.379008574103741893192274821856872770756462643091360525934940067397245698027561293331 - 1.234297566930478985655109673884237654035539930748192848315425833500484878378061439761;
+constexpr double e5 = -3.290069515436080679901047585711363850115683290894936158531296799594813811049925401677 + 2.707712349983525454881109975059321670689605166938197378763992255714444407154902012702;
+constexpr double e6 = 2.324710524099773982415355918398765796109060233222962411944060046314465391054716027841 - 1.866628418170587035753719399566211498666255505244122593996591602841258328965767580089;
+constexpr double e7 = -1.0 / 66.0;
+}  // namespace tab
+
+// Step-size controller constants (Hairer-Norsett-Wanner II.4; gradsolve/solvers/tsit5_step.py:26).
+constexpr double kSafety = 0.9;
+constexpr double kErrFloor = 1e-10;
+constexpr double kFacMin = 0.2;
+constexpr double kFacMax = 5.0;
+constexpr double kTsit5Exp = -0.2;
+
+// Python's builtin min/max argument semantics (min(a, b) is a unless b < a), which the
+// reference controller uses (gradsolve/solvers/tsit5_replay.py:108,124). They propagate a NaN
+// in the first operand, unlike fmin/fmax.
+template <class T>
+GSX_HD T py_min(T a, T b) { return b < a ? b : a; }
+template <class T>
+GSX_HD T py_max(T a, T b) { return b > a ? b : a; }
+
+
+// ---------------------------------------------------------------------------------------
+// Fields. f(t, y, p, dy) and the vector-Jacobian product vjp(t, y, p, v, ybar+=, pbar+=).
+// Each field's textual association follows the registered field it restates.
+// ---------------------------------------------------------------------------------------
+
+// Lorenz (D=3, P=1: rho; sigma=10, beta=8/3): gradsolve/warp/_warp_kernel.py:111-130,
+// gradsolve/cuda/_fields.py:12-18.
+struct FLorenz {
+  static constexpr int D = 3, P = 1;
+  static constexpr bool autonomous = true;
+  template <class T>
+  GSX_HD static void f(T, const T* y, const T* p, T* dy, const double*) {
+    const T sigma = T(10.0), beta = T(8.0 / 3.0), rho = p[0];
+    dy[0] = sigma * (y[1] - y[0]);
+    dy[1] = rho * y[0] - y[1] - y[0] * y[2];
+    dy[2] = y[0] * y[1] - beta * y[2];
+  }
+  template <class T>
+  GSX_HD static void vjp(T, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    const T sigma = T(10.0), beta = T(8.0 / 3.0), rho = p[0];
+    yb[0] += -sigma * v[0] + (rho - y[2]) * v[1] + y[1] * v[2];
+    yb[1] += sigma * v[0] - v[1] + y[0] * v[2];
+    yb[2] += -y[0] * v[1] - beta * v[2];
+    pb[0] += y[0] * v[1];
+  }
+};
+
+// Linear decay f = -p0*y (test field with a closed-form solution), fixed D.
+template <int DD>
+struct FLinearDecay {
+  static constexpr int D = DD, P = 1;
+  static constexpr bool autonomous = true;
+  template <class T>
+  GSX_HD static void f(T, const T* y, const T* p, T* dy, const double*) {
+    GSX_UNROLL for (int i = 0; i < D; ++i) dy[i] = -p[0] * y[i];
+  }
+  template <class T>
+  GSX_HD static void vjp(T, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    T acc = T(0);
+    GSX_UNROLL for (int i = 0; i < D; ++i) {
+      yb[i] += -p[0] * v[i];
+      acc += y[i] * v[i];
+    }
+    pb[0] -= acc;
+  }
+};
+
+// Van der Pol (D=2, P=1: mu): gradsolve/warp/_warp_kernel.py:158-174.
+struct FVdp {
+  static constexpr int D = 2, P = 1;
+  static constexpr bool autonomous = true;
+  template <class T>
+  GSX_HD static void f(T, const T* y, const T* p, T* dy, const double*) {
+    const T x = y[0], v = y[1];
+    dy[0] = v;
+    dy[1] = p[0] * ((T(1.0) - x * x) * v - x);
+  }
+  template <class T>
+  GSX_HD static void vjp(T, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    const T x = y[0], w = y[1];
+    yb[0] += p[0] * (T(-2.0) * x * w - T(1.0)) * v[1];
+    yb[1] += v[0] + p[0] * (T(1.0) - x * x) * v[1];
+    pb[0] += ((T(1.0) - x * x) * w - x) * v[1];
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// NumPy-order reductions: the reference's WRMS norm is np.mean((e/sc)**2)
+// (gradsolve/solvers/tsit5_replay.py:118). Below 8 elements NumPy sums left to right; from 8
+// to 128 it uses eight interleaved accumulators combined pairwise.
+// ---------------------------------------------------------------------------------------
+template <int D, class T>
+GSX_HD T np_sum(const T* a) {
+  if constexpr (D < 8) {
+    T r = a[0];
+    GSX_UNROLL for (int i = 1; i < D; ++i) r = r + a[i];
+    return r;
+  } else {
+    static_assert(D <= 128, "pairwise block limit");
+    T r[8];
+    GSX_UNROLL for (int q = 0; q < 8; ++q) r[q] = a[q];
+    constexpr int full = D - (D % 8);
+    GSX_UNROLL for (int i = 8; i < full; i += 8) {
+      GSX_UNROLL for (int q = 0; q < 8; ++q) r[q] = r[q] + a[i + q];
+    }
+    T res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    GSX_UNROLL for (int i = full; i < D; ++i) res = res + a[i];
+    return res;
+  }
+}
+
+template <int D>
+GSX_HD double wrms(const double* e, const double* y, const double* ynew, double rtol,
+                   double atol) {
+  double q[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) {
+    const double sc = atol + rtol * py_max(std::fabs(y[i]), std::fabs(ynew[i]));
+    const double r = e[i] / sc;
+    q[i] = r * r;
+  }
+  return std::sqrt(np_sum<D>(q) / double(D));
+}
+
+// ---------------------------------------------------------------------------------------
+// Tsit5 stage block. Given k1 (FSAL) computes z2..z6 / k2..k6 and y5 = y + h*sum b_i k_i.
+// Stage inputs follow tsit5_step.py:88-99: z_i = y + h*(a_i1 k1 + ... ).
+// ---------------------------------------------------------------------------------------
+template <class F>
+GSX_HD void tsit5_stages(double t, const double* y, double h, const double* p,
+                         double (&k)[7][F::D], double* y5, const double* aux) {
+  constexpr int D = F::D;
+  using namespace tab;
+  double z[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (a21 * k[0][i]);
+  F::f(t + c2 * h, z, p, k[1], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (a31 * k[0][i] + a32 * k[1][i]);
+  F::f(t + c3 * h, z, p, k[2], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[i] = y[i] + h * (a41 * k[0][i] + a42 * k[1][i] + a43 * k[2][i]);
+  F::f(t + c4 * h, z, p, k[3], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[i] = y[i] + h * (a51 * k[0][i] + a52 * k[1][i] + a53 * k[2][i] + a54 * k[3][i]);
+  F::f(t + c5 * h, z, p, k[4], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[i] = y[i] + h * (a61 * k[0][i] + a62 * k[1][i] + a63 * k[2][i] + a64 * k[3][i] +
+                       a65 * k[4][i]);
+  F::f(t + c6 * h, z, p, k[5], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    y5[i] = y[i] + h * (b1 * k[0][i] + b2 * k[1][i] + b3 * k[2][i] + b4 * k[3][i] +
+                        b5 * k[4][i] + b6 * k[5][i]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward + accepted-h record for one trajectory (gradsolve/solvers/tsit5_replay.py:100-124,
+// gradsolve/warp/_warp_kernel.py:338-399). The recorder writes h into the tape at row acc
+// and the state into checkpoint slot acc/C every C accepted steps.
+// ---------------------------------------------------------------------------------------
+
+template <class F>
+GSX_HD constexpr int ckpt_rows() { return F::D + (F::autonomous ? 0 : 1); }
+
+template <class F>
+GSX_HD void ckpt_store(double* ck, int64_t slot, int64_t n, int64_t j, const double* y,
+                       double t) {
+  constexpr int DCK = ckpt_rows<F>();
+  double* base = ck + slot * DCK * n + j;
+  GSX_UNROLL for (int d = 0; d < F::D; ++d) base[d * n] = y[d];
+  if constexpr (!F::autonomous) base[F::D * n] = t;
+  (void)t;
+}
+
+template <class F>
+GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
+  constexpr int D = F::D, P = F::P;
+  using namespace tab;
+  const int64_t n = a.n;
+  double y[D], p[P], y5[D];
+  double k[7][D];
+  GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.y0[d * n + j];
+  GSX_UNROLL for (int q = 0; q < P; ++q) p[q] = a.p[q * n + j];
+  double t = a.t0;
+  const double t1 = a.t1_per ? a.t1_per[j] : a.t1;
+  double dt = (t1 - t) / 100.0;  // dt0 = (t1 - t0)/100 (gradsolve/warp/warp_replay.py:157)
+  F::f(t, y, p, k[0], a.aux);
+  int32_t acc = 0, rej = 0, status = kReached;
+  const bool rec = a.tape != nullptr;
+  const bool ck = a.ckpt != nullptr;
+  int32_t ck_left = a.ckpt_interval;
+  if (ck) ckpt_store<F>(a.ckpt, 0, n, j, y, t);
+  for (int64_t it = 0; it < a.max_steps; ++it) {
+    if (t >= t1) break;
+    const double h = py_min(dt, t1 - t);
+    tsit5_stages<F>(t, y, h, p, k, y5, a.aux);
+    F::f(t + h, y5, p, k[6], a.aux);
+    double e[D];
+    GSX_UNROLL for (int i = 0; i < D; ++i)
+      e[i] = h * (e1 * k[0][i] + e2 * k[1][i] + e3 * k[2][i] + e4 * k[3][i] + e5 * k[4][i] +
+                  e6 * k[5][i] + e7 * k[6][i]);
+    const double err = wrms<D>(e, y, y5, a.rtol, a.atol);
+    if (err != err) {
+      // A NaN error makes the next step size NaN; the reference loop then rejects every
+      // remaining attempt (tsit5_replay.py:108-124) and ends with status 1 at the same state.
+      rej += static_cast<int32_t>(a.max_steps - it);
+      break;
+    }
+    if (err <= 1.0) {
+      if (rec && acc >= a.tape_cap) {
+        status = kTapeFull;
+        break;
+      }
+      t = t + h;
+      GSX_UNROLL for (int d = 0; d < D; ++d) {
+        y[d] = y5[d];
+        k[0][d] = k[6][d];
+      }
+      if (rec) a.tape[static_cast<int64_t>(acc) * n + j] = h;
+      ++acc;
+      if (ck && --ck_left == 0) {
+        ckpt_store<F>(a.ckpt, acc / a.ckpt_interval, n, j, y, t);
+        ck_left = a.ckpt_interval;
+      }
+    } else {
+      ++rej;
+    }
+    const double fac = kSafety * std::pow(py_max(err, kErrFloor), kTsit5Exp);
+    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+  }
+  if (status == kReached && !(t >= t1)) status = kExhausted;
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];
+  a.n_acc[j] = acc;
+  a.n_rej[j] = rej;
+  a.status[j] = status;
+}
+
+// ---------------------------------------------------------------------------------------
+// Discrete adjoint of the frozen-h replay for one trajectory (the VJP of
+// gradsolve/warp/warp_replay.py:30-55 / tsit5_step.py:80-102). Segments of C accepted steps
+// are re-integrated forward from their checkpoint into `seg` (the states at the start of each
+// step, stride `sstride` between consecutive doubles of one lane), then walked in reverse,
+// recomputing each step's stages from its start state. k7 carries weight b7 = 0 and is not
+// formed.
+// ---------------------------------------------------------------------------------------
+
+// One replay step without the error estimate and without k7 (y_next == y5).
+template <class F>
+GSX_HD void tsit5_advance(double t, double* y, double h, const double* p, const double* aux) {
+  constexpr int D = F::D;
+  double k[7][D];
+  double y5[D];
+  F::f(t, y, p, k[0], aux);
+  tsit5_stages<F>(t, y, h, p, k, y5, aux);
+  GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = y5[d];
+}
+
+// lam <- (d y_next / d y)^T lam ; pb += (d y_next / d p)^T lam, at the step (t, y, h).
+template <class F>
+GSX_HD void tsit5_step_vjp(double t, const double* y, double h, const double* p, double* lam,
+                           double* pb, const double* aux) {
+  constexpr int D = F::D;
+  using namespace tab;
+  // Recompute the stage inputs z1..z6 (z1 = y).
+  double z[6][D];
+  double k[5][D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[0][i] = y[i];
+  F::f(t, z[0], p, k[0], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[1][i] = y[i] + h * (a21 * k[0][i]);
+  F::f(t + c2 * h, z[1], p, k[1], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[2][i] = y[i] + h * (a31 * k[0][i] + a32 * k[1][i]);
+  F::f(t + c3 * h, z[2], p, k[2], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[3][i] = y[i] + h * (a41 * k[0][i] + a42 * k[1][i] + a43 * k[2][i]);
+  F::f(t + c4 * h, z[3], p, k[3], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[4][i] = y[i] + h * (a51 * k[0][i] + a52 * k[1][i] + a53 * k[2][i] + a54 * k[3][i]);
+  F::f(t + c5 * h, z[4], p, k[4], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i)
+    z[5][i] = y[i] + h * (a61 * k[0][i] + a62 * k[1][i] + a63 * k[2][i] + a64 * k[3][i] +
+                          a65 * k[4][i]);
+  // Reverse: kbar_s = h*b_s*lam + sum_{r>s} a_rs * (h*zbar_r); zbar_s = J(z_s)^T kbar_s.
+  constexpr double A[6][6] = {{0, 0, 0, 0, 0, 0},
+                              {a21, 0, 0, 0, 0, 0},
+                              {a31, a32, 0, 0, 0, 0},
+                              {a41, a42, a43, 0, 0, 0},
+                              {a51, a52, a53, a54, 0, 0},
+                              {a61, a62, a63, a64, a65, 0}};
+  constexpr double B[6] = {b1, b2, b3, b4, b5, b6};
+  constexpr double Cn[6] = {0.0, c2, c3, c4, c5, c6};
+  double hl[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) hl[i] = h * lam[i];
+  double kb[6][D];
+  GSX_UNROLL for (int s = 0; s < 6; ++s) {
+    GSX_UNROLL for (int i = 0; i < D; ++i) kb[s][i] = B[s] * hl[i];
+  }
+  double lam_new[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) lam_new[i] = lam[i];
+  GSX_UNROLL for (int s = 5; s >= 0; --s) {
+    double zb[D];
+    GSX_UNROLL for (int i = 0; i < D; ++i) zb[i] = 0.0;
+    F::vjp(t + Cn[s] * h, z[s], p, kb[s], zb, pb, aux);
+    GSX_UNROLL for (int i = 0; i < D; ++i) lam_new[i] += zb[i];
+    double hz[D];
+    GSX_UNROLL for (int i = 0; i < D; ++i) hz[i] = h * zb[i];
+    GSX_UNROLL for (int r = 0; r < s; ++r) {
+      GSX_UNROLL for (int i = 0; i < D; ++i) kb[r][i] += A[s][r] * hz[i];
+    }
+  }
+  GSX_UNROLL for (int i = 0; i < D; ++i) lam[i] = lam_new[i];
+}
+
+// Frozen-mesh replay (the forward of gradsolve/warp/warp_replay.py:30-55) from possibly new
+// (y0, p) over a recorded tape, writing the checkpoints the adjoint needs.
+
+template <class F>
+GSX_HD void tsit5_replay_lane(const ReplayArgs& a, int64_t j) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t n = a.n;
+  double y[D], p[P];
+  GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.y0[d * n + j];
+  GSX_UNROLL for (int q = 0; q < P; ++q) p[q] = a.p[q * n + j];
+  double t = a.t0;
+  const int32_t na = a.n_acc[j];
+  const bool ck = a.ckpt != nullptr;
+  if (ck) ckpt_store<F>(a.ckpt, 0, n, j, y, t);
+  int32_t ck_left = a.ckpt_interval;
+  for (int32_t s = 0; s < na; ++s) {
+    const double h = a.tape[static_cast<int64_t>(s) * n + j];
+    tsit5_advance<F>(t, y, h, p, a.aux);
+    t = t + h;
+    if (ck && --ck_left == 0) {
+      ckpt_store<F>(a.ckpt, (s + 1) / a.ckpt_interval, n, j, y, t);
+      ck_left = a.ckpt_interval;
+    }
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];
+}
+
+template <class F>
+GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
+  constexpr int D = F::D, P = F::P;
+  constexpr int DCK = ckpt_rows<F>();
+  const int64_t n = a.n;
+  double p[P], pb[P], lam[D];
+  GSX_UNROLL for (int q = 0; q < P; ++q) {
+    p[q] = a.p[q * n + j];
+    pb[q] = 0.0;
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  const int32_t na = a.n_acc[j];
+  const int32_t C = a.ckpt_interval;
+  const int32_t nseg = (na + C - 1) / C;
+  for (int32_t m = nseg - 1; m >= 0; --m) {
+    const int32_t s0 = m * C;
+    const int32_t s1 = (s0 + C < na) ? s0 + C : na;
+    double y[D];
+    double t;
+    const double* base = a.ckpt + static_cast<int64_t>(m) * DCK * n + j;
+    GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = base[d * n];
+    if constexpr (!F::autonomous) {
+      t = base[D * n];
+    } else {
+      t = 0.0;
+    }
+    // Re-integrate the segment, keeping the state at the start of each step.
+    for (int32_t s = s0; s < s1; ++s) {
+      double* slot = seg + static_cast<int64_t>(s - s0) * DCK * sstride;
+      GSX_UNROLL for (int d = 0; d < D; ++d) slot[d * sstride] = y[d];
+      if constexpr (!F::autonomous) slot[D * sstride] = t;
+      if (s + 1 < s1) {
+        const double h = a.tape[static_cast<int64_t>(s) * n + j];
+        tsit5_advance<F>(t, y, h, p, a.aux);
+        t = t + h;
+      }
+    }
+    // Walk the segment in reverse.
+    for (int32_t s = s1 - 1; s >= s0; --s) {
+      const double* slot = seg + static_cast<int64_t>(s - s0) * DCK * sstride;
+      GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = slot[d * sstride];
+      if constexpr (!F::autonomous) t = slot[D * sstride];
+      const double h = a.tape[static_cast<int64_t>(s) * n + j];
+      tsit5_step_vjp<F>(t, y, h, p, lam, pb, a.aux);
+    }
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
+  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
+}
+
+}  // namespace GSX_NS

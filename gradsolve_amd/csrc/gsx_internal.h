@@ -1,0 +1,36 @@
+// gsx_internal.h -- declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gradsolve_amd.h"
+#include "gsx_core.h"
+
+namespace gsx {
+
+constexpr int kMaxCkptInterval = 32;
+
+// Records a printf-style message as the calling thread's last error; returns `code`.
+int set_error(int code, const char* fmt, ...);
+// Checks hipGetLastError after a launch; returns GSX_OK or GSX_E_HIP.
+int check_launch(const char* what);
+
+struct FieldInfo {
+  int D, P;
+  bool autonomous;
+  bool stiff;
+};
+// Static facts about a built-in field at dimension D; false if (field, D) is not built.
+bool field_info(int field, int D, FieldInfo* out);
+
+int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
+int tsit5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
+int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+
+}  // namespace gsx
+
+namespace gsx_strict {
+int tsit5_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
+int tsit5_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
+int tsit5_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+}  // namespace gsx_strict

@@ -1,0 +1,123 @@
+/*
+ * gradsolve_amd.h -- C ABI of the MI355X ensemble ODE record-and-replay library
+ * (libgradsolve_amd.so, gfx950).
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream handle passed as
+ * `void*` (a hipStream_t; NULL = the legacy default stream). Calls are stream-ordered and
+ * asynchronous: they enqueue kernels and return; the caller synchronizes. No entry point
+ * allocates device memory: the caller owns every buffer, sized with the query functions.
+ *
+ * Device layout (structure of arrays; j = trajectory index, n = ensemble size):
+ *   y0, y_final, ybar, y0bar : double[D][n]
+ *   p, pbar                  : double[P][n]
+ *   t1_per                   : double[n]                 (optional per-trajectory end time)
+ *   n_acc, n_rej, status     : int32[n]
+ *   tape                     : double[tape_cap][n]       accepted step s of trajectory j
+ *                                                         at tape[s*n + j]
+ *   ckpt                     : double[gsx_ckpt_slots(..)][gsx_ckpt_rows(field)][n]
+ *
+ * Status codes per trajectory (status[j]): 0 reached t1, 1 exhausted max_steps,
+ * 2 step-size underflow / degenerate step, 3 tape capacity exceeded.
+ *
+ * Return codes: GSX_OK (0) or a negative GSX_E* value; gsx_last_error() gives the text.
+ */
+#ifndef GRADSOLVE_AMD_H
+#define GRADSOLVE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gsx_status {
+  GSX_OK = 0,
+  GSX_E_INVALID = -1,     /* bad argument: unknown field, dimension mismatch, null pointer */
+  GSX_E_HIP = -2,         /* a HIP runtime call or kernel launch failed */
+  GSX_E_UNSUPPORTED = -3  /* valid request this build does not implement */
+};
+
+/* Built-in fields (the registered fields of gradsolve's fused kernels). */
+enum gsx_field {
+  GSX_FIELD_LORENZ = 1,       /* D=3, P=1 (rho); sigma=10, beta=8/3 */
+  GSX_FIELD_LINEAR_DECAY = 2, /* D in {1,2,3}, P=1: f = -p0*y */
+  GSX_FIELD_ROBERTSON = 3,    /* D=3, P=3 (k1, k2, k3); stiff */
+  GSX_FIELD_KURAMOTO = 4,     /* D=32, P=32 (omega); aux = K[32*32] row-major */
+  GSX_FIELD_VDP = 5           /* D=2, P=1 (mu) */
+};
+
+/* Arithmetic flags. Default (0): the kernels contract a*b + c into fused multiply-adds.
+ * GSX_STRICT: contraction off, so every operation rounds once in the reference loop's
+ * textual order (gradsolve/solvers/tsit5_replay.py:100-124); the recorded tape then matches
+ * an IEEE-sequential host loop up to the last-ulp differences of the device pow. */
+enum gsx_flags { GSX_STRICT = 1 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gsx_version(void);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* gsx_last_error(void);
+
+/* The Tsit5 tableau compiled into the kernels: c2..c6 (5), a21..a65 (15, row order),
+ * b1..b6 (6), e1..e7 (7). Host memory. No reference counterpart; used to check the
+ * kernels' constants against gradsolve/solvers/tsit5_step.py:31-77. */
+int gsx_tsit5_tableau(double* c, double* a, double* b, double* e);
+
+/* Rows per checkpoint slot for `field` (D, plus one for t when the field is
+ * non-autonomous), and the number of checkpoint slots needed for a tape of `tape_cap`
+ * accepted steps checkpointed every `ckpt_interval` accepted steps. */
+int gsx_ckpt_rows(int field, int D);
+int64_t gsx_ckpt_slots(int64_t tape_cap, int32_t ckpt_interval);
+
+/* Fused adaptive Tsit5 forward + accepted-h record, one trajectory per lane.
+ * Replaces the XLA FFI handler Tsit5Fwd (gradsolve/cuda/kernels/tsit5_template.cu.in:127-178:
+ * y0[D,n], rho[n] -> yf[D,n], nsteps[n]; attrs t1, rtol, atol, max_steps) and the Warp
+ * forward+record kernel _tsit5_adaptive_general
+ * (gradsolve/warp/_warp_kernel.py:325-399, driven by warp_ode._launch, warp_ode.py:215-327).
+ * tape may be NULL (forward only); ckpt may be NULL (no adjoint will follow).
+ * t1_per may be NULL (every trajectory ends at t1). dt0 = (t1 - t0)/100 per trajectory. */
+int gsx_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux,
+                  const double* y0, const double* p, double t0, double t1,
+                  const double* t1_per, double rtol, double atol, int64_t max_steps,
+                  double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                  double* tape, int64_t tape_cap, double* ckpt, int32_t ckpt_interval,
+                  int flags, void* stream);
+
+/* Frozen-mesh replay from (possibly new) y0, p over a recorded tape: the forward of
+ * replay_solve_jax (gradsolve/warp/warp_replay.py:30-55, one tsit5_step per recorded h,
+ * gradsolve/solvers/tsit5_step.py:80-102), writing the checkpoints gsx_tsit5_adjoint reads.
+ * The tape holds the first n_acc[j] accepted steps of trajectory j at tape[s*n + j]. */
+int gsx_tsit5_replay(int field, int64_t n, int D, int P, const double* aux,
+                     const double* y0, const double* p, double t0,
+                     const double* tape, const int32_t* n_acc,
+                     double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                     void* stream);
+
+/* Reverse sweep: the discrete adjoint of the frozen-h Tsit5 replay, walking the tape
+ * backwards from the checkpoints written by gsx_tsit5_fwd (same tape_cap and
+ * ckpt_interval). ybar = dL/dy_final; outputs dL/dy0 and dL/dp.
+ * Replaces jax.grad through replay_solve_jax (gradsolve/warp/warp_replay.py:30-55). */
+int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                      const double* y0, const double* p, double t0,
+                      const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                      const double* ckpt, int32_t ckpt_interval,
+                      const double* ybar, double* y0bar, double* pbar, int flags,
+                      void* stream);
+
+/* Export the tape to the reference's zero-padded trajectory-major form
+ * out[n][S] (gradsolve/warp/warp_replay.py:180-182): out[j*S+s] = tape[s*n+j] for
+ * s < n_acc[j], else 0. */
+int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap,
+                       const int32_t* n_acc, int64_t S, double* out, void* stream);
+
+/* Import a reference-format padded tape in[n][S] (zero past each trajectory's accepted
+ * count) into the step-major tape layout; n_acc[j] is set to the index after the last
+ * non-zero step of row j. */
+int gsx_tape_from_padded(int64_t n, const double* in, int64_t S, double* tape,
+                         int64_t tape_cap, int32_t* n_acc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRADSOLVE_AMD_H */
