@@ -1,0 +1,98 @@
+"""The per-lane kernel bodies (gradsolve_amd/csrc/gsx_core.h), compiled for the CPU with
+contraction off, against the oracle: bitwise forward/tape/counts, adjoint to 1e-13, and
+guard regions around every output buffer (sized exactly as the C ABI sizes them) intact."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gradsolve_amd import workloads as W
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "native_cpu", "liblane_harness.so")
+G = 512  # guard elements either side
+
+
+@pytest.fixture(scope="module")
+def lh():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
+    L = ctypes.CDLL(LIB)
+    vp, i64, i32, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_int
+    L.lh_tsit5_fwd.argtypes = [i, i64, vp, vp, d, d, vp, d, d, i64, vp, vp, vp, vp, vp, i64, vp, i32]
+    L.lh_tsit5_replay.argtypes = [i, i64, vp, vp, d, vp, vp, vp, vp, i32]
+    L.lh_tsit5_adjoint.argtypes = [i, i64, vp, d, vp, i64, vp, vp, i32, vp, vp, vp]
+    return L
+
+
+class Guarded:
+    def __init__(self, shape, dtype=np.float64):
+        size = int(np.prod(shape))
+        fill = np.nan if dtype == np.float64 else -7
+        self.buf = np.full(size + 2 * G, fill, dtype=dtype)
+        self.a = self.buf[G:G + size].reshape(shape)
+        self.dtype = dtype
+
+    @property
+    def ptr(self):
+        return self.a.ctypes.data
+
+    def intact(self):
+        g = np.concatenate([self.buf[:G], self.buf[-G:]])
+        return bool(np.all(np.isnan(g))) if self.dtype == np.float64 else bool(np.all(g == -7))
+
+
+@pytest.mark.parametrize("C", [1, 3, 8, 32])
+def test_lane_code_matches_oracle(lh, C):
+    e = W.lorenz(200, seed=11, noise=0.1)
+    n = 200
+    cap = 200
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape = Guarded((cap, n))
+    ck = Guarded((cap // C + 1, 3, n))
+    lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 100000,
+                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C)
+    assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
+    oyf, otape, ona, onr, ost = O.tsit5_record(O.LORENZ, e.y0, e.params, 0.0, 1.0, rtol=1e-8,
+                                               atol=1e-8)
+    assert np.array_equal(na.a, ona) and np.array_equal(nr.a, onr) and np.all(st.a == 0)
+    assert np.array_equal(yf.a.T, oyf)
+    padded = np.zeros_like(otape)
+    for j in range(n):
+        padded[j, :na.a[j]] = tape.a[:na.a[j], j]
+    assert np.array_equal(padded, otape)
+    ry, rck = Guarded((3, n)), Guarded((cap // C + 1, 3, n))
+    lh.lh_tsit5_replay(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr,
+                       rck.ptr, C)
+    assert ry.intact() and rck.intact()
+    assert np.array_equal(ry.a, yf.a)
+    ybar = np.ascontiguousarray(2.0 * yf.a)
+    gy, gp = Guarded((3, n)), Guarded((1, n))
+    lh.lh_tsit5_adjoint(1, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+                        ybar.ctypes.data, gy.ptr, gp.ptr)
+    assert gy.intact() and gp.intact()
+    og0, ogp = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, otape, 2.0 * oyf)
+    assert np.max(np.abs(gp.a[0] - ogp[:, 0]) / np.abs(ogp[:, 0])) < 1e-13
+    assert np.max(np.abs(gy.a.T - og0) / np.abs(og0).max(axis=1, keepdims=True)) < 1e-13
+
+
+def test_lane_tape_full_and_exhausted(lh):
+    e = W.lorenz(8, seed=12)
+    n, cap = 8, 20
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape, ck = Guarded((cap, n)), Guarded((cap // 4 + 1, 3, n))
+    lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 100000,
+                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, 4)
+    assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
+    assert np.all(st.a == 3) and np.all(na.a == cap)
+    lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 7,
+                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, 4)
+    assert np.all(st.a == 1) and np.all(na.a + nr.a == 7)
