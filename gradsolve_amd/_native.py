@@ -24,6 +24,7 @@ FIELD_LINEAR_DECAY = 2
 FIELD_ROBERTSON = 3
 FIELD_KURAMOTO = 4
 FIELD_VDP = 5
+FIELD_LORENZ3 = 6
 
 _lock = threading.Lock()
 _lib = None
@@ -47,6 +48,12 @@ SIGNATURES = {
                               _i, _vp]),
     "gsx_tsit5_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                _i32, _vp, _vp, _vp, _i, _vp]),
+    "gsx_rb23_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _d, _d, _i64,
+                          _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _i, _vp]),
+    "gsx_rb23_replay": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i32,
+                             _i, _vp]),
+    "gsx_rb23_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
+                              _i32, _vp, _vp, _vp, _i, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
 }

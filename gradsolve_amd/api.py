@@ -1,0 +1,264 @@
+"""Public API: ``solve`` and ``grad_closure`` with the reference's signatures
+(gradsolve/api.py:291-400 and :792-1030), routed to the native engines.
+
+Differences from the reference, all reported rather than hidden:
+
+* engines run on the GPU (``device`` "cuda"/"gpu"); ``device="cpu"`` raises, because this
+  build has no CPU engine;
+* the reference's general-RHS engines (``tsit5_replay``, ``rodas5p_replay``, ``vern7_replay``,
+  ``fixed_step_*``, ``diffrax``) are not part of this build: a request that needs one raises
+  ``ValueError`` naming it;
+* closures are differentiated with torch.autograd (a ``torch.autograd.Function`` whose
+  backward is the native adjoint kernel) -- JAX is not installed here.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import dispatch as D
+from . import fields as F
+from .base import Route, SolveResult
+
+_VALID_WRT = "'params', 'y0', or ('y0', 'params')"
+_NOT_BUILT = {"tsit5_replay", "rodas5p_replay", "vern7_replay", "fixed_step_tsit5",
+              "fixed_step_imex", "diffrax", "fused_rosenbrock_backward"}
+
+
+@dataclass(frozen=True)
+class EngineSpec:
+    """An engine: name, ``supports(problem)``, forward ``solve`` and optional ``reverse``
+    closure factory (gradsolve/api.py:48-71)."""
+
+    name: str
+    supports: Callable
+    solve: Callable
+    reverse: Optional[Callable] = None
+
+
+def _normalize_wrt(wrt) -> tuple:
+    if wrt == "params":
+        return ("params",)
+    if wrt == "y0":
+        return ("y0",)
+    if isinstance(wrt, (tuple, list, set, frozenset)) and set(wrt) == {"y0", "params"}:
+        return ("y0", "params")
+    raise ValueError(f"invalid wrt={wrt!r}; expected {_VALID_WRT}")
+
+
+def _device(device):
+    import torch
+    d = str(device)
+    if d in ("cuda", "gpu"):
+        d = "cuda:0"
+    if not d.startswith("cuda"):
+        raise RuntimeError(
+            f"device={device!r}: this build has no CPU engine; the native engines run on the "
+            "GPU (pass device='cuda').")
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU is visible (torch.cuda.is_available() is False); the "
+                           "native engines need an MI355X")
+    return torch.device(d)
+
+
+def _check_shapes(problem, y0, params):
+    y0 = np.asarray(y0) if not hasattr(y0, "shape") else y0
+    if len(y0.shape) != 2 or y0.shape[1] != problem.dim:
+        raise ValueError(f"y0 must have shape (n, {problem.dim}); got {tuple(y0.shape)}")
+    if len(params.shape) != 2 or params.shape[0] != y0.shape[0]:
+        raise ValueError(f"params must have shape (n, P) with n={y0.shape[0]}; got "
+                         f"{tuple(params.shape)}")
+
+
+def _native_ok(binding, problem, params) -> bool:
+    if binding is None:
+        return False
+    if binding.param_cols is not None and params.shape[1] <= max(binding.param_cols):
+        return False
+    if binding.param_cols is None and params.shape[1] != binding.n_params:
+        return False
+    return True
+
+
+def _supports_tsit5(problem) -> bool:
+    b = F.binding_for(problem)
+    return b is not None and not b.stiff and not problem.is_stiff
+
+
+def _supports_rb23(problem) -> bool:
+    b = F.binding_for(problem)
+    return b is not None and b.stiff and problem.is_stiff
+
+
+def _solve_tsit5(problem, y0, params, *, rtol, atol, device, strict=False,
+                 max_steps=100000) -> SolveResult:
+    import torch
+
+    from . import hip_tsit5 as H
+    b = F.binding_for(problem)
+    dev = _device(device)
+    y0_t = torch.as_tensor(np.asarray(y0, np.float64), device=dev) \
+        if not isinstance(y0, torch.Tensor) else y0.to(dev, torch.float64)
+    p_t = torch.as_tensor(np.asarray(params, np.float64), device=dev) \
+        if not isinstance(params, torch.Tensor) else params.to(dev, torch.float64)
+    aux = None
+    if b.aux_of is not None:
+        aux = torch.as_tensor(b.aux_of(problem), device=dev)
+    yf, na, nr, st = H.forward_only(b.field_id, y0_t.t().contiguous(),
+                                    b.native_params(p_t).t().contiguous(), t0=problem.t0,
+                                    t1=problem.t1, rtol=rtol, atol=atol, max_steps=max_steps,
+                                    aux=aux, strict=strict)
+    y_final = yf.t().cpu().numpy().copy()
+    status = st.cpu().numpy()
+    y_final[status != 0] = np.nan  # failed rows NaN'd by the driver (warp_ode.py:319)
+    return SolveResult(y_final=y_final, accepted_steps=na.cpu().numpy().astype(np.int64),
+                       rejected_steps=nr.cpu().numpy().astype(np.int64), solver=D.HIP_TSIT5)
+
+
+def _solve_rb23(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult:
+    from . import hip_rosenbrock23 as R
+    return R.solve(problem, y0, params, rtol=rtol, atol=atol, device=device)
+
+
+ENGINE_REGISTRY = {
+    D.HIP_TSIT5: EngineSpec(D.HIP_TSIT5, _supports_tsit5, _solve_tsit5, None),
+    D.HIP_REPLAY: EngineSpec(D.HIP_REPLAY, _supports_tsit5, _solve_tsit5, "tsit5"),
+    D.HIP_ROSENBROCK23: EngineSpec(D.HIP_ROSENBROCK23, _supports_rb23, _solve_rb23, "rb23"),
+}
+
+
+def _resolve(engine, problem, need_grad, batch_n, accuracy_target):
+    reasons = []
+    if engine == "auto":
+        name = D.choose_engine(problem.dim, problem.is_stiff, need_grad, batch_n=batch_n,
+                               accuracy_target=accuracy_target)
+    elif engine in ENGINE_REGISTRY:
+        name = engine
+    elif engine in D.ALIASES:
+        name = D.ALIASES[engine]
+        if need_grad and name == D.HIP_TSIT5:
+            name = D.HIP_REPLAY
+        reasons.append(f"alias:{engine}")
+    elif engine in _NOT_BUILT:
+        raise ValueError(f"engine {engine!r} is one of the reference's general engines, which "
+                         "this build does not include; use engine='auto' with a registered "
+                         f"field ({F.registered_names()})")
+    else:
+        raise ValueError(f"unknown engine {engine!r}; known: {sorted(ENGINE_REGISTRY)} + "
+                         f"aliases {sorted(D.ALIASES)}")
+    if name in _NOT_BUILT:
+        raise ValueError(
+            f"problem {problem.name!r} (dim {problem.dim}, stiff={problem.is_stiff}) routes to "
+            f"the reference's general engine {name!r}, which this build does not include")
+    spec = ENGINE_REGISTRY[name]
+    if not spec.supports(problem):
+        raise ValueError(
+            f"no native field serves problem {problem.name!r}: the reference would fall back "
+            "to its general-RHS engines, which this build does not include. Register the "
+            "field with register_jax_field or use one of " + str(F.registered_names()))
+    return name, reasons
+
+
+def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
+          engine: str = "auto", fused="auto", rtol: float = 1e-6, atol: float = 1e-9,
+          device: str = "cuda", batch_n: Optional[int] = None,
+          accuracy_target: Optional[float] = None, precision: str = "float64",
+          strict: bool = False) -> SolveResult:
+    """Route and run a batched ensemble solve (gradsolve/api.py:291-400).
+
+    Returns host NumPy arrays; rows whose trajectory did not reach ``t1`` are NaN. ``strict``
+    selects the contraction-off kernels (rounding like the IEEE-sequential reference loop).
+    """
+    if fused not in ("auto", True, False):
+        raise ValueError(f"fused must be True, False, or 'auto'; got {fused!r}")
+    if saveat_dense:
+        raise ValueError("saveat_dense is implemented only on the reference's rodas5p_replay "
+                         "lane, which this build does not include")
+    if precision not in ("float64", "float32"):
+        raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
+    _check_shapes(problem, y0, params)
+    name, reasons = _resolve(engine, problem, need_grad=saveat is not None, batch_n=batch_n,
+                             accuracy_target=accuracy_target)
+    b = F.binding_for(problem)
+    if not _native_ok(b, problem, params):
+        raise ValueError(f"params shape {tuple(params.shape)} does not match the field "
+                         f"{b.key!r} ({b.n_params} parameters)")
+    if saveat is not None:
+        from . import dense
+        res = dense.solve_saveat(name, b, problem, y0, params, saveat, rtol=rtol, atol=atol,
+                                 device=device, precision=precision)
+    elif precision == "float32":
+        from . import fp32
+        res = fp32.solve(b, problem, y0, params, rtol=rtol, atol=atol, device=device)
+    else:
+        res = ENGINE_REGISTRY[name].solve(problem, y0, params, rtol=rtol, atol=atol,
+                                          device=device, strict=strict)
+    res.solver = name
+    res.route = Route(requested=engine, actual=name,
+                      reason="; ".join(reasons) if reasons else None)
+    return res
+
+
+def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense: bool = False,
+                 precision: str = "float64", engine: str = "auto", fused="auto",
+                 rtol: float = 1e-6, atol: float = 1e-9, device: str = "cuda",
+                 batch_n: Optional[int] = None, accuracy_target: Optional[float] = None,
+                 remat: Optional[bool] = None, ckpt_interval: Optional[int] = None,
+                 strict: bool = False):
+    """A differentiable closure over a mesh recorded once at ``(y0, params)``
+    (gradsolve/api.py:792-1030).
+
+    ``wrt`` selects the closure's arguments: ``params -> y_final``, ``y0 -> y_final`` or
+    ``(y0, params) -> y_final``. Call it with torch tensors and differentiate with
+    torch.autograd (``.backward()`` / ``torch.autograd.grad``); the backward is the native
+    adjoint kernel. ``remat`` maps onto the checkpoint interval (``False``: a checkpoint at
+    every accepted step; ``True``/``None``: sqrt(S)-spaced); ``ckpt_interval`` sets it
+    directly. The closure carries ``.route``.
+    """
+    wrt_t = _normalize_wrt(wrt)
+    if fused not in ("auto", True, False):
+        raise ValueError(f"fused must be True, False, or 'auto'; got {fused!r}")
+    if saveat_dense:
+        raise ValueError("saveat_dense=True requires the reference's rodas5p_replay lane, "
+                         "which this build does not include")
+    if precision not in ("float64", "float32"):
+        raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
+    _check_shapes(problem, y0, params)
+    name, reasons = _resolve(engine, problem, need_grad=True, batch_n=batch_n,
+                             accuracy_target=accuracy_target)
+    if name == D.HIP_TSIT5:
+        name = D.HIP_REPLAY
+    b = F.binding_for(problem)
+    if not _native_ok(b, problem, params):
+        raise ValueError(f"params shape {tuple(params.shape)} does not match the field "
+                         f"{b.key!r} ({b.n_params} parameters)")
+    dev = _device(device)
+    if ckpt_interval is None and remat is False:
+        ckpt_interval = 1
+    aux = None
+    if b.aux_of is not None:
+        import torch
+        aux = torch.as_tensor(b.aux_of(problem), device=dev)
+    if saveat is not None:
+        from . import dense
+        closure = dense.make_saveat_closure(name, b, problem, y0, params, saveat, wrt=wrt_t,
+                                            rtol=rtol, atol=atol, device=dev, aux=aux,
+                                            precision=precision)
+    elif precision == "float32":
+        from . import fp32
+        closure = fp32.make_closure(name, b, problem, y0, params, wrt=wrt_t, rtol=rtol,
+                                    atol=atol, device=dev, aux=aux)
+    elif name == D.HIP_REPLAY:
+        from .closure import make_tsit5_closure
+        closure = make_tsit5_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
+                                     device=dev, ckpt_interval=ckpt_interval, strict=strict,
+                                     aux=aux)
+    else:
+        from . import hip_rosenbrock23 as R
+        closure = R.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
+                                 device=dev, ckpt_interval=ckpt_interval)
+    closure.route = Route(requested=engine, actual=name,
+                          reason="; ".join(reasons) if reasons else None)
+    return closure

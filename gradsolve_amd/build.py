@@ -42,6 +42,7 @@ def hipcc() -> str:
 # contraction off (namespace gsx_strict).
 VARIANTS = {
     "gsx_tsit5.hip": [("", []), ("strict", ["-DGSX_NS=gsx_strict", "-ffp-contract=off"])],
+    "gsx_rb23.hip": [("", []), ("strict", ["-DGSX_NS=gsx_strict", "-ffp-contract=off"])],
 }
 
 

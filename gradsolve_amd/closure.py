@@ -1,0 +1,180 @@
+"""Record-once gradient closures on the native engines, differentiable with torch.autograd.
+
+``ReplayClosure`` records the accepted-step mesh once, at the ``(y0, params)`` it is built
+with, exactly as the reference's closure factories do (gradsolve/warp/warp_replay.py:186-237,
+gradsolve/solvers/tsit5_replay.py:294-334). Calling it replays that frozen mesh from the
+given inputs; the backward pass is the native adjoint kernel. At the recording inputs the
+replay is the recording itself (its checkpoints are reused); at other inputs the replay
+kernel re-integrates the frozen mesh and writes fresh checkpoints.
+
+The gradient semantics are the reference's: step sizes are data, so the gradient is the
+exact discrete adjoint of the frozen-step integration, valid near the recording point.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import hip_tsit5 as H
+
+
+def _as_device(x, dev) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+
+
+class _Tsit5Lane:
+    """Native calls of the Tsit5 record-and-replay lane."""
+
+    def __init__(self, binding, aux, strict):
+        self.b, self.aux, self.strict = binding, aux, strict
+
+    def record(self, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, ckpt_interval):
+        return H.record(self.b.field_id, y0_soa, p_soa, t0=t0, t1=t1, rtol=rtol, atol=atol,
+                        max_steps=max_steps, aux=self.aux, ckpt_interval=ckpt_interval,
+                        strict=self.strict)
+
+    def replay(self, rec, y0_soa, p_soa):
+        y_final = torch.empty_like(rec.y_final)
+        ckpt = torch.empty_like(rec.ckpt)
+        N.call("gsx_tsit5_replay", rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux),
+               y0_soa.data_ptr(), p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(),
+               rec.n_acc.data_ptr(), y_final.data_ptr(), ckpt.data_ptr(), rec.ckpt_interval,
+               N.STRICT if self.strict else 0, H._stream_handle(None))
+        return y_final, ckpt
+
+    def adjoint(self, rec, ckpt, p_soa, ybar_soa):
+        y0bar = torch.empty((rec.D, rec.n), dtype=torch.float64, device=ybar_soa.device)
+        pbar = torch.empty((rec.P, rec.n), dtype=torch.float64, device=ybar_soa.device)
+        N.call("gsx_tsit5_adjoint", rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux), None,
+               p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(), rec.cap, rec.n_acc.data_ptr(),
+               ckpt.data_ptr(), rec.ckpt_interval, ybar_soa.data_ptr(), y0bar.data_ptr(),
+               pbar.data_ptr(), 0, H._stream_handle(None))
+        return y0bar, pbar
+
+
+class _ReplayFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, p_user, state):
+        y0_soa = z.detach().t().contiguous()
+        p_soa = state.binding.native_params(p_user.detach()).t().contiguous()
+        rec = state.rec
+        if state.is_recording_point(y0_soa, p_soa):
+            y_final, ckpt = rec.y_final, rec.ckpt
+        else:
+            y_final, ckpt = state.lane.replay(rec, y0_soa, p_soa)
+        ctx.state = state
+        ctx.ckpt = ckpt
+        ctx.p_soa = p_soa
+        ctx.p_cols = p_user.shape[1]
+        return y_final.t().contiguous()
+
+    @staticmethod
+    def backward(ctx, gy):
+        st = ctx.state
+        ybar = gy.to(torch.float64).t().contiguous()
+        y0bar, pbar = st.lane.adjoint(st.rec, ctx.ckpt, ctx.p_soa, ybar)
+        g_p = None
+        if ctx.needs_input_grad[1]:
+            g_p = torch.zeros((st.rec.n, ctx.p_cols), dtype=torch.float64, device=gy.device)
+            cols = st.binding.param_cols
+            if cols is None:
+                g_p = pbar.t().contiguous()
+            else:
+                g_p[:, list(cols)] = pbar.t()
+        g_z = y0bar.t().contiguous() if ctx.needs_input_grad[0] else None
+        return g_z, g_p, None
+
+
+class ReplayClosure:
+    """Callable closure over a frozen mesh; see the module docstring.
+
+    Arguments follow ``wrt``: ``closure(params)``, ``closure(y0)`` or ``closure(y0, params)``,
+    each shaped like the corresponding input. torch tensors (any device; computed on the
+    recording device) give a differentiable torch result; NumPy inputs give a NumPy result.
+    """
+
+    def __init__(self, lane, binding, problem, y0, params, *, wrt, rtol, atol, device,
+                 ckpt_interval=None, max_steps=100000):
+        self.lane, self.binding, self.problem, self.wrt = lane, binding, problem, wrt
+        self.device = device
+        y0_t = _as_device(y0, device)
+        p_t = _as_device(params, device)
+        if y0_t.ndim == 1:
+            y0_t = y0_t[None, :]
+        if p_t.ndim == 1:
+            p_t = p_t[None, :]
+        self._y0 = y0_t
+        self._params = p_t
+        self._y0_soa = y0_t.t().contiguous()
+        self._p_soa = binding.native_params(p_t).t().contiguous()
+        self.rec = lane.record(self._y0_soa, self._p_soa, t0=problem.t0, t1=problem.t1,
+                               rtol=rtol, atol=atol, max_steps=max_steps,
+                               ckpt_interval=ckpt_interval)
+        st = self.rec.status
+        bad = torch.nonzero(st != 0).flatten()
+        if bad.numel():
+            i = int(bad[0].item())
+            code = int(st[i].item())
+            if code == 2:
+                raise RuntimeError(
+                    f"record: trajectory {i} reached the step-size underflow floor before "
+                    f"t1={problem.t1}; the recorded dt sequence is degenerate and a frozen "
+                    "replay of it would be silently wrong.")
+            raise RuntimeError(
+                f"record: {bad.numel()} trajectories (first: {i}) exhausted max_steps="
+                f"{max_steps} before t1={problem.t1} (status {code}); the recorded dt "
+                "sequence is incomplete and a frozen replay of it would be silently wrong. "
+                "Increase max_steps, loosen the tol, or shorten the horizon.")
+
+    def is_recording_point(self, y0_soa, p_soa) -> bool:
+        return (y0_soa.shape == self._y0_soa.shape and p_soa.shape == self._p_soa.shape
+                and bool(torch.equal(y0_soa, self._y0_soa))
+                and bool(torch.equal(p_soa, self._p_soa)))
+
+    # -- public ------------------------------------------------------------------------
+    @property
+    def n_acc(self) -> np.ndarray:
+        return self.rec.n_acc.cpu().numpy().astype(np.int64)
+
+    @property
+    def n_rej(self) -> np.ndarray:
+        return self.rec.n_rej.cpu().numpy().astype(np.int64)
+
+    def dts_padded(self) -> np.ndarray:
+        """The recorded mesh in the reference's zero-padded ``[n, S]`` form."""
+        return self.rec.padded_tape().cpu().numpy()
+
+    def __call__(self, *args):
+        if self.wrt == ("params",):
+            (p,), z = args, None
+        elif self.wrt == ("y0",):
+            (z,), p = args, None
+        else:
+            z, p = args
+        numpy_in = not any(isinstance(a, torch.Tensor) for a in args)
+        zt = self._y0 if z is None else self._coerce(z)
+        pt = self._params if p is None else self._coerce(p)
+        out = _ReplayFn.apply(zt, pt, self)
+        if numpy_in:
+            return out.detach().cpu().numpy()
+        return out
+
+    def _coerce(self, x):
+        if isinstance(x, torch.Tensor):
+            if x.device != self.device or x.dtype != torch.float64:
+                x = x.to(device=self.device, dtype=torch.float64)
+            return x
+        return _as_device(x, self.device)
+
+
+def make_tsit5_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
+                       ckpt_interval=None, strict=False, aux=None,
+                       max_steps=100000) -> ReplayClosure:
+    lane = _Tsit5Lane(binding, aux, strict)
+    return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                         device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)

@@ -35,6 +35,7 @@ bool field_info(int field, int D, FieldInfo* out) {
     case GSX_FIELD_ROBERTSON: *out = {3, 3, true, true}; return D == 3;
     case GSX_FIELD_KURAMOTO: *out = {32, 32, true, false}; return D == 32;
     case GSX_FIELD_VDP: *out = {2, 1, true, false}; return D == 2;
+    case GSX_FIELD_LORENZ3: *out = {3, 3, true, false}; return D == 3;
     default: return false;
   }
 }
@@ -182,6 +183,72 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux, con
   if (flags & GSX_STRICT)
     return gsx_strict::tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
   return tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                 const double* p, double t0, double t1, const double* t1_per, double rtol,
+                 double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
+                 int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
+                 double* ckpt, int32_t ckpt_interval, int flags, void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
+  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+    return set_error(GSX_E_INVALID, "rb23_fwd: null input/output pointer");
+  if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (ckpt && !tape) return set_error(GSX_E_INVALID, "checkpoints need a tape");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  FwdArgs a{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::rb23_fwd_dispatch(field, D, a, st)
+                              : rb23_fwd_dispatch(field, D, a, st);
+}
+
+int gsx_rb23_replay(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                    const double* p, double t0, const double* tape, const int32_t* n_acc,
+                    double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                    void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!y0 || !p || !tape || !n_acc || !y_final))
+    return set_error(GSX_E_INVALID, "rb23_replay: null pointer");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  ReplayArgs a{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::rb23_replay_dispatch(field, D, a, st)
+                              : rb23_replay_dispatch(field, D, a, st);
+}
+
+int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                     const double* p, double t0, const double* tape, int64_t tape_cap,
+                     const int32_t* n_acc, const double* ckpt, int32_t ckpt_interval,
+                     const double* ybar, double* y0bar, double* pbar, int flags,
+                     void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!p || !tape || !n_acc || !ckpt || !ybar || !y0bar || !pbar))
+    return set_error(GSX_E_INVALID, "rb23_adjoint: null pointer");
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
+                              : rb23_adj_dispatch(field, D, a, st);
 }
 
 int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap, const int32_t* n_acc,

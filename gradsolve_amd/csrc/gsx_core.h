@@ -129,6 +129,31 @@ struct FLorenz {
   }
 };
 
+// Lorenz with all three parameters free (D=3, P=3: sigma, beta, rho), the "lorenz" problem's
+// [sigma, beta, rho] columns (gradsolve/warp/warp_ode.py:552-589). Same association as FLorenz,
+// so at sigma=10, beta=8/3 both fields round identically.
+struct FLorenz3 {
+  static constexpr int D = 3, P = 3;
+  static constexpr bool autonomous = true;
+  template <class T>
+  GSX_HD static void f(T, const T* y, const T* p, T* dy, const double*) {
+    const T sigma = p[0], beta = p[1], rho = p[2];
+    dy[0] = sigma * (y[1] - y[0]);
+    dy[1] = rho * y[0] - y[1] - y[0] * y[2];
+    dy[2] = y[0] * y[1] - beta * y[2];
+  }
+  template <class T>
+  GSX_HD static void vjp(T, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    const T sigma = p[0], beta = p[1], rho = p[2];
+    yb[0] += -sigma * v[0] + (rho - y[2]) * v[1] + y[1] * v[2];
+    yb[1] += sigma * v[0] - v[1] + y[0] * v[2];
+    yb[2] += -y[0] * v[1] - beta * v[2];
+    pb[0] += (y[1] - y[0]) * v[0];
+    pb[1] -= y[2] * v[2];
+    pb[2] += y[0] * v[1];
+  }
+};
+
 // Linear decay f = -p0*y (test field with a closed-form solution), fixed D.
 template <int DD>
 struct FLinearDecay {
@@ -146,6 +171,68 @@ struct FLinearDecay {
       acc += y[i] * v[i];
     }
     pb[0] -= acc;
+  }
+  // Analytic Jacobian (row-major) and d/d(y,p) of sum_ab Jbar_ab J_ab.
+  template <class T>
+  GSX_HD static void jac(T, const T*, const T* p, T* J, const double*) {
+    GSX_UNROLL for (int i = 0; i < D * D; ++i) J[i] = T(0);
+    GSX_UNROLL for (int i = 0; i < D; ++i) J[i * D + i] = -p[0];
+  }
+  template <class T>
+  GSX_HD static void jac_vjp(const T*, const T*, const T* Jb, T*, T* pb, const double*) {
+    T tr = T(0);
+    GSX_UNROLL for (int i = 0; i < D; ++i) tr += Jb[i * D + i];
+    pb[0] -= tr;
+  }
+};
+
+// Robertson kinetics (D=3, P=3: k1, k2, k3), stiff: gradsolve/warp/_warp_rosenbrock.py:96-149
+// (field and analytic Jacobian), gradsolve tests/_oracles/gradsolve_g2_spike_oracle.py:53-69.
+struct FRobertson {
+  static constexpr int D = 3, P = 3;
+  static constexpr bool autonomous = true;
+  template <class T>
+  GSX_HD static void f(T, const T* y, const T* p, T* dy, const double*) {
+    const T k1 = p[0], k2 = p[1], k3 = p[2];
+    dy[0] = -k1 * y[0] + k3 * y[1] * y[2];
+    dy[1] = k1 * y[0] - k3 * y[1] * y[2] - k2 * y[1] * y[1];
+    dy[2] = k2 * y[1] * y[1];
+  }
+  template <class T>
+  GSX_HD static void vjp(T, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    const T k1 = p[0], k2 = p[1], k3 = p[2];
+    yb[0] += -k1 * v[0] + k1 * v[1];
+    yb[1] += k3 * y[2] * v[0] - (k3 * y[2] + T(2.0) * k2 * y[1]) * v[1] +
+             T(2.0) * k2 * y[1] * v[2];
+    yb[2] += k3 * y[1] * v[0] - k3 * y[1] * v[1];
+    pb[0] += y[0] * (v[1] - v[0]);
+    pb[1] += y[1] * y[1] * (v[2] - v[1]);
+    pb[2] += y[1] * y[2] * (v[0] - v[1]);
+  }
+  template <class T>
+  GSX_HD static void jac(T, const T* y, const T* p, T* J, const double*) {
+    const T k1 = p[0], k2 = p[1], k3 = p[2];
+    const T y1 = y[1], y2 = y[2];
+    J[0] = -k1;
+    J[1] = k3 * y2;
+    J[2] = k3 * y1;
+    J[3] = k1;
+    J[4] = -k3 * y2 - T(2.0) * k2 * y1;
+    J[5] = -k3 * y1;
+    J[6] = T(0.0);
+    J[7] = T(2.0) * k2 * y1;
+    J[8] = T(0.0);
+  }
+  template <class T>
+  GSX_HD static void jac_vjp(const T* y, const T* p, const T* Jb, T* yb, T* pb,
+                             const double*) {
+    const T k2 = p[1], k3 = p[2];
+    const T y1 = y[1], y2 = y[2];
+    yb[1] += k3 * Jb[2] - T(2.0) * k2 * Jb[4] - k3 * Jb[5] + T(2.0) * k2 * Jb[7];
+    yb[2] += k3 * Jb[1] - k3 * Jb[4];
+    pb[0] += Jb[3] - Jb[0];
+    pb[1] += T(2.0) * y1 * (Jb[7] - Jb[4]);
+    pb[2] += y2 * Jb[1] + y1 * Jb[2] - y2 * Jb[4] - y1 * Jb[5];
   }
 };
 

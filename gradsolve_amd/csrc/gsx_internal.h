@@ -26,6 +26,9 @@ bool field_info(int field, int D, FieldInfo* out);
 int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int tsit5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
+int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
+int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
 
 }  // namespace gsx
 
@@ -33,4 +36,7 @@ namespace gsx_strict {
 int tsit5_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int tsit5_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int tsit5_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+int rb23_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
+int rb23_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
+int rb23_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
 }  // namespace gsx_strict

@@ -1,0 +1,122 @@
+// gsx_rb23.hip -- Rosenbrock23 forward+record, replay and adjoint kernels for gfx950 (one
+// trajectory per lane; per-lane bodies in gsx_rb23.h). Compiled twice like gsx_tsit5.hip:
+// namespace gsx (contraction on) and -DGSX_NS=gsx_strict -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "gsx_internal.h"
+#include "gsx_rb23.h"
+
+namespace GSX_NS {
+
+using ::gsx::check_launch;
+using ::gsx::kMaxCkptInterval;
+using ::gsx::set_error;
+
+namespace {
+
+constexpr int kFwdBlock = 256;
+constexpr int kAdjBlock = 64;
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_fwd_kernel(FwdArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_fwd_lane<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_replay_kernel(ReplayArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_replay_lane<F>(a, j);
+}
+
+template <class F, int CMAX>
+__global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
+  __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+}
+
+template <class F>
+int launch_fwd(const FwdArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_fwd_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_fwd_kernel");
+}
+
+template <class F>
+int launch_replay(const ReplayArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_replay_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_replay_kernel");
+}
+
+template <class F, int CMAX>
+int launch_adj_c(const AdjArgs& a, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((a.n + kAdjBlock - 1) / kAdjBlock);
+  rb23_adj_kernel<F, CMAX><<<grid, kAdjBlock, 0, s>>>(a);
+  return check_launch("rb23_adj_kernel");
+}
+
+template <class F>
+int launch_adj(const AdjArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const int C = a.ckpt_interval;
+  if (C <= 1) return launch_adj_c<F, 1>(a, s);
+  if (C <= 4) return launch_adj_c<F, 4>(a, s);
+  if (C <= 8) return launch_adj_c<F, 8>(a, s);
+  if (C <= 16) return launch_adj_c<F, 16>(a, s);
+  if (C <= 32) return launch_adj_c<F, 32>(a, s);
+  return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d", C,
+                   kMaxCkptInterval);
+}
+
+}  // namespace
+
+int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_fwd<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_fwd<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_fwd<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_fwd<FLinearDecay<3>>(a, s);
+      break;
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 forward: field %d with D=%d has no "
+                   "analytic Jacobian in this build", field, D);
+}
+
+int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_replay<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_replay<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_replay<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_replay<FLinearDecay<3>>(a, s);
+      break;
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 replay: field %d with D=%d has no "
+                   "analytic Jacobian in this build", field, D);
+}
+
+int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_adj<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_adj<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_adj<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_adj<FLinearDecay<3>>(a, s);
+      break;
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 adjoint: field %d with D=%d has no "
+                   "analytic Jacobian in this build", field, D);
+}
+
+}  // namespace GSX_NS

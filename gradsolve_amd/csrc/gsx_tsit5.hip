@@ -84,6 +84,7 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
 int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_LORENZ: return launch_fwd<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_fwd<FLorenz3>(a, s);
     case GSX_FIELD_VDP: return launch_fwd<FVdp>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_fwd<FLinearDecay<1>>(a, s);
@@ -99,6 +100,7 @@ int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
 int tsit5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_LORENZ: return launch_replay<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_replay<FLorenz3>(a, s);
     case GSX_FIELD_VDP: return launch_replay<FVdp>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_replay<FLinearDecay<1>>(a, s);
@@ -114,6 +116,7 @@ int tsit5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) 
 int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_LORENZ: return launch_adj<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_adj<FLorenz3>(a, s);
     case GSX_FIELD_VDP: return launch_adj<FVdp>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_adj<FLinearDecay<1>>(a, s);

@@ -43,7 +43,8 @@ enum gsx_field {
   GSX_FIELD_LINEAR_DECAY = 2, /* D in {1,2,3}, P=1: f = -p0*y */
   GSX_FIELD_ROBERTSON = 3,    /* D=3, P=3 (k1, k2, k3); stiff */
   GSX_FIELD_KURAMOTO = 4,     /* D=32, P=32 (omega); aux = K[32*32] row-major */
-  GSX_FIELD_VDP = 5           /* D=2, P=1 (mu) */
+  GSX_FIELD_VDP = 5,          /* D=2, P=1 (mu) */
+  GSX_FIELD_LORENZ3 = 6       /* D=3, P=3 (sigma, beta, rho) */
 };
 
 /* Arithmetic flags. Default (0): the kernels contract a*b + c into fused multiply-adds.
@@ -103,6 +104,39 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux,
                       const double* ckpt, int32_t ckpt_interval,
                       const double* ybar, double* y0bar, double* pbar, int flags,
                       void* stream);
+
+/* Fused adaptive Rosenbrock23 (ode23s, autonomous) forward + accepted-h record for stiff
+ * fields with an analytic Jacobian: W = I - d*h*J(y, p) factorised once per step (LU with
+ * partial pivoting in registers) and reused by the three stage solves; I-controller with
+ * exponent -1/3. Same argument contract as gsx_tsit5_fwd. Replaces the XLA FFI handler
+ * Rosenbrock23Fwd (gradsolve/cuda/kernels/rosenbrock23_template.cu.in:151-223) and the Warp
+ * kernel _rosenbrock23_adaptive (gradsolve/warp/_warp_rosenbrock.py:383-452, driven by
+ * warp_rosenbrock._launch, gradsolve/warp/warp_rosenbrock.py:166-218). */
+int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux,
+                 const double* y0, const double* p, double t0, double t1,
+                 const double* t1_per, double rtol, double atol, int64_t max_steps,
+                 double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                 double* tape, int64_t tape_cap, double* ckpt, int32_t ckpt_interval,
+                 int flags, void* stream);
+
+/* Frozen-mesh Rosenbrock23 replay (rosenbrock23_step without the error stage): the forward
+ * of replay_solve_rosenbrock_jax (gradsolve/warp/warp_replay.py:267-293;
+ * gradsolve/solvers/rosenbrock23_step.py:47-66). */
+int gsx_rb23_replay(int field, int64_t n, int D, int P, const double* aux,
+                    const double* y0, const double* p, double t0,
+                    const double* tape, const int32_t* n_acc,
+                    double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                    void* stream);
+
+/* Discrete adjoint of the Rosenbrock23 replay, differentiating through the stage solves and
+ * W = I - d*h*J(y, p). Replaces jax.grad through replay_solve_rosenbrock_jax
+ * (gradsolve/warp/warp_replay.py:267-293). */
+int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                     const double* y0, const double* p, double t0,
+                     const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                     const double* ckpt, int32_t ckpt_interval,
+                     const double* ybar, double* y0bar, double* pbar, int flags,
+                     void* stream);
 
 /* Export the tape to the reference's zero-padded trajectory-major form
  * out[n][S] (gradsolve/warp/warp_replay.py:180-182): out[j*S+s] = tape[s*n+j] for

@@ -119,6 +119,7 @@ struct FieldSpec {
 bool field_ok(const FieldSpec& fs) {
   switch (fs.id) {
     case GSO_F_LORENZ: return fs.D == 3 && fs.P >= 1;
+    case GSO_F_LORENZ3: return fs.D == 3 && fs.P >= 3;
     case GSO_F_LINEAR_DECAY: return fs.D >= 1 && fs.D <= kMaxD && fs.P >= 1;
     case GSO_F_ROBERTSON: return fs.D == 3 && fs.P >= 3;
     case GSO_F_KURAMOTO: return fs.D >= 1 && fs.D <= kMaxD && fs.P >= fs.D && fs.aux != nullptr;
@@ -135,6 +136,13 @@ void field_eval(const FieldSpec& fs, T t, const T* y, const double* p, T* dy) {
     case GSO_F_LORENZ: {
       // _warp_kernel.py:120-128: [sigma*(y1-y0), rho*y0 - y1 - y0*y2, y0*y1 - beta*y2]
       const T sigma = 10.0, beta = T(8.0 / 3.0), rho = p[0];
+      dy[0] = sigma * (y[1] - y[0]);
+      dy[1] = rho * y[0] - y[1] - y[0] * y[2];
+      dy[2] = y[0] * y[1] - beta * y[2];
+      break;
+    }
+    case GSO_F_LORENZ3: {
+      const T sigma = p[0], beta = p[1], rho = p[2];
       dy[0] = sigma * (y[1] - y[0]);
       dy[1] = rho * y[0] - y[1] - y[0] * y[2];
       dy[2] = y[0] * y[1] - beta * y[2];
@@ -200,6 +208,16 @@ void field_vjp(const FieldSpec& fs, T t, const T* y, const double* p, const T* v
       ybar[1] = ybar[1] + (sigma * v[0] - v[1] + y[0] * v[2]);
       ybar[2] = ybar[2] + (-y[0] * v[1] - beta * v[2]);
       pbar[0] = pbar[0] + y[0] * v[1];
+      break;
+    }
+    case GSO_F_LORENZ3: {
+      const T sigma = p[0], beta = p[1], rho = p[2];
+      ybar[0] = ybar[0] + (-sigma * v[0] + (rho - y[2]) * v[1] + y[1] * v[2]);
+      ybar[1] = ybar[1] + (sigma * v[0] - v[1] + y[0] * v[2]);
+      ybar[2] = ybar[2] + (-y[0] * v[1] - beta * v[2]);
+      pbar[0] = pbar[0] + (y[1] - y[0]) * v[0];
+      pbar[1] = pbar[1] - y[2] * v[2];
+      pbar[2] = pbar[2] + y[0] * v[1];
       break;
     }
     case GSO_F_LINEAR_DECAY: {

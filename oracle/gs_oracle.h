@@ -36,7 +36,8 @@ enum {
   GSO_F_ROBERTSON = 3,     /* D=3, P=3 (k1,k2,k3) (_warp_rosenbrock.py:96-139) */
   GSO_F_KURAMOTO = 4,      /* any D, P=D (omega); aux = K[D*D] row-major */
   GSO_F_VDP = 5,           /* D=2, P=1 (mu) (_warp_kernel.py:158-174) */
-  GSO_F_LINEAR_LADDER = 6  /* any D, P ignored: f = 1.01*y (_warp_kernel.py:133-155) */
+  GSO_F_LORENZ3 = 6,       /* D=3, P=3 (sigma, beta, rho) (warp_ode.py:552-589 columns) */
+  GSO_F_LINEAR_LADDER = 7  /* any D, P ignored: f = 1.01*y (_warp_kernel.py:133-155) */
 };
 
 /* Tsitouras (2011) tableau as the oracle holds it: c2..c6, a21..a65 (15), b1..b6, e1..e7. */

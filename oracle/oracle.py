@@ -22,7 +22,8 @@ LINEAR_DECAY = 2
 ROBERTSON = 3
 KURAMOTO = 4
 VDP = 5
-LINEAR_LADDER = 6
+LORENZ3 = 6
+LINEAR_LADDER = 7
 
 _lib = None
 

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../gradsolve_amd/csrc/gsx_core.h"
+#include "../../gradsolve_amd/csrc/gsx_rb23.h"
 
 using namespace gsx;
 
@@ -26,9 +27,51 @@ int run_adj(const AdjArgs& a) {
   for (int64_t j = 0; j < a.n; ++j) tsit5_adj_lane<F>(a, j, seg.data(), 1);
   return 0;
 }
+template <class F>
+int run_rb_fwd(const FwdArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) rb23_fwd_lane<F>(a, j);
+  return 0;
+}
+template <class F>
+int run_rb_replay(const ReplayArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) rb23_replay_lane<F>(a, j);
+  return 0;
+}
+template <class F>
+int run_rb_adj(const AdjArgs& a) {
+  std::vector<double> seg(static_cast<size_t>(a.ckpt_interval) * ckpt_rows<F>());
+  for (int64_t j = 0; j < a.n; ++j) rb23_adj_lane<F>(a, j, seg.data(), 1);
+  return 0;
+}
 }  // namespace
 
 extern "C" {
+int lh_rb23_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
+                const double* t1_per, double rtol, double atol, int64_t max_steps,
+                double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,
+                int64_t cap, double* ckpt, int32_t C) {
+  FwdArgs a{n, y0, p, nullptr, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, tape ? cap : 0, ckpt, ckpt ? C : 1};
+  if (field == 3) return run_rb_fwd<FRobertson>(a);
+  if (field == 2) return run_rb_fwd<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_rb23_replay(int field, int64_t n, const double* y0, const double* p, double t0,
+                   const double* tape, const int32_t* n_acc, double* y_final, double* ckpt,
+                   int32_t C) {
+  ReplayArgs a{n, y0, p, nullptr, t0, tape, n_acc, y_final, ckpt, ckpt ? C : 1};
+  if (field == 3) return run_rb_replay<FRobertson>(a);
+  if (field == 2) return run_rb_replay<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_rb23_adjoint(int field, int64_t n, const double* p, double t0, const double* tape,
+                    int64_t cap, const int32_t* n_acc, const double* ckpt, int32_t C,
+                    const double* ybar, double* y0bar, double* pbar) {
+  AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  if (field == 3) return run_rb_adj<FRobertson>(a);
+  if (field == 2) return run_rb_adj<FLinearDecay<2>>(a);
+  return -1;
+}
 int lh_tsit5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
                  const double* t1_per, double rtol, double atol, int64_t max_steps,
                  double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,
@@ -36,6 +79,7 @@ int lh_tsit5_fwd(int field, int64_t n, const double* y0, const double* p, double
   FwdArgs a{n, y0, p, nullptr, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
             status, tape, tape ? cap : 0, ckpt, ckpt ? C : 1};
   if (field == 1) return run_fwd<FLorenz>(a);
+  if (field == 6) return run_fwd<FLorenz3>(a);
   if (field == 5) return run_fwd<FVdp>(a);
   if (field == 2) return run_fwd<FLinearDecay<2>>(a);
   return -1;
@@ -45,6 +89,7 @@ int lh_tsit5_replay(int field, int64_t n, const double* y0, const double* p, dou
                     int32_t C) {
   ReplayArgs a{n, y0, p, nullptr, t0, tape, n_acc, y_final, ckpt, ckpt ? C : 1};
   if (field == 1) return run_replay<FLorenz>(a);
+  if (field == 6) return run_replay<FLorenz3>(a);
   if (field == 5) return run_replay<FVdp>(a);
   if (field == 2) return run_replay<FLinearDecay<2>>(a);
   return -1;
@@ -54,6 +99,7 @@ int lh_tsit5_adjoint(int field, int64_t n, const double* p, double t0, const dou
                      const double* ybar, double* y0bar, double* pbar) {
   AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
   if (field == 1) return run_adj<FLorenz>(a);
+  if (field == 6) return run_adj<FLorenz3>(a);
   if (field == 5) return run_adj<FVdp>(a);
   if (field == 2) return run_adj<FLinearDecay<2>>(a);
   return -1;
