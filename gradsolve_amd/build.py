@@ -39,10 +39,11 @@ def hipcc() -> str:
 
 # Translation units compiled more than once, with extra flags per variant: the kernel file
 # is built with the default fused multiply-add contraction (namespace gsx) and again with
-# contraction off (namespace gsx_strict).
+# contraction off and the library's pow and IEEE division (namespace gsx_strict).
+STRICT_FLAGS = ["-DGSX_NS=gsx_strict", "-DGSX_STRICT_ARITH", "-ffp-contract=off"]
 VARIANTS = {
-    "gsx_tsit5.hip": [("", []), ("strict", ["-DGSX_NS=gsx_strict", "-ffp-contract=off"])],
-    "gsx_rb23.hip": [("", []), ("strict", ["-DGSX_NS=gsx_strict", "-ffp-contract=off"])],
+    "gsx_tsit5.hip": [("", []), ("strict", STRICT_FLAGS)],
+    "gsx_rb23.hip": [("", []), ("strict", STRICT_FLAGS)],
 }
 
 

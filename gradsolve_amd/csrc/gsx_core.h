@@ -93,6 +93,78 @@ constexpr double kFacMin = 0.2;
 constexpr double kFacMax = 5.0;
 constexpr double kTsit5Exp = -0.2;
 
+// ---------------------------------------------------------------------------------------
+// Controller power and error-norm reciprocal.
+//
+// The strict variant (GSX_STRICT_ARITH, contraction off) calls the correctly rounded IEEE
+// division and the math library's pow, as the reference loop does. The default variant
+// computes x^e for the controller exponents e = -0.2 (Tsit5) and e = -1/3 (Rosenbrock23)
+// from a single-precision hardware seed (2^(e log2 x)) refined by two Newton steps on
+// g(y) = x y^m - 1 (m = 5 or 3) in double precision, then applies the factor x^(e + 1/m)
+// = exp((e + 1/m) ln x) that separates the binary64 exponent from -1/m; the result is
+// within a few ulp of the correctly rounded power. Inputs are first clamped to
+// [1e-5, 1e4]: outside that range the controller factor 0.9 x^e saturates at the
+// [0.2, 5] clamp for both exponents, so the step size is unchanged.
+// ---------------------------------------------------------------------------------------
+#if defined(__HIP_DEVICE_COMPILE__)
+GSX_HD float fast_log2f(float x) { return __builtin_amdgcn_logf(x); }
+GSX_HD float fast_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
+#else
+GSX_HD float fast_log2f(float x) { return std::log2(x); }
+GSX_HD float fast_exp2f(float x) { return std::exp2(x); }
+#endif
+
+// x^(-1/m) * exp(-c ln x) for positive finite x, m in {3, 5}: see above.
+template <int M>
+GSX_HD double ctrl_pow_newton(double x, double c) {
+  const double xc = x < 1e-5 ? 1e-5 : (x > 1e4 ? 1e4 : x);
+  const float lf = fast_log2f(static_cast<float>(xc));
+  double y = static_cast<double>(fast_exp2f(lf * (-1.0f / M)));
+  GSX_UNROLL for (int it = 0; it < 2; ++it) {
+    const double y2 = y * y;
+    const double ym = (M == 5) ? y2 * y2 * y : y2 * y;
+    const double r = std::fma(xc, ym, -1.0);
+    y = std::fma(y * (-1.0 / M), r, y);
+  }
+  const double lnx = static_cast<double>(lf) * 0.6931471805599453;
+  return std::fma(y, -c * lnx, y);
+}
+
+// x^(-0.2) with -0.2 in binary64: the double nearest 0.2 exceeds 1/5 by
+// 1.1102230246251565e-17, so x^(-0.2) = x^(-1/5) * exp(-1.1102230246251565e-17 * ln x).
+GSX_HD double ctrl_pow_tsit5(double x) {
+#if defined(GSX_STRICT_ARITH)
+  return std::pow(x, -0.2);
+#else
+  if (!(x < 1e300)) return std::pow(x, -0.2);  // inf (and NaN) keep the library's result
+  return ctrl_pow_newton<5>(x, 1.1102230246251565e-17);
+#endif
+}
+
+// x^(-1/3) with -1/3 in binary64: the double nearest 1/3 falls short of it by
+// 1.850371707708594e-17, so x^(-1/3) = x^(-1/3 exact) * exp(+1.850371707708594e-17 * ln x).
+GSX_HD double ctrl_pow_rb23(double x) {
+#if defined(GSX_STRICT_ARITH)
+  return std::pow(x, -1.0 / 3.0);
+#else
+  if (!(x < 1e300)) return std::pow(x, -1.0 / 3.0);
+  return ctrl_pow_newton<3>(x, -1.850371707708594e-17);
+#endif
+}
+
+// a / b for the error norm (b >= atol > 0): the IEEE quotient in the strict variant; in the
+// default variant a times the hardware reciprocal refined by two Newton steps.
+GSX_HD double norm_div(double a, double b) {
+#if defined(GSX_STRICT_ARITH) || !defined(__HIP_DEVICE_COMPILE__)
+  return a / b;
+#else
+  double r = __builtin_amdgcn_rcp(b);
+  r = std::fma(r, std::fma(-b, r, 1.0), r);
+  r = std::fma(r, std::fma(-b, r, 1.0), r);
+  return a * r;
+#endif
+}
+
 // Python's builtin min/max argument semantics (min(a, b) is a unless b < a), which the
 // reference controller uses (gradsolve/solvers/tsit5_replay.py:108,124). They propagate a NaN
 // in the first operand, unlike fmin/fmax.
@@ -286,7 +358,7 @@ GSX_HD double wrms(const double* e, const double* y, const double* ynew, double 
   double q[D];
   GSX_UNROLL for (int i = 0; i < D; ++i) {
     const double sc = atol + rtol * py_max(std::fabs(y[i]), std::fabs(ynew[i]));
-    const double r = e[i] / sc;
+    const double r = norm_div(e[i], sc);
     q[i] = r * r;
   }
   return std::sqrt(np_sum<D>(q) / double(D));
@@ -393,7 +465,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = kSafety * std::pow(py_max(err, kErrFloor), kTsit5Exp);
+    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;

@@ -242,7 +242,7 @@ GSX_HD void rb23_fwd_lane(const FwdArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = kSafety * std::pow(py_max(err, kErrFloor), kRb23Exp);
+    const double fac = kSafety * ctrl_pow_rb23(py_max(err, kErrFloor));
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;

@@ -46,6 +46,16 @@ int run_rb_adj(const AdjArgs& a) {
 }  // namespace
 
 extern "C" {
+// The controller's clamped step-size factor clamp(0.9 * max(err, 1e-10)^e, 0.2, 5) with the
+// build's power (which: 0 Tsit5, 1 Rosenbrock23), and the raw power.
+double lh_ctrl_fac(int which, double err) {
+  const double x = py_max(err, kErrFloor);
+  const double pw = which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
+  return py_min(py_max(kSafety * pw, kFacMin), kFacMax);
+}
+double lh_ctrl_pow(int which, double x) {
+  return which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
+}
 int lh_rb23_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
                 const double* t1_per, double rtol, double atol, int64_t max_steps,
                 double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,

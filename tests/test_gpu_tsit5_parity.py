@@ -143,7 +143,7 @@ def test_replay_kernel_reproduces_forward(gpu):
     ck = torch.empty_like(rec.ckpt)
     N.call("gsx_tsit5_replay", N.FIELD_LORENZ, rec.n, 3, 1, None, rec.y0.data_ptr(),
            rec.p.data_ptr(), 0.0, rec.tape.data_ptr(), rec.n_acc.data_ptr(), yf.data_ptr(),
-           ck.data_ptr(), 8, torch.cuda.current_stream().cuda_stream)
+           ck.data_ptr(), 8, 0, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert torch.max(torch.abs(yf - rec.y_final) / torch.abs(rec.y_final)).item() < 1e-13
 

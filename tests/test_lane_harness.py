@@ -96,3 +96,24 @@ def test_lane_tape_full_and_exhausted(lh):
     lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 7,
                     yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, 4)
     assert np.all(st.a == 1) and np.all(na.a + nr.a == 7)
+
+
+def test_controller_power_fast_variant_accuracy():
+    """The default variant's controller power (hardware f32 seed + two f64 Newton steps) is
+    within 1 ulp of the C library's pow, and the clamped step factor within 2 ulp."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
+    F = ctypes.CDLL(os.path.join(HERE, "native_cpu", "liblane_harness_fast.so"))
+    F.lh_ctrl_pow.restype = ctypes.c_double
+    F.lh_ctrl_pow.argtypes = [ctypes.c_int, ctypes.c_double]
+    F.lh_ctrl_fac.restype = ctypes.c_double
+    F.lh_ctrl_fac.argtypes = [ctypes.c_int, ctypes.c_double]
+    rng = np.random.default_rng(0)
+    xs = np.exp(rng.uniform(np.log(1e-5), np.log(1e4), 20000))
+    xx = np.exp(rng.uniform(np.log(1e-14), np.log(1e12), 20000))
+    for which, e in ((0, -0.2), (1, -1.0 / 3.0)):
+        got = np.array([F.lh_ctrl_pow(which, x) for x in xs])
+        ref = xs ** e
+        assert np.max(np.abs(got - ref) / np.spacing(ref)) <= 1.0
+        fac = np.array([F.lh_ctrl_fac(which, x) for x in xx])
+        rf = np.minimum(np.maximum(0.9 * np.maximum(xx, 1e-10) ** e, 0.2), 5.0)
+        assert np.max(np.abs(fac - rf) / np.spacing(rf)) <= 2.0
