@@ -27,8 +27,10 @@ def _as_device(x, dev) -> torch.Tensor:
     return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
 
 
-class _Tsit5Lane:
-    """Native calls of the Tsit5 record-and-replay lane."""
+class _Lane:
+    """Native calls of one record-and-replay lane (Tsit5 or Rosenbrock23)."""
+
+    FWD, REPLAY, ADJ = "gsx_tsit5_fwd", "gsx_tsit5_replay", "gsx_tsit5_adjoint"
 
     def __init__(self, binding, aux, strict):
         self.b, self.aux, self.strict = binding, aux, strict
@@ -36,12 +38,12 @@ class _Tsit5Lane:
     def record(self, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, ckpt_interval):
         return H.record(self.b.field_id, y0_soa, p_soa, t0=t0, t1=t1, rtol=rtol, atol=atol,
                         max_steps=max_steps, aux=self.aux, ckpt_interval=ckpt_interval,
-                        strict=self.strict)
+                        strict=self.strict, entry=self.FWD)
 
     def replay(self, rec, y0_soa, p_soa):
         y_final = torch.empty_like(rec.y_final)
         ckpt = torch.empty_like(rec.ckpt)
-        N.call("gsx_tsit5_replay", rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux),
+        N.call(self.REPLAY, rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux),
                y0_soa.data_ptr(), p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(),
                rec.n_acc.data_ptr(), y_final.data_ptr(), ckpt.data_ptr(), rec.ckpt_interval,
                N.STRICT if self.strict else 0, H._stream_handle(None))
@@ -50,11 +52,19 @@ class _Tsit5Lane:
     def adjoint(self, rec, ckpt, p_soa, ybar_soa):
         y0bar = torch.empty((rec.D, rec.n), dtype=torch.float64, device=ybar_soa.device)
         pbar = torch.empty((rec.P, rec.n), dtype=torch.float64, device=ybar_soa.device)
-        N.call("gsx_tsit5_adjoint", rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux), None,
+        N.call(self.ADJ, rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux), None,
                p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(), rec.cap, rec.n_acc.data_ptr(),
                ckpt.data_ptr(), rec.ckpt_interval, ybar_soa.data_ptr(), y0bar.data_ptr(),
                pbar.data_ptr(), 0, H._stream_handle(None))
         return y0bar, pbar
+
+
+class _Tsit5Lane(_Lane):
+    pass
+
+
+class _Rb23Lane(_Lane):
+    FWD, REPLAY, ADJ = "gsx_rb23_fwd", "gsx_rb23_replay", "gsx_rb23_adjoint"
 
 
 class _ReplayFn(torch.autograd.Function):
@@ -176,5 +186,13 @@ def make_tsit5_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
                        ckpt_interval=None, strict=False, aux=None,
                        max_steps=100000) -> ReplayClosure:
     lane = _Tsit5Lane(binding, aux, strict)
+    return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                         device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
+
+
+def make_rb23_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
+                      ckpt_interval=None, strict=False, aux=None,
+                      max_steps=50000) -> ReplayClosure:
+    lane = _Rb23Lane(binding, aux, strict)
     return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                          device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)

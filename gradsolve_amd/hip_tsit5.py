@@ -92,11 +92,13 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
            t1_per: Optional[torch.Tensor] = None, aux: Optional[torch.Tensor] = None,
            tape_cap: Optional[int] = None, ckpt_interval: Optional[int] = None,
            stream: Optional[torch.cuda.Stream] = None, grow: bool = True,
-           strict: bool = False) -> Tsit5Record:
+           strict: bool = False, entry: str = "gsx_tsit5_fwd") -> Tsit5Record:
     """Adaptive forward with the accepted-h record and checkpoints (device-resident).
 
     ``y0_soa [D, n]`` and ``p_soa [P, n]`` are float64 device tensors. Returns a record whose
     ``status`` the caller inspects (0 reached t1, 1 exhausted max_steps, 3 tape full).
+    ``entry`` selects the method's forward entry point (``gsx_tsit5_fwd`` or ``gsx_rb23_fwd``;
+    both share the tape and checkpoint contract).
     """
     D, n = y0_soa.shape
     P = p_soa.shape[0]
@@ -112,7 +114,7 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
         status = torch.empty(n, dtype=torch.int32, device=dev)
         tape = torch.empty((cap, n), dtype=torch.float64, device=dev)
         ckpt = _alloc_ckpt(field, D, n, cap, C, dev)
-        N.call("gsx_tsit5_fwd", field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa),
+        N.call(entry, field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa),
                float(t0), float(t1), _ptr(t1_per), float(rtol), float(atol), int(max_steps),
                _ptr(y_final), _ptr(n_acc), _ptr(n_rej), _ptr(status), _ptr(tape), cap,
                _ptr(ckpt), C, N.STRICT if strict else 0, _stream_handle(stream))
@@ -128,7 +130,8 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
 def forward_only(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float,
                  t1: float, rtol: float, atol: float, max_steps: int = 4096,
                  t1_per: Optional[torch.Tensor] = None, aux: Optional[torch.Tensor] = None,
-                 stream: Optional[torch.cuda.Stream] = None, strict: bool = False):
+                 stream: Optional[torch.cuda.Stream] = None, strict: bool = False,
+                 entry: str = "gsx_tsit5_fwd"):
     """The forward-only lane (no tape, no checkpoints): (y_final, n_acc, n_rej, status)."""
     D, n = y0_soa.shape
     P = p_soa.shape[0]
@@ -137,7 +140,7 @@ def forward_only(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: f
     n_acc = torch.empty(n, dtype=torch.int32, device=dev)
     n_rej = torch.empty(n, dtype=torch.int32, device=dev)
     status = torch.empty(n, dtype=torch.int32, device=dev)
-    N.call("gsx_tsit5_fwd", field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa), float(t0),
+    N.call(entry, field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa), float(t0),
            float(t1), _ptr(t1_per), float(rtol), float(atol), int(max_steps), _ptr(y_final),
            _ptr(n_acc), _ptr(n_rej), _ptr(status), None, 0, None, 1,
            N.STRICT if strict else 0, _stream_handle(stream))
