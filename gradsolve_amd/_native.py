@@ -18,6 +18,7 @@ GSX_E_HIP = -2
 GSX_E_UNSUPPORTED = -3
 
 STRICT = 1  # GSX_STRICT: no fused multiply-add contraction
+FP32 = 2    # GSX_FP32: FP32 stages, compensated state, FP64 controller
 
 FIELD_LORENZ = 1
 FIELD_LINEAR_DECAY = 2
@@ -54,6 +55,12 @@ SIGNATURES = {
                              _i, _vp]),
     "gsx_rb23_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                               _i32, _vp, _vp, _vp, _i, _vp]),
+    "gsx_persist_lanes": (_i64, [_i]),
+    "gsx_tsit5_persist_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _d, _d, _i64,
+                                   _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp,
+                                   _i64, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "gsx_tsit5_persist_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _d, _vp, _i64, _vp, _vp,
+                                       _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
 }

@@ -136,9 +136,14 @@ int gsx_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux, const d
   if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
   FwdArgs a{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
             status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1};
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & ~(GSX_STRICT | GSX_FP32))
+    return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if ((flags & GSX_STRICT) && (flags & GSX_FP32))
+    return set_error(GSX_E_INVALID, "GSX_STRICT and GSX_FP32 are exclusive");
   if (flags & GSX_STRICT)
     return gsx_strict::tsit5_fwd_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  if (flags & GSX_FP32)
+    return tsit5_fwd_f32_dispatch(field, D, a, static_cast<hipStream_t>(stream));
   return tsit5_fwd_dispatch(field, D, a, static_cast<hipStream_t>(stream));
 }
 
@@ -157,9 +162,14 @@ int gsx_tsit5_replay(int field, int64_t n, int D, int P, const double* aux, cons
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
   ReplayArgs a{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1};
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & ~(GSX_STRICT | GSX_FP32))
+    return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if ((flags & GSX_STRICT) && (flags & GSX_FP32))
+    return set_error(GSX_E_INVALID, "GSX_STRICT and GSX_FP32 are exclusive");
   if (flags & GSX_STRICT)
     return gsx_strict::tsit5_replay_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  if (flags & GSX_FP32)
+    return tsit5_replay_f32_dispatch(field, D, a, static_cast<hipStream_t>(stream));
   return tsit5_replay_dispatch(field, D, a, static_cast<hipStream_t>(stream));
 }
 
@@ -179,9 +189,14 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux, con
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
   AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (flags & ~(GSX_STRICT | GSX_FP32))
+    return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if ((flags & GSX_STRICT) && (flags & GSX_FP32))
+    return set_error(GSX_E_INVALID, "GSX_STRICT and GSX_FP32 are exclusive");
   if (flags & GSX_STRICT)
     return gsx_strict::tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
+  if (flags & GSX_FP32)
+    return tsit5_adj_f32_dispatch(field, D, a, static_cast<hipStream_t>(stream));
   return tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
 }
 
@@ -249,6 +264,94 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, cons
   hipStream_t st = static_cast<hipStream_t>(stream);
   return (flags & GSX_STRICT) ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
                               : rb23_adj_dispatch(field, D, a, st);
+}
+
+int64_t gsx_persist_lanes(int waves_per_simd) {
+  if (waves_per_simd < 1 || waves_per_simd > 8)
+    return set_error(GSX_E_INVALID, "waves_per_simd %d outside [1, 8]", waves_per_simd);
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return set_error(GSX_E_HIP, "persist_lanes: cannot query the compute-unit count");
+  return static_cast<int64_t>(cus) * 4 * waves_per_simd * 64;
+}
+
+int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
+                          const double* y0, const double* p, double t0, double t1,
+                          const double* t1_per, double rtol, double atol, int64_t max_steps,
+                          double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                          int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
+                          int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
+                          int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
+                          int32_t* tape_start, int32_t* ckpt_start, void* queue, int flags,
+                          void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff", field);
+  if (n < 0 || n > INT32_MAX || max_steps < 0)
+    return set_error(GSX_E_INVALID, "n must be in [0, 2^31) and max_steps >= 0");
+  if (lanes <= 0 || lanes % 256 != 0)
+    return set_error(GSX_E_INVALID, "lanes %lld must be a positive multiple of 256",
+                     static_cast<long long>(lanes));
+  if (lane_cap < 1 || ckpt_cap < 1 || traj_cap < 1 || lane_cap > INT32_MAX ||
+      ckpt_cap > INT32_MAX)
+    return set_error(GSX_E_INVALID, "capacities must be in [1, 2^31)");
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_FP32) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (!queue || !tape || !ckpt || !order || !lane_count ||
+      (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status || !lane_of ||
+                 !tape_start || !ckpt_start)))
+    return set_error(GSX_E_INVALID, "tsit5_persist_fwd: null pointer");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  PersistFwdArgs pa{};
+  pa.f = FwdArgs{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+                 status, tape, lane_cap, ckpt, ckpt_interval};
+  pa.lanes = lanes;
+  pa.lane_cap = lane_cap;
+  pa.ckpt_cap = ckpt_cap;
+  pa.traj_cap = traj_cap;
+  pa.order = order;
+  pa.lane_count = lane_count;
+  pa.lane_of = lane_of;
+  pa.tape_start = tape_start;
+  pa.ckpt_start = ckpt_start;
+  pa.queue = static_cast<unsigned long long*>(queue);
+  return persist_fwd_dispatch(field, D, (flags & GSX_FP32) != 0, pa,
+                              static_cast<hipStream_t>(stream));
+}
+
+int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                              const double* p, double t0, const int32_t* n_acc, int64_t lanes,
+                              const double* tape, const double* ckpt, int32_t ckpt_interval,
+                              const int32_t* order, const int32_t* lane_count,
+                              const int32_t* tape_start, const int32_t* ckpt_start,
+                              const double* ybar, double* y0bar, double* pbar, int flags,
+                              void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (n < 0 || lanes <= 0 || lanes % 256 != 0)
+    return set_error(GSX_E_INVALID, "bad n or lanes");
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_FP32) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (!tape || !ckpt || !order || !lane_count ||
+      (n > 0 && (!p || !n_acc || !tape_start || !ckpt_start || !ybar || !y0bar || !pbar)))
+    return set_error(GSX_E_INVALID, "tsit5_persist_adjoint: null pointer");
+  PersistAdjArgs pa{};
+  pa.a = AdjArgs{n, nullptr, p, aux, t0, tape, 0, n_acc, ckpt, ckpt_interval, ybar, y0bar,
+                 pbar};
+  pa.lanes = lanes;
+  pa.order = order;
+  pa.lane_count = lane_count;
+  pa.tape_start = tape_start;
+  pa.ckpt_start = ckpt_start;
+  return persist_adj_dispatch(field, D, (flags & GSX_FP32) != 0, pa,
+                              static_cast<hipStream_t>(stream));
 }
 
 int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap, const int32_t* n_acc,

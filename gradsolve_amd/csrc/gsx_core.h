@@ -368,29 +368,43 @@ GSX_HD double wrms(const double* e, const double* y, const double* ynew, double 
 // Tsit5 stage block. Given k1 (FSAL) computes z2..z6 / k2..k6 and y5 = y + h*sum b_i k_i.
 // Stage inputs follow tsit5_step.py:88-99: z_i = y + h*(a_i1 k1 + ... ).
 // ---------------------------------------------------------------------------------------
-template <class F>
-GSX_HD void tsit5_stages(double t, const double* y, double h, const double* p,
-                         double (&k)[7][F::D], double* y5, const double* aux) {
+// With `inc` non-null the increment h*sum(b_i k_i) is also returned (rounded once), and
+// y5 = y + inc; the FP32 variant adds it to its compensated state.
+template <class F, class T>
+GSX_HD void tsit5_stages(T t, const T* y, T h, const T* p, T (&k)[7][F::D], T* y5,
+                         const double* aux, T* inc = nullptr) {
   constexpr int D = F::D;
   using namespace tab;
-  double z[D];
-  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (a21 * k[0][i]);
-  F::f(t + c2 * h, z, p, k[1], aux);
-  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (a31 * k[0][i] + a32 * k[1][i]);
-  F::f(t + c3 * h, z, p, k[2], aux);
+  const T A21 = T(a21), A31 = T(a31), A32 = T(a32), A41 = T(a41), A42 = T(a42), A43 = T(a43);
+  const T A51 = T(a51), A52 = T(a52), A53 = T(a53), A54 = T(a54);
+  const T A61 = T(a61), A62 = T(a62), A63 = T(a63), A64 = T(a64), A65 = T(a65);
+  const T B1 = T(b1), B2 = T(b2), B3 = T(b3), B4 = T(b4), B5 = T(b5), B6 = T(b6);
+  T z[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (A21 * k[0][i]);
+  F::f(t + T(c2) * h, z, p, k[1], aux);
+  GSX_UNROLL for (int i = 0; i < D; ++i) z[i] = y[i] + h * (A31 * k[0][i] + A32 * k[1][i]);
+  F::f(t + T(c3) * h, z, p, k[2], aux);
   GSX_UNROLL for (int i = 0; i < D; ++i)
-    z[i] = y[i] + h * (a41 * k[0][i] + a42 * k[1][i] + a43 * k[2][i]);
-  F::f(t + c4 * h, z, p, k[3], aux);
+    z[i] = y[i] + h * (A41 * k[0][i] + A42 * k[1][i] + A43 * k[2][i]);
+  F::f(t + T(c4) * h, z, p, k[3], aux);
   GSX_UNROLL for (int i = 0; i < D; ++i)
-    z[i] = y[i] + h * (a51 * k[0][i] + a52 * k[1][i] + a53 * k[2][i] + a54 * k[3][i]);
-  F::f(t + c5 * h, z, p, k[4], aux);
+    z[i] = y[i] + h * (A51 * k[0][i] + A52 * k[1][i] + A53 * k[2][i] + A54 * k[3][i]);
+  F::f(t + T(c5) * h, z, p, k[4], aux);
   GSX_UNROLL for (int i = 0; i < D; ++i)
-    z[i] = y[i] + h * (a61 * k[0][i] + a62 * k[1][i] + a63 * k[2][i] + a64 * k[3][i] +
-                       a65 * k[4][i]);
-  F::f(t + c6 * h, z, p, k[5], aux);
-  GSX_UNROLL for (int i = 0; i < D; ++i)
-    y5[i] = y[i] + h * (b1 * k[0][i] + b2 * k[1][i] + b3 * k[2][i] + b4 * k[3][i] +
-                        b5 * k[4][i] + b6 * k[5][i]);
+    z[i] = y[i] + h * (A61 * k[0][i] + A62 * k[1][i] + A63 * k[2][i] + A64 * k[3][i] +
+                       A65 * k[4][i]);
+  F::f(t + T(c6) * h, z, p, k[5], aux);
+  if (inc) {
+    GSX_UNROLL for (int i = 0; i < D; ++i) {
+      inc[i] = h * (B1 * k[0][i] + B2 * k[1][i] + B3 * k[2][i] + B4 * k[3][i] +
+                    B5 * k[4][i] + B6 * k[5][i]);
+      y5[i] = y[i] + inc[i];
+    }
+  } else {
+    GSX_UNROLL for (int i = 0; i < D; ++i)
+      y5[i] = y[i] + h * (B1 * k[0][i] + B2 * k[1][i] + B3 * k[2][i] + B4 * k[3][i] +
+                          B5 * k[4][i] + B6 * k[5][i]);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -433,7 +447,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
     const double h = py_min(dt, t1 - t);
-    tsit5_stages<F>(t, y, h, p, k, y5, a.aux);
+    tsit5_stages<F, double>(t, y, h, p, k, y5, a.aux);
     F::f(t + h, y5, p, k[6], a.aux);
     double e[D];
     GSX_UNROLL for (int i = 0; i < D; ++i)
@@ -491,7 +505,7 @@ GSX_HD void tsit5_advance(double t, double* y, double h, const double* p, const 
   double k[7][D];
   double y5[D];
   F::f(t, y, p, k[0], aux);
-  tsit5_stages<F>(t, y, h, p, k, y5, aux);
+  tsit5_stages<F, double>(t, y, h, p, k, y5, aux);
   GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = y5[d];
 }
 

@@ -5,6 +5,7 @@
 
 #include "../../include/gradsolve_amd.h"
 #include "gsx_core.h"
+#include "gsx_persist.h"
 
 namespace gsx {
 
@@ -29,6 +30,11 @@ int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int tsit5_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
+int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
+int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int persist_fwd_dispatch(int field, int D, bool fp32, const PersistFwdArgs& pa, hipStream_t s);
+int persist_adj_dispatch(int field, int D, bool fp32, const PersistAdjArgs& pa, hipStream_t s);
 
 }  // namespace gsx
 
@@ -39,4 +45,7 @@ int tsit5_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s)
 int rb23_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+int tsit5_fwd_f32_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
+int tsit5_replay_f32_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
+int tsit5_adj_f32_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
 }  // namespace gsx_strict

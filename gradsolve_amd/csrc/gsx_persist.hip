@@ -1,0 +1,295 @@
+// gsx_persist.hip -- persistent work-queue Tsit5 forward+record and its adjoint (gfx950).
+// Layout and scheduling are described in gsx_persist.h.
+#include <hip/hip_runtime.h>
+
+#include "gsx_internal.h"
+#include "gsx_persist.h"
+
+namespace GSX_NS {
+
+using ::gsx::check_launch;
+using ::gsx::kMaxCkptInterval;
+using ::gsx::set_error;
+
+namespace {
+
+constexpr int kPBlock = 256;
+constexpr int kPAdjBlock = 64;
+
+__global__ void queue_reset_kernel(unsigned long long* q) { *q = 0ull; }
+
+template <class F, class State>
+__global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa) {
+  const FwdArgs& a = pa.f;
+  const int64_t L = pa.lanes;
+  const int64_t lane = static_cast<int64_t>(blockIdx.x) * kPBlock + threadIdx.x;
+  const int wl = static_cast<int>(__lane_id());
+  const int32_t C = a.ckpt_interval;
+  const int64_t n = a.n;
+  State st;
+  bool active = false, done = false;
+  int64_t j = 0;
+  double t = 0.0, t1 = 0.0, dt = 0.0;
+  int32_t acc = 0, rej = 0, ck_left = C;
+  int64_t it = 0;
+  int32_t cursor = 0, ccursor = 0, tcount = 0;
+
+  auto finish = [&](int32_t status) {
+    st.write_final(a, j);
+    a.n_acc[j] = acc;
+    a.n_rej[j] = rej;
+    a.status[j] = status;
+    active = false;
+  };
+
+  while (true) {
+    // Refill: one atomic per wavefront for all of its idle lanes.
+    const bool need = !active && !done;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int cnt = __popcll(mask);
+      const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+      unsigned long long base = 0ull;
+      if (wl == leader) base = atomicAdd(pa.queue, static_cast<unsigned long long>(cnt));
+      base = __shfl(base, leader);
+      if (need) {
+        const int rank = __popcll(mask & ((1ull << wl) - 1ull));
+        const unsigned long long jj = base + static_cast<unsigned long long>(rank);
+        if (jj < static_cast<unsigned long long>(n)) {
+          j = static_cast<int64_t>(jj);
+          st.load(a, j);
+          t = a.t0;
+          t1 = a.t1_per ? a.t1_per[j] : a.t1;
+          dt = (t1 - t) / 100.0;
+          acc = 0;
+          rej = 0;
+          it = 0;
+          ck_left = C;
+          active = true;
+          if (tcount >= pa.traj_cap || ccursor >= pa.ckpt_cap) {
+            finish(kTapeFull);
+          } else {
+            pa.order[static_cast<int64_t>(tcount) * L + lane] = static_cast<int32_t>(j);
+            ++tcount;
+            pa.lane_of[j] = static_cast<int32_t>(lane);
+            pa.tape_start[j] = cursor;
+            pa.ckpt_start[j] = ccursor;
+            st.store_ckpt(a.ckpt + static_cast<int64_t>(ccursor) * ckpt_rows<F>() * L + lane, 0,
+                          L);
+            ++ccursor;
+            if (!(t < t1)) finish(kReached);
+          }
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (__ballot(!done || active) == 0ull) break;
+    if (!active) continue;
+
+    // One trial step (gradsolve/solvers/tsit5_replay.py:107-124).
+    const double h = py_min(dt, t1 - t);
+    const double err = st.trial(h, a.rtol, a.atol, a.aux);
+    ++it;
+    if (err != err) {
+      rej += static_cast<int32_t>(a.max_steps - (it - 1));
+      finish(kExhausted);
+      continue;
+    }
+    if (err <= 1.0) {
+      if (cursor >= pa.lane_cap) {
+        finish(kTapeFull);
+        continue;
+      }
+      t = t + h;
+      st.accept();
+      a.tape[static_cast<int64_t>(cursor) * L + lane] = h;
+      ++cursor;
+      ++acc;
+      if (--ck_left == 0) {
+        if (ccursor >= pa.ckpt_cap) {
+          finish(kTapeFull);
+          continue;
+        }
+        st.store_ckpt(a.ckpt + static_cast<int64_t>(ccursor) * ckpt_rows<F>() * L + lane, 0, L);
+        ++ccursor;
+        ck_left = C;
+      }
+    } else {
+      ++rej;
+    }
+    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
+    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+    if (!(t < t1)) {
+      finish(kReached);
+    } else if (it >= a.max_steps) {
+      finish(kExhausted);
+    }
+  }
+  pa.lane_count[lane] = tcount;
+}
+
+// One lane walks its trajectories backwards, one checkpoint segment per iteration.
+template <class F, int CMAX, bool FP32>
+__global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs pa) {
+  constexpr int D = F::D, P = F::P, DCK = ckpt_rows<F>();
+  __shared__ double seg_all[CMAX * DCK * kPAdjBlock];
+  const AdjArgs& a = pa.a;
+  const int64_t L = pa.lanes;
+  const int64_t lane = static_cast<int64_t>(blockIdx.x) * kPAdjBlock + threadIdx.x;
+  if (lane >= L) return;
+  double* seg = seg_all + threadIdx.x;
+  const int64_t n = a.n;
+  const int32_t C = a.ckpt_interval;
+  int32_t k = pa.lane_count[lane] - 1;
+  bool have = false;
+  int64_t j = 0;
+  int32_t na = 0, m = 0, tstart = 0, cstart = 0;
+  double lam_d[D], pb_d[P], p_d[P];
+  float lam_f[D], lam_lo[D], pb_f[P], pb_lo[P], p_f[P];
+  while (true) {
+    if (!have) {
+      if (k < 0) break;
+      j = pa.order[static_cast<int64_t>(k) * L + lane];
+      --k;
+      na = a.n_acc[j];
+      tstart = pa.tape_start[j];
+      cstart = pa.ckpt_start[j];
+      m = (na + C - 1) / C - 1;
+      GSX_UNROLL for (int d = 0; d < D; ++d) {
+        const double v = a.ybar[d * n + j];
+        lam_d[d] = v;
+        lam_f[d] = static_cast<float>(v);
+        lam_lo[d] = static_cast<float>(v - static_cast<double>(lam_f[d]));
+      }
+      GSX_UNROLL for (int q = 0; q < P; ++q) {
+        p_d[q] = a.p[q * n + j];
+        p_f[q] = static_cast<float>(p_d[q]);
+        pb_d[q] = 0.0;
+        pb_f[q] = 0.0f;
+        pb_lo[q] = 0.0f;
+      }
+      have = true;
+    }
+    if (m >= 0) {
+      const int32_t s0 = m * C;
+      const int32_t s1 = (s0 + C < na) ? s0 + C : na;
+      const double* ck = a.ckpt + static_cast<int64_t>(cstart + m) * DCK * L + lane;
+      if constexpr (FP32) {
+        float y[D], lo[D];
+        GSX_UNROLL for (int d = 0; d < D; ++d) {
+          const float* cell = reinterpret_cast<const float*>(ck + d * L);
+          y[d] = cell[0];
+          lo[d] = cell[1];
+        }
+        for (int32_t s = s0; s < s1; ++s) {
+          double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
+          GSX_UNROLL for (int d = 0; d < D; ++d) {
+            float* cell = reinterpret_cast<float*>(slot + d * kPAdjBlock);
+            cell[0] = y[d];
+            cell[1] = lo[d];
+          }
+          if (s + 1 < s1) {
+            const float hf = static_cast<float>(a.tape[static_cast<int64_t>(tstart + s) * L + lane]);
+            tsit5_advance_f32<F>(y, lo, hf, p_f, a.aux);
+          }
+        }
+        for (int32_t s = s1 - 1; s >= s0; --s) {
+          const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
+          GSX_UNROLL for (int d = 0; d < D; ++d)
+            y[d] = reinterpret_cast<const float*>(slot + d * kPAdjBlock)[0];
+          const float hf = static_cast<float>(a.tape[static_cast<int64_t>(tstart + s) * L + lane]);
+          tsit5_step_vjp_f32<F>(y, hf, p_f, lam_f, lam_lo, pb_f, pb_lo, a.aux);
+        }
+      } else {
+        double y[D];
+        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d * L];
+        for (int32_t s = s0; s < s1; ++s) {
+          double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
+          GSX_UNROLL for (int d = 0; d < D; ++d) slot[d * kPAdjBlock] = y[d];
+          if (s + 1 < s1) {
+            const double h = a.tape[static_cast<int64_t>(tstart + s) * L + lane];
+            tsit5_advance<F>(0.0, y, h, p_d, a.aux);
+          }
+        }
+        for (int32_t s = s1 - 1; s >= s0; --s) {
+          const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
+          GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = slot[d * kPAdjBlock];
+          const double h = a.tape[static_cast<int64_t>(tstart + s) * L + lane];
+          tsit5_step_vjp<F>(0.0, y, h, p_d, lam_d, pb_d, a.aux);
+        }
+      }
+      --m;
+    }
+    if (m < 0) {
+      GSX_UNROLL for (int d = 0; d < D; ++d)
+        a.y0bar[d * n + j] = FP32 ? static_cast<double>(lam_f[d]) + static_cast<double>(lam_lo[d])
+                                  : lam_d[d];
+      GSX_UNROLL for (int q = 0; q < P; ++q)
+        a.pbar[q * n + j] = FP32 ? static_cast<double>(pb_f[q]) + static_cast<double>(pb_lo[q])
+                                 : pb_d[q];
+      have = false;
+    }
+  }
+}
+
+template <class F, class State>
+int launch_pfwd(const PersistFwdArgs& pa, hipStream_t s) {
+  queue_reset_kernel<<<1, 1, 0, s>>>(pa.queue);
+  int rc = check_launch("queue_reset_kernel");
+  if (rc) return rc;
+  const unsigned grid = static_cast<unsigned>(pa.lanes / kPBlock);
+  persist_fwd_kernel<F, State><<<grid, kPBlock, 0, s>>>(pa);
+  return check_launch("persist_fwd_kernel");
+}
+
+template <class F, int CMAX, bool FP32>
+int launch_padj_c(const PersistAdjArgs& pa, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((pa.lanes + kPAdjBlock - 1) / kPAdjBlock);
+  persist_adj_kernel<F, CMAX, FP32><<<grid, kPAdjBlock, 0, s>>>(pa);
+  return check_launch("persist_adj_kernel");
+}
+
+template <class F, bool FP32>
+int launch_padj(const PersistAdjArgs& pa, hipStream_t s) {
+  const int C = pa.a.ckpt_interval;
+  if (C <= 1) return launch_padj_c<F, 1, FP32>(pa, s);
+  if (C <= 4) return launch_padj_c<F, 4, FP32>(pa, s);
+  if (C <= 8) return launch_padj_c<F, 8, FP32>(pa, s);
+  if (C <= 16) return launch_padj_c<F, 16, FP32>(pa, s);
+  if (C <= 32) return launch_padj_c<F, 32, FP32>(pa, s);
+  return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d", C,
+                   kMaxCkptInterval);
+}
+
+}  // namespace
+
+int persist_fwd_dispatch(int field, int D, bool fp32, const PersistFwdArgs& pa,
+                         hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ:
+      return fp32 ? launch_pfwd<FLorenz, StateF32<FLorenz>>(pa, s)
+                  : launch_pfwd<FLorenz, StateF64<FLorenz>>(pa, s);
+    case GSX_FIELD_LORENZ3:
+      return fp32 ? launch_pfwd<FLorenz3, StateF32<FLorenz3>>(pa, s)
+                  : launch_pfwd<FLorenz3, StateF64<FLorenz3>>(pa, s);
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "persistent tsit5 forward: field %d with D=%d is not "
+                   "built", field, D);
+}
+
+int persist_adj_dispatch(int field, int D, bool fp32, const PersistAdjArgs& pa,
+                         hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ:
+      return fp32 ? launch_padj<FLorenz, true>(pa, s) : launch_padj<FLorenz, false>(pa, s);
+    case GSX_FIELD_LORENZ3:
+      return fp32 ? launch_padj<FLorenz3, true>(pa, s) : launch_padj<FLorenz3, false>(pa, s);
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "persistent tsit5 adjoint: field %d with D=%d is not "
+                   "built", field, D);
+}
+
+}  // namespace GSX_NS

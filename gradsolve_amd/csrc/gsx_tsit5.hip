@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gsx_internal.h"
+#include "gsx_f32.h"
 
 namespace GSX_NS {
 
@@ -34,6 +35,28 @@ __global__ __launch_bounds__(kAdjBlock) void tsit5_adj_kernel(AdjArgs a) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
   if (j >= a.n) return;
   tsit5_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void tsit5_fwd_f32_kernel(FwdArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  tsit5_fwd_lane_f32<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void tsit5_replay_f32_kernel(ReplayArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  tsit5_replay_lane_f32<F>(a, j);
+}
+
+template <class F, int CMAX>
+__global__ __launch_bounds__(kAdjBlock) void tsit5_adj_f32_kernel(AdjArgs a) {
+  __shared__ double seg[CMAX * F::D * kAdjBlock];
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  if (j >= a.n) return;
+  tsit5_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock);
 }
 
 template <class F>
@@ -79,7 +102,73 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
                    kMaxCkptInterval);
 }
 
+template <class F>
+int launch_fwd_f32(const FwdArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  tsit5_fwd_f32_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("tsit5_fwd_f32_kernel");
+}
+
+template <class F>
+int launch_replay_f32(const ReplayArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  tsit5_replay_f32_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("tsit5_replay_f32_kernel");
+}
+
+template <class F, int CMAX>
+int launch_adj_f32_c(const AdjArgs& a, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((a.n + kAdjBlock - 1) / kAdjBlock);
+  tsit5_adj_f32_kernel<F, CMAX><<<grid, kAdjBlock, 0, s>>>(a);
+  return check_launch("tsit5_adj_f32_kernel");
+}
+
+template <class F>
+int launch_adj_f32(const AdjArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const int C = a.ckpt_interval;
+  if (C <= 1) return launch_adj_f32_c<F, 1>(a, s);
+  if (C <= 4) return launch_adj_f32_c<F, 4>(a, s);
+  if (C <= 8) return launch_adj_f32_c<F, 8>(a, s);
+  if (C <= 16) return launch_adj_f32_c<F, 16>(a, s);
+  if (C <= 32) return launch_adj_f32_c<F, 32>(a, s);
+  return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d", C,
+                   kMaxCkptInterval);
+}
+
 }  // namespace
+
+int tsit5_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: return launch_fwd_f32<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_fwd_f32<FLorenz3>(a, s);
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "tsit5 FP32 forward: field %d with D=%d is not built",
+                   field, D);
+}
+
+int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: return launch_replay_f32<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_replay_f32<FLorenz3>(a, s);
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "tsit5 FP32 replay: field %d with D=%d is not built",
+                   field, D);
+}
+
+int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: return launch_adj_f32<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_adj_f32<FLorenz3>(a, s);
+    default: break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "tsit5 FP32 adjoint: field %d with D=%d is not built",
+                   field, D);
+}
 
 int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
   switch (field) {

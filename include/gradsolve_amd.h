@@ -50,8 +50,12 @@ enum gsx_field {
 /* Arithmetic flags. Default (0): the kernels contract a*b + c into fused multiply-adds.
  * GSX_STRICT: contraction off, so every operation rounds once in the reference loop's
  * textual order (gradsolve/solvers/tsit5_replay.py:100-124); the recorded tape then matches
- * an IEEE-sequential host loop up to the last-ulp differences of the device pow. */
-enum gsx_flags { GSX_STRICT = 1 };
+ * an IEEE-sequential host loop up to the last-ulp differences of the device pow.
+ * GSX_FP32 (Tsit5 only): stages, the field and its adjoint in FP32 with a Kahan-compensated
+ * state and compensated adjoint accumulators; time, step size, error norm, controller and tape
+ * stay FP64; checkpoints hold the compensated pair in the bytes of an FP64 checkpoint.
+ * The two flags are exclusive. */
+enum gsx_flags { GSX_STRICT = 1, GSX_FP32 = 2 };
 
 /* Library version (major*10000 + minor*100 + patch). */
 int gsx_version(void);
@@ -137,6 +141,41 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux,
                      const double* ckpt, int32_t ckpt_interval,
                      const double* ybar, double* y0bar, double* pbar, int flags,
                      void* stream);
+
+/* ---- Persistent work-queue form (ensembles with very uneven step counts) ----------------
+ * No reference counterpart as a kernel: the reference integrates one trajectory per thread
+ * with one t1 per ensemble (gradsolve/warp/warp_ode.py:292) and leaves a wave idle until its
+ * slowest lane finishes. Here a fixed grid of `lanes` lanes (a multiple of 256; see
+ * gsx_persist_lanes) takes one trial step per lane per iteration and refills idle lanes from
+ * an atomic queue. Each lane appends its accepted step sizes to its own tape stream
+ * tape[lane_cap][lanes], its checkpoints to ckpt[ckpt_cap][ckpt_rows][lanes] (one slot at each
+ * trajectory's start and one every ckpt_interval accepted steps), and the trajectories it ran
+ * to order[traj_cap][lanes]; lane_count[lanes] counts them; per trajectory j the lane and the
+ * stream positions are lane_of[j], tape_start[j], ckpt_start[j]. `queue` is 8 bytes of device
+ * memory the call resets (with a kernel, so the sequence can be captured in a graph).
+ * A trajectory that does not fit its lane's streams gets status 3; the caller grows the
+ * capacities and calls again. flags: 0 or GSX_FP32. t1_per may be NULL. */
+int64_t gsx_persist_lanes(int waves_per_simd);
+int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
+                          const double* y0, const double* p, double t0, double t1,
+                          const double* t1_per, double rtol, double atol, int64_t max_steps,
+                          double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                          int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
+                          int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
+                          int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
+                          int32_t* tape_start, int32_t* ckpt_start, void* queue, int flags,
+                          void* stream);
+
+/* Adjoint of a persistent record: lane l walks its trajectories (order) backwards, one
+ * checkpoint segment per iteration. Same lanes, streams and ckpt_interval as the forward;
+ * call only when every status is 0. */
+int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                              const double* p, double t0, const int32_t* n_acc, int64_t lanes,
+                              const double* tape, const double* ckpt, int32_t ckpt_interval,
+                              const int32_t* order, const int32_t* lane_count,
+                              const int32_t* tape_start, const int32_t* ckpt_start,
+                              const double* ybar, double* y0bar, double* pbar, int flags,
+                              void* stream);
 
 /* Export the tape to the reference's zero-padded trajectory-major form
  * out[n][S] (gradsolve/warp/warp_replay.py:180-182): out[j*S+s] = tape[s*n+j] for

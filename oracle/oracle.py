@@ -106,7 +106,8 @@ def tableau():
     return c, a, b, e
 
 
-def _record(fn, field, y0, params, t0, t1, rtol, atol, max_steps, cap, aux, dt0, nthreads):
+def _record(fn, field, y0, params, t0, t1, rtol, atol, max_steps, cap, aux, dt0, nthreads,
+            keep_tape=True):
     y0 = _f64(np.atleast_2d(y0))
     params = _f64(np.atleast_2d(params))
     n, D = y0.shape
@@ -118,22 +119,24 @@ def _record(fn, field, y0, params, t0, t1, rtol, atol, max_steps, cap, aux, dt0,
     nacc = np.zeros(n, np.int64)
     nrej = np.zeros(n, np.int64)
     st = np.zeros(n, np.int32)
-    cap = int(min(cap, max_steps))
-    tape = np.zeros((n, cap))
+    cap = int(min(cap, max_steps)) if keep_tape else 0
+    tape = np.zeros((n, cap)) if keep_tape else None
     rc = fn(field, n, D, P, _ptr(auxa), _ptr(y0), _ptr(params), float(t0), _ptr(t1a),
             _ptr(dt0a), float(rtol), float(atol), int(max_steps), _ptr(yf),
             nacc.ctypes.data_as(_i64p), nrej.ctypes.data_as(_i64p), st.ctypes.data_as(_i32p),
             _ptr(tape), cap, _threads(nthreads))
     _check(rc, "record")
+    if not keep_tape:
+        return yf, None, nacc, nrej, st
     S = int(nacc.max()) if n else 0
     return yf, np.ascontiguousarray(tape[:, :S]), nacc, nrej, st
 
 
 def tsit5_record(field, y0, params, t0, t1, *, rtol, atol, max_steps=50000, cap=4096,
-                 aux=None, dt0=None, nthreads=None):
-    """Adaptive Tsit5 record -> (y_final[n,D], tape[n,S], n_acc, n_rej, status)."""
+                 aux=None, dt0=None, nthreads=None, keep_tape=True):
+    """Adaptive Tsit5 record -> (y_final[n,D], tape[n,S] or None, n_acc, n_rej, status)."""
     return _record(lib().gso_tsit5_record, field, y0, params, t0, t1, rtol, atol, max_steps,
-                   cap, aux, dt0, nthreads)
+                   cap, aux, dt0, nthreads, keep_tape)
 
 
 def rb23_record(field, y0, params, t0, t1, *, rtol, atol, max_steps=50000, cap=8192,

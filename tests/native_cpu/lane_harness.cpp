@@ -7,6 +7,7 @@
 
 #include "../../gradsolve_amd/csrc/gsx_core.h"
 #include "../../gradsolve_amd/csrc/gsx_rb23.h"
+#include "../../gradsolve_amd/csrc/gsx_f32.h"
 
 using namespace gsx;
 
@@ -46,6 +47,21 @@ int run_rb_adj(const AdjArgs& a) {
 }  // namespace
 
 extern "C" {
+int lh_tsit5_f32(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
+                 const double* t1_per, double rtol, double atol, int64_t max_steps,
+                 double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,
+                 int64_t cap, double* ckpt, int32_t C, const double* ybar, double* y0bar,
+                 double* pbar) {
+  if (field != 1) return -1;
+  FwdArgs a{n, y0, p, nullptr, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, cap, ckpt, C};
+  for (int64_t j = 0; j < n; ++j) tsit5_fwd_lane_f32<FLorenz>(a, j);
+  if (!ybar) return 0;
+  AdjArgs b{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  std::vector<double> seg(static_cast<size_t>(C) * 3);
+  for (int64_t j = 0; j < n; ++j) tsit5_adj_lane_f32<FLorenz>(b, j, seg.data(), 1);
+  return 0;
+}
 // The controller's clamped step-size factor clamp(0.9 * max(err, 1e-10)^e, 0.2, 5) with the
 // build's power (which: 0 Tsit5, 1 Rosenbrock23), and the raw power.
 double lh_ctrl_fac(int which, double err) {
