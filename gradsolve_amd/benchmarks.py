@@ -1,0 +1,222 @@
+"""Throughput of the engines on each workload (one JSON line per workload).
+
+    python -m gradsolve_amd.benchmarks --workload uneven|robertson|lorenz1m|all
+
+Every measurement: inputs resident in HBM, buffers preallocated, one warm-up, then the best
+of ``--repeat`` timed calls with device synchronisation on both sides (the reference's
+benchmark protocol, gradsolve benchmarks/reverse_mode_vs_diffrax.py:88-98). Reported:
+trajectories/s and accepted steps/s for the forward and for forward + gradient, and the
+algorithmic FP64 TFLOP/s of the forward + adjoint against the 78.6 TFLOP/s peak.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import autotune as AT
+from . import hip_tsit5 as H
+from . import persist as PS
+from . import workcounts as WC
+from . import workloads as W
+
+
+def _best(fn, repeat):
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(repeat):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    return best
+
+
+def _soa(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a, np.float64).T), device=dev)
+
+
+def _log(msg):
+    print(f"[benchmarks] {msg}", file=sys.stderr, flush=True)
+
+
+def uneven(args, dev):
+    """Uneven lengths: Lorenz, N = 4,194,304, rho ~ U(0, 60), t1 ~ U(0.5, 10), rtol = atol =
+    1e-6: persistent work-queue launch (autotuned, FP64 and FP32, graph-captured) against
+    the one-trajectory-per-thread launch."""
+    e = W.lorenz_uneven(args.n or (1 << 22), seed=0)
+    n = e.y0.shape[0]
+    y0, p = _soa(e.y0, dev), _soa(e.params, dev)
+    t1 = torch.as_tensor(e.t1_per, device=dev)
+    kw = dict(t0=0.0, t1=float(e.t1_per.max()), rtol=1e-6, atol=1e-6, max_steps=100000,
+              t1_per=t1)
+    out = {"workload": "lorenz_uneven", "n": n, "rtol": 1e-6}
+    for fp32 in (False, True):
+        tag = "fp32" if fp32 else "fp64"
+        ch = AT.tune(N.FIELD_LORENZ, y0, p, fp32=fp32, sample=args.tune_sample, **kw)
+        _log(f"{tag} autotune: {ch.table} -> waves {ch.waves_per_simd}, C {ch.ckpt_interval}")
+        est = PS.estimate_steps(N.FIELD_LORENZ, y0, p, t0=0.0, t1=kw["t1"], t1_per=t1,
+                                rtol=1e-6, atol=1e-6)
+        eng = PS.PersistentTsit5(N.FIELD_LORENZ, 3, 1, n, est=est,
+                                 waves_per_simd=ch.waves_per_simd,
+                                 ckpt_interval=ch.ckpt_interval, fp32=fp32, device=dev)
+        eng.run_forward(y0, p, **kw)
+        torch.cuda.synchronize()
+        acc = int(eng.n_acc.to(torch.int64).sum().item())
+        trials = acc + int(eng.n_rej.to(torch.int64).sum().item())
+
+        def fwd():
+            eng.forward(y0, p, **kw)
+
+        def both():
+            eng.forward(y0, p, **kw)
+            torch.mul(eng.y_final, 2.0, out=eng.ybar)
+            eng.adjoint(p, eng.ybar, t0=0.0)
+
+        tf = _best(fwd, args.repeat)
+        tb = _best(both, args.repeat)
+        cap = PS.CapturedStep(eng, y0, p, lambda yf, o: torch.mul(yf, 2.0, out=o), **kw)
+        tg = _best(cap.replay, args.repeat)
+        bad = int((eng.status != 0).sum().item())
+        flops = WC.LORENZ_TSIT5_TRIAL_FLOPS * trials + WC.LORENZ_TSIT5_ADJ_STEP_FLOPS * acc
+        out[f"persistent_{tag}"] = {
+            "waves_per_simd": ch.waves_per_simd, "ckpt_interval": ch.ckpt_interval,
+            "autotune_ms": ch.table, "lanes": eng.lanes, "workspace_GB": round(eng.bytes() / 1e9, 2),
+            "fwd_ms": round(tf * 1e3, 3), "fwd_adj_ms": round(tb * 1e3, 3),
+            "graph_fwd_adj_ms": round(tg * 1e3, 3),
+            "fwd_traj_per_s": round(n / tf, 1), "fwd_adj_traj_per_s": round(n / tb, 1),
+            "fwd_acc_steps_per_s": round(acc / tf, 1),
+            "fwd_adj_acc_steps_per_s": round(acc / tb, 1),
+            "fwd_adj_tflops": round(flops / tb / 1e12, 3),
+            "roofline_frac": round(flops / tb / 1e12 / WC.PEAK_FP64_TFLOPS, 4),
+            "mean_accepted": round(acc / n, 2),
+            "min_accepted": int(eng.n_acc.min().item()), "max_accepted": int(eng.n_acc.max().item()),
+            "status_nonzero": bad}
+        _log(f"{tag}: {out[f'persistent_{tag}']}")
+        del eng, cap
+        torch.cuda.empty_cache()
+    # one trajectory per thread, same arithmetic
+    est = PS.estimate_steps(N.FIELD_LORENZ, y0, p, t0=0.0, t1=kw["t1"], t1_per=t1, rtol=1e-6,
+                            atol=1e-6)
+    cap = est.max_steps * 2 + 64
+    ws = H.Workspace(N.FIELD_LORENZ, 3, 1, n, cap, out["persistent_fp64"]["ckpt_interval"], dev)
+
+    def both_plain():
+        ws.forward(y0, p, t0=0.0, t1=kw["t1"], rtol=1e-6, atol=1e-6, max_steps=100000,
+                   t1_per=t1)
+        torch.mul(ws.y_final, 2.0, out=ws.ybar)
+        ws.adjoint(p, ws.ybar, t0=0.0)
+
+    tb = _best(both_plain, args.repeat)
+    out["per_trajectory_fp64"] = {"fwd_adj_ms": round(tb * 1e3, 3),
+                                  "fwd_adj_traj_per_s": round(n / tb, 1),
+                                  "status_nonzero": int((ws.status != 0).sum().item())}
+    out["persistent_speedup"] = round(tb * 1e3 / out["persistent_fp64"]["fwd_adj_ms"], 3)
+    return out
+
+
+def robertson(args, dev):
+    """Stiff: Robertson, N = 262,144, Rosenbrock23 rtol 1e-6 atol 1e-10, t in [0, 100];
+    gradient of sum(y_final * [1, 10, 1]) w.r.t. k1..k3."""
+    e = W.robertson(args.n or (1 << 18), seed=0)
+    n = e.y0.shape[0]
+    y0, p = _soa(e.y0, dev), _soa(e.params, dev)
+    rec = H.record(N.FIELD_ROBERTSON, y0, p, t0=0.0, t1=100.0, rtol=1e-6, atol=1e-10,
+                   max_steps=50000, tape_cap=1024, ckpt_interval=8, entry="gsx_rb23_fwd")
+    torch.cuda.synchronize()
+    S = rec.max_steps_taken()
+    acc = int(rec.n_acc.to(torch.int64).sum().item())
+    out = {"workload": "robertson_rb23", "n": n, "mean_accepted": round(acc / n, 2),
+           "max_accepted": S}
+    wvec = torch.tensor([1.0, 10.0, 1.0], dtype=torch.float64, device=dev)[:, None]
+    best = None
+    for C in (4, 8, 16):
+        ws = H.Workspace(N.FIELD_ROBERTSON, 3, 3, n, S + 16, C, dev)
+        ybar = (wvec * torch.ones((3, n), dtype=torch.float64, device=dev)).contiguous()
+
+        def fwd():
+            N.call("gsx_rb23_fwd", N.FIELD_ROBERTSON, n, 3, 3, None, y0.data_ptr(),
+                   p.data_ptr(), 0.0, 100.0, None, 1e-6, 1e-10, 50000, ws.y_final.data_ptr(),
+                   ws.n_acc.data_ptr(), ws.n_rej.data_ptr(), ws.status.data_ptr(),
+                   ws.tape.data_ptr(), ws.cap, ws.ckpt.data_ptr(), C, 0,
+                   H._stream_handle(None))
+
+        def both():
+            fwd()
+            N.call("gsx_rb23_adjoint", N.FIELD_ROBERTSON, n, 3, 3, None, None, p.data_ptr(),
+                   0.0, ws.tape.data_ptr(), ws.cap, ws.n_acc.data_ptr(), ws.ckpt.data_ptr(), C,
+                   ybar.data_ptr(), ws.y0bar.data_ptr(), ws.pbar.data_ptr(), 0,
+                   H._stream_handle(None))
+
+        tf = _best(fwd, args.repeat)
+        tb = _best(both, args.repeat)
+        out[f"C{C}"] = {"fwd_ms": round(tf * 1e3, 3), "fwd_adj_ms": round(tb * 1e3, 3)}
+        if best is None or tb < best[0]:
+            best = (tb, tf, C)
+        del ws
+    tb, tf, C = best
+    out.update({"ckpt_interval": C, "fwd_traj_per_s": round(n / tf, 1),
+                "fwd_adj_traj_per_s": round(n / tb, 1),
+                "fwd_acc_steps_per_s": round(acc / tf, 1),
+                "fwd_adj_acc_steps_per_s": round(acc / tb, 1)})
+    return out
+
+
+def lorenz1m(args, dev):
+    """The headline workload's forward alone and forward + gradient (FP64 and FP32)."""
+    e = W.lorenz_1m(seed=0, n=args.n or (1 << 20))
+    n = e.y0.shape[0]
+    y0, p = _soa(e.y0, dev), _soa(e.params, dev)
+    kw = dict(t0=0.0, t1=1.0, rtol=1e-8, atol=1e-8, max_steps=100000)
+    rec = H.record(N.FIELD_LORENZ, y0, p, tape_cap=256, ckpt_interval=8, **kw)
+    torch.cuda.synchronize()
+    cap = rec.max_steps_taken() + 16
+    acc = int(rec.n_acc.to(torch.int64).sum().item())
+    out = {"workload": "lorenz_1m", "n": n, "mean_accepted": round(acc / n, 3)}
+    for tag, flags in (("fp64", 0), ("fp32", N.FP32), ("fp64_strict", N.STRICT)):
+        ws = H.Workspace(N.FIELD_LORENZ, 3, 1, n, cap, 4, dev)
+        ws.flags = flags
+
+        def fwd_only():
+            ws.forward(y0, p, record=False, **kw)
+
+        def both():
+            ws.forward(y0, p, **kw)
+            torch.mul(ws.y_final, 2.0, out=ws.ybar)
+            ws.adjoint(p, ws.ybar, t0=0.0)
+
+        tf = _best(fwd_only, args.repeat)
+        tb = _best(both, args.repeat)
+        out[tag] = {"fwd_ms": round(tf * 1e3, 3), "fwd_adj_ms": round(tb * 1e3, 3),
+                    "fwd_traj_per_s": round(n / tf, 1), "fwd_adj_traj_per_s": round(n / tb, 1),
+                    "fwd_adj_acc_steps_per_s": round(acc / tb, 1)}
+        del ws
+    return out
+
+
+WORKLOADS = {"uneven": uneven, "robertson": robertson, "lorenz1m": lorenz1m}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="all", choices=["all", *WORKLOADS])
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--repeat", type=int, default=5)
+    ap.add_argument("--tune-sample", type=int, default=1 << 18)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    names = list(WORKLOADS) if args.workload == "all" else [args.workload]
+    for name in names:
+        _log(f"workload {name}")
+        print(json.dumps(WORKLOADS[name](args, dev)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
