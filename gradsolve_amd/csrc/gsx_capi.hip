@@ -297,6 +297,9 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
   if (lane_cap < 1 || ckpt_cap < 1 || traj_cap < 1 || lane_cap > INT32_MAX ||
       ckpt_cap > INT32_MAX)
     return set_error(GSX_E_INVALID, "capacities must be in [1, 2^31)");
+  if (lane_cap % gsx::kTapeBlock != 0)
+    return set_error(GSX_E_INVALID, "lane_cap %lld must be a multiple of %d",
+                     static_cast<long long>(lane_cap), gsx::kTapeBlock);
   if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);

@@ -3,28 +3,47 @@
 // parameter ranges). Compiled in GSX_NS.
 //
 // Forward. A fixed grid of L lanes loops until the queue is empty. Every iteration each busy
-// lane takes one trial step of its trajectory; lanes that have finished (or not started) pull
-// the next trajectory indices from a global counter, one atomic per wavefront for all its idle
-// lanes. So a wavefront's lanes stay busy even when their trajectories' lengths differ by 20x.
+// lane takes one trial step of its trajectory; lanes that have finished (or not started) take
+// the next trajectory indices from their wavefront's claimed range of the queue. A wavefront
+// claims kQueueChunk indices with one atomic on the global counter when its range runs out,
+// so the counter sees one atomic per kQueueChunk trajectories (one device-scope atomic per
+// refill put all 3072 wavefronts in line behind one address). A wavefront's lanes stay busy
+// even when their trajectories' lengths differ by 20x.
 //
-// Record layout. Each lane appends to its own streams, interleaved across lanes so that the
-// lanes of a wavefront write neighbouring addresses:
-//   tape  [lane_cap][L]            accepted step sizes, appended per lane
-//   ckpt  [ckpt_cap][DCK][L]       trajectory start states and every C-th accepted state
+// Record layout. Each lane appends to its own streams:
+//   tape  [lane_cap/16][L][16]     accepted step sizes; a lane's steps 16k..16k+15 are one
+//                                  128-byte line, so lanes whose positions have drifted apart
+//                                  still write and read whole lines
+//   ckpt  [ckpt_cap][L][DCK]       trajectory start states and every C-th accepted state;
+//                                  one checkpoint's rows are contiguous
 //   order [traj_cap][L]            the trajectories a lane processed, in order
 // and per trajectory j: lane[j], tape_start[j], ckpt_start[j] (positions in that lane's
 // streams), n_acc[j], n_rej[j], status[j].
 //
 // Adjoint. Lane l walks its own trajectory list backwards, one checkpoint segment (C accepted
 // steps: re-integrate into LDS, then reverse) per iteration, so per-iteration work is uniform
-// across lanes and the reads of its streams stay interleaved like the forward's writes.
+// across lanes.
 #pragma once
 
 #include "gsx_core.h"
 #include "gsx_f32.h"
 
+namespace gsx {
+
+constexpr int kTapeBlock = 16;                 // tape entries of one lane per 128-byte line
+constexpr unsigned long long kQueueChunk = 64;  // queue indices a wavefront claims per atomic
+
+// Position pos of lane `lane`'s tape stream in the [lane_cap/16][L][16] array.
+GSX_HD int64_t persist_tape_index(int64_t pos, int64_t lane, int64_t L) {
+  return ((pos >> 4) * L + lane) * kTapeBlock + (pos & (kTapeBlock - 1));
+}
+
+}  // namespace gsx
+
 namespace GSX_NS {
 
+using ::gsx::kQueueChunk;
+using ::gsx::persist_tape_index;
 using ::gsx::PersistAdjArgs;
 using ::gsx::PersistFwdArgs;
 

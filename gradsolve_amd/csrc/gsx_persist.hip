@@ -20,12 +20,13 @@ __global__ void queue_reset_kernel(unsigned long long* q) { *q = 0ull; }
 
 template <class F, class State>
 __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa) {
+  constexpr int DCK = ckpt_rows<F>();
   const FwdArgs& a = pa.f;
   const int64_t L = pa.lanes;
   const int64_t lane = static_cast<int64_t>(blockIdx.x) * kPBlock + threadIdx.x;
   const int wl = static_cast<int>(__lane_id());
   const int32_t C = a.ckpt_interval;
-  const int64_t n = a.n;
+  const unsigned long long n = static_cast<unsigned long long>(a.n);
   State st;
   bool active = false, done = false;
   int64_t j = 0;
@@ -33,6 +34,10 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
   int32_t acc = 0, rej = 0, ck_left = C;
   int64_t it = 0;
   int32_t cursor = 0, ccursor = 0, tcount = 0;
+  // The wavefront's claimed queue range [cbase, cend) and whether the queue has run dry
+  // (wavefront-uniform).
+  unsigned long long cbase = 0ull, cend = 0ull;
+  bool drained = false;
 
   auto finish = [&](int32_t status) {
     st.write_final(a, j);
@@ -43,19 +48,34 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
   };
 
   while (true) {
-    // Refill: one atomic per wavefront for all of its idle lanes.
+    // Refill the idle lanes from the claimed range; claim kQueueChunk more indices (one
+    // atomic for the wavefront) when the range holds fewer than the idle lanes.
     const bool need = !active && !done;
     const unsigned long long mask = __ballot(need);
     if (mask) {
-      const int cnt = __popcll(mask);
-      const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-      unsigned long long base = 0ull;
-      if (wl == leader) base = atomicAdd(pa.queue, static_cast<unsigned long long>(cnt));
-      base = __shfl(base, leader);
+      const unsigned long long cnt = static_cast<unsigned long long>(__popcll(mask));
+      const unsigned long long avail = cend - cbase;
+      unsigned long long nb = 0ull, ne = 0ull;
+      if (avail < cnt && !drained) {
+        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+        if (wl == leader) nb = atomicAdd(pa.queue, kQueueChunk);
+        nb = __shfl(nb, leader);
+        ne = nb < n ? (nb + kQueueChunk < n ? nb + kQueueChunk : n) : nb;
+        drained = (ne == nb);
+      }
       if (need) {
-        const int rank = __popcll(mask & ((1ull << wl) - 1ull));
-        const unsigned long long jj = base + static_cast<unsigned long long>(rank);
-        if (jj < static_cast<unsigned long long>(n)) {
+        const unsigned long long r =
+            static_cast<unsigned long long>(__popcll(mask & ((1ull << wl) - 1ull)));
+        bool got = true;
+        unsigned long long jj = 0ull;
+        if (r < avail) {
+          jj = cbase + r;
+        } else if (r - avail < ne - nb) {
+          jj = nb + (r - avail);
+        } else {
+          got = false;
+        }
+        if (got) {
           j = static_cast<int64_t>(jj);
           st.load(a, j);
           t = a.t0;
@@ -74,14 +94,22 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
             pa.lane_of[j] = static_cast<int32_t>(lane);
             pa.tape_start[j] = cursor;
             pa.ckpt_start[j] = ccursor;
-            st.store_ckpt(a.ckpt + static_cast<int64_t>(ccursor) * ckpt_rows<F>() * L + lane, 0,
-                          L);
+            st.store_ckpt(a.ckpt + (static_cast<int64_t>(ccursor) * L + lane) * DCK, 0, 1);
             ++ccursor;
             if (!(t < t1)) finish(kReached);
           }
         } else {
           done = true;
         }
+      }
+      if (cnt <= avail) {
+        cbase += cnt;
+      } else if (ne > nb) {
+        const unsigned long long used = cnt - avail;
+        cbase = nb + (used < ne - nb ? used : ne - nb);
+        cend = ne;
+      } else {
+        cbase = cend;
       }
     }
     if (__ballot(!done || active) == 0ull) break;
@@ -103,7 +131,7 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
       }
       t = t + h;
       st.accept();
-      a.tape[static_cast<int64_t>(cursor) * L + lane] = h;
+      a.tape[persist_tape_index(cursor, lane, L)] = h;
       ++cursor;
       ++acc;
       if (--ck_left == 0) {
@@ -111,7 +139,7 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
           finish(kTapeFull);
           continue;
         }
-        st.store_ckpt(a.ckpt + static_cast<int64_t>(ccursor) * ckpt_rows<F>() * L + lane, 0, L);
+        st.store_ckpt(a.ckpt + (static_cast<int64_t>(ccursor) * L + lane) * DCK, 0, 1);
         ++ccursor;
         ck_left = C;
       }
@@ -174,11 +202,11 @@ __global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs 
     if (m >= 0) {
       const int32_t s0 = m * C;
       const int32_t s1 = (s0 + C < na) ? s0 + C : na;
-      const double* ck = a.ckpt + static_cast<int64_t>(cstart + m) * DCK * L + lane;
+      const double* ck = a.ckpt + (static_cast<int64_t>(cstart + m) * L + lane) * DCK;
       if constexpr (FP32) {
         float y[D], lo[D];
         GSX_UNROLL for (int d = 0; d < D; ++d) {
-          const float* cell = reinterpret_cast<const float*>(ck + d * L);
+          const float* cell = reinterpret_cast<const float*>(ck + d);
           y[d] = cell[0];
           lo[d] = cell[1];
         }
@@ -190,7 +218,8 @@ __global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs 
             cell[1] = lo[d];
           }
           if (s + 1 < s1) {
-            const float hf = static_cast<float>(a.tape[static_cast<int64_t>(tstart + s) * L + lane]);
+            const float hf =
+                static_cast<float>(a.tape[persist_tape_index(tstart + s, lane, L)]);
             tsit5_advance_f32<F>(y, lo, hf, p_f, a.aux);
           }
         }
@@ -198,24 +227,25 @@ __global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs 
           const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d)
             y[d] = reinterpret_cast<const float*>(slot + d * kPAdjBlock)[0];
-          const float hf = static_cast<float>(a.tape[static_cast<int64_t>(tstart + s) * L + lane]);
+          const float hf =
+              static_cast<float>(a.tape[persist_tape_index(tstart + s, lane, L)]);
           tsit5_step_vjp_f32<F>(y, hf, p_f, lam_f, lam_lo, pb_f, pb_lo, a.aux);
         }
       } else {
         double y[D];
-        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d * L];
+        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d];
         for (int32_t s = s0; s < s1; ++s) {
           double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d) slot[d * kPAdjBlock] = y[d];
           if (s + 1 < s1) {
-            const double h = a.tape[static_cast<int64_t>(tstart + s) * L + lane];
+            const double h = a.tape[persist_tape_index(tstart + s, lane, L)];
             tsit5_advance<F>(0.0, y, h, p_d, a.aux);
           }
         }
         for (int32_t s = s1 - 1; s >= s0; --s) {
           const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = slot[d * kPAdjBlock];
-          const double h = a.tape[static_cast<int64_t>(tstart + s) * L + lane];
+          const double h = a.tape[persist_tape_index(tstart + s, lane, L)];
           tsit5_step_vjp<F>(0.0, y, h, p_d, lam_d, pb_d, a.aux);
         }
       }

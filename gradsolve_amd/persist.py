@@ -2,6 +2,9 @@
 ``gsx_tsit5_persist_adjoint``): buffer sizing, the forward+adjoint pair, and capture of the
 pair in a HIP graph.
 
+Layout. tape[lane_cap/16][lanes][16] (a lane's steps 16k..16k+15 contiguous),
+ckpt[ckpt_cap][lanes][rows], order[traj_cap][lanes].
+
 Capacities. Each lane appends to its own tape, checkpoint and trajectory-list streams, so
 the per-lane capacities follow from the expected number of accepted steps per lane:
 ``lane_cap = margin * total_steps / lanes + longest_trajectory`` (the longest trajectory
@@ -22,6 +25,9 @@ import torch
 
 from . import _native as N
 from . import hip_tsit5 as H
+
+
+TAPE_BLOCK = 16  # accepted steps of one lane stored contiguously in the tape
 
 
 def _ptr(t):
@@ -70,11 +76,13 @@ class PersistentTsit5:
 
     def _alloc(self, margin):
         L, n, dev = self.lanes, self.n, self.device
+        # the tape stores a lane's steps in blocks of TAPE_BLOCK (one 128-byte line)
+        self.lane_cap = -(-self.lane_cap // TAPE_BLOCK) * TAPE_BLOCK
         self.ckpt_cap = self.lane_cap // self.C + self.traj_cap + 8
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
-        self.tape = torch.empty((self.lane_cap, L), **f64)
-        self.ckpt = torch.empty((self.ckpt_cap, self.rows, L), **f64)
+        self.tape = torch.empty((self.lane_cap // TAPE_BLOCK, L, TAPE_BLOCK), **f64)
+        self.ckpt = torch.empty((self.ckpt_cap, L, self.rows), **f64)
         self.order = torch.empty((self.traj_cap, L), **i32)
         self.lane_count = torch.empty(L, **i32)
         self.lane_of = torch.empty(n, **i32)
@@ -137,7 +145,8 @@ class PersistentTsit5:
         pos = self.tape_start.to(torch.int64)[:, None] + s
         lane = self.lane_of.to(torch.int64)[:, None]
         valid = s < self.n_acc.to(torch.int64)[:, None]
-        flat = (pos.clamp(max=self.lane_cap - 1) * self.lanes + lane)
+        pos = pos.clamp(max=self.lane_cap - 1)
+        flat = ((pos // TAPE_BLOCK) * self.lanes + lane) * TAPE_BLOCK + pos % TAPE_BLOCK
         out = self.tape.reshape(-1)[flat.reshape(-1)].reshape(self.n, S)
         return torch.where(valid, out, torch.zeros((), dtype=out.dtype, device=out.device))
 

@@ -147,10 +147,12 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux,
  * with one t1 per ensemble (gradsolve/warp/warp_ode.py:292) and leaves a wave idle until its
  * slowest lane finishes. Here a fixed grid of `lanes` lanes (a multiple of 256; see
  * gsx_persist_lanes) takes one trial step per lane per iteration and refills idle lanes from
- * an atomic queue. Each lane appends its accepted step sizes to its own tape stream
- * tape[lane_cap][lanes], its checkpoints to ckpt[ckpt_cap][ckpt_rows][lanes] (one slot at each
- * trajectory's start and one every ckpt_interval accepted steps), and the trajectories it ran
- * to order[traj_cap][lanes]; lane_count[lanes] counts them; per trajectory j the lane and the
+ * an atomic queue (each wavefront claims 64 indices per atomic). Each lane appends its
+ * accepted step sizes to its own tape stream, stored as tape[lane_cap/16][lanes][16] (lane l's
+ * position s at ((s/16)*lanes + l)*16 + s%16; lane_cap a multiple of 16), its checkpoints to
+ * ckpt[ckpt_cap][lanes][ckpt_rows] (one slot at each trajectory's start and one every
+ * ckpt_interval accepted steps), and the trajectories it ran to order[traj_cap][lanes];
+ * lane_count[lanes] counts them; per trajectory j the lane and the
  * stream positions are lane_of[j], tape_start[j], ckpt_start[j]. `queue` is 8 bytes of device
  * memory the call resets (with a kernel, so the sequence can be captured in a graph).
  * A trajectory that does not fit its lane's streams gets status 3; the caller grows the
