@@ -191,7 +191,8 @@ def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
                                  device=device, precision=precision)
     elif precision == "float32":
         from . import fp32
-        res = fp32.solve(b, problem, y0, params, rtol=rtol, atol=atol, device=device)
+        res = fp32.solve(b, problem, y0, params, rtol=rtol, atol=atol, device=device,
+                         name=name)
     else:
         res = ENGINE_REGISTRY[name].solve(problem, y0, params, rtol=rtol, atol=atol,
                                           device=device, strict=strict)
@@ -249,7 +250,8 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
     elif precision == "float32":
         from . import fp32
         closure = fp32.make_closure(name, b, problem, y0, params, wrt=wrt_t, rtol=rtol,
-                                    atol=atol, device=dev, aux=aux)
+                                    atol=atol, device=dev, aux=aux,
+                                    ckpt_interval=ckpt_interval)
     elif name == D.HIP_REPLAY:
         from .closure import make_tsit5_closure
         closure = make_tsit5_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,

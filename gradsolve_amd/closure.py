@@ -32,13 +32,14 @@ class _Lane:
 
     FWD, REPLAY, ADJ = "gsx_tsit5_fwd", "gsx_tsit5_replay", "gsx_tsit5_adjoint"
 
-    def __init__(self, binding, aux, strict):
-        self.b, self.aux, self.strict = binding, aux, strict
+    def __init__(self, binding, aux, strict, fp32=False):
+        self.b, self.aux, self.strict, self.fp32 = binding, aux, strict, fp32
+        self.flags = H.flags_for(strict, fp32)
 
     def record(self, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, ckpt_interval):
         return H.record(self.b.field_id, y0_soa, p_soa, t0=t0, t1=t1, rtol=rtol, atol=atol,
                         max_steps=max_steps, aux=self.aux, ckpt_interval=ckpt_interval,
-                        strict=self.strict, entry=self.FWD)
+                        strict=self.strict, entry=self.FWD, fp32=self.fp32)
 
     def replay(self, rec, y0_soa, p_soa):
         y_final = torch.empty_like(rec.y_final)
@@ -46,7 +47,7 @@ class _Lane:
         N.call(self.REPLAY, rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux),
                y0_soa.data_ptr(), p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(),
                rec.n_acc.data_ptr(), y_final.data_ptr(), ckpt.data_ptr(), rec.ckpt_interval,
-               N.STRICT if self.strict else 0, H._stream_handle(None))
+               self.flags, H._stream_handle(None))
         return y_final, ckpt
 
     def adjoint(self, rec, ckpt, p_soa, ybar_soa):
@@ -55,7 +56,7 @@ class _Lane:
         N.call(self.ADJ, rec.field, rec.n, rec.D, rec.P, H._ptr(self.aux), None,
                p_soa.data_ptr(), rec.t0, rec.tape.data_ptr(), rec.cap, rec.n_acc.data_ptr(),
                ckpt.data_ptr(), rec.ckpt_interval, ybar_soa.data_ptr(), y0bar.data_ptr(),
-               pbar.data_ptr(), 0, H._stream_handle(None))
+               pbar.data_ptr(), self.flags, H._stream_handle(None))
         return y0bar, pbar
 
 
@@ -186,6 +187,16 @@ def make_tsit5_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
                        ckpt_interval=None, strict=False, aux=None,
                        max_steps=100000) -> ReplayClosure:
     lane = _Tsit5Lane(binding, aux, strict)
+    return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                         device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
+
+
+def make_tsit5_f32_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
+                           ckpt_interval=None, aux=None, max_steps=100000) -> ReplayClosure:
+    """The FP32 lane: FP32 stages with a compensated state, FP64 time, error norm and
+    controller; the tape stays FP64 and the adjoint runs the FP32 step VJP with compensated
+    accumulators."""
+    lane = _Tsit5Lane(binding, aux, False, fp32=True)
     return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                          device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
 

@@ -38,6 +38,14 @@ def _ptr(t: Optional[torch.Tensor]):
     return None if t is None else t.data_ptr()
 
 
+def flags_for(strict: bool = False, fp32: bool = False) -> int:
+    """Kernel-variant flags: contraction-off FP64 (strict) or FP32 stages with compensated
+    state (fp32); the two are exclusive."""
+    if strict and fp32:
+        raise ValueError("strict and fp32 are exclusive kernel variants")
+    return N.STRICT if strict else (N.FP32 if fp32 else 0)
+
+
 def default_ckpt_interval(expected_steps: int) -> int:
     """sqrt(S)-spaced checkpoints: the interval that balances checkpoint storage against
     the re-integration a segment needs."""
@@ -92,7 +100,8 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
            t1_per: Optional[torch.Tensor] = None, aux: Optional[torch.Tensor] = None,
            tape_cap: Optional[int] = None, ckpt_interval: Optional[int] = None,
            stream: Optional[torch.cuda.Stream] = None, grow: bool = True,
-           strict: bool = False, entry: str = "gsx_tsit5_fwd") -> Tsit5Record:
+           strict: bool = False, entry: str = "gsx_tsit5_fwd",
+           fp32: bool = False) -> Tsit5Record:
     """Adaptive forward with the accepted-h record and checkpoints (device-resident).
 
     ``y0_soa [D, n]`` and ``p_soa [P, n]`` are float64 device tensors. Returns a record whose
@@ -117,7 +126,7 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
         N.call(entry, field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa),
                float(t0), float(t1), _ptr(t1_per), float(rtol), float(atol), int(max_steps),
                _ptr(y_final), _ptr(n_acc), _ptr(n_rej), _ptr(status), _ptr(tape), cap,
-               _ptr(ckpt), C, N.STRICT if strict else 0, _stream_handle(stream))
+               _ptr(ckpt), C, flags_for(strict, fp32), _stream_handle(stream))
         rec = Tsit5Record(field, n, D, P, float(t0), float(t1), y0_soa, p_soa, t1_per, aux,
                           y_final, n_acc, n_rej, status, tape, cap, ckpt, C)
         if not grow or cap >= max_steps or n == 0:
@@ -131,7 +140,7 @@ def forward_only(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: f
                  t1: float, rtol: float, atol: float, max_steps: int = 4096,
                  t1_per: Optional[torch.Tensor] = None, aux: Optional[torch.Tensor] = None,
                  stream: Optional[torch.cuda.Stream] = None, strict: bool = False,
-                 entry: str = "gsx_tsit5_fwd"):
+                 entry: str = "gsx_tsit5_fwd", fp32: bool = False):
     """The forward-only lane (no tape, no checkpoints): (y_final, n_acc, n_rej, status)."""
     D, n = y0_soa.shape
     P = p_soa.shape[0]
@@ -143,12 +152,13 @@ def forward_only(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: f
     N.call(entry, field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa), float(t0),
            float(t1), _ptr(t1_per), float(rtol), float(atol), int(max_steps), _ptr(y_final),
            _ptr(n_acc), _ptr(n_rej), _ptr(status), None, 0, None, 1,
-           N.STRICT if strict else 0, _stream_handle(stream))
+           flags_for(strict, fp32), _stream_handle(stream))
     return y_final, n_acc, n_rej, status
 
 
 def adjoint(rec: Tsit5Record, ybar_soa: torch.Tensor,
-            stream: Optional[torch.cuda.Stream] = None, strict: bool = False):
+            stream: Optional[torch.cuda.Stream] = None, strict: bool = False,
+            fp32: bool = False):
     """dL/dy0 [D, n] and dL/dp [P, n] for the cotangent ``ybar_soa = dL/dy_final [D, n]``."""
     ybar_soa = ybar_soa.to(dtype=torch.float64).contiguous()
     y0bar = torch.empty((rec.D, rec.n), dtype=torch.float64, device=rec.y0.device)
@@ -156,7 +166,7 @@ def adjoint(rec: Tsit5Record, ybar_soa: torch.Tensor,
     N.call("gsx_tsit5_adjoint", rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux), _ptr(rec.y0),
            _ptr(rec.p), rec.t0, _ptr(rec.tape), rec.cap, _ptr(rec.n_acc), _ptr(rec.ckpt),
            rec.ckpt_interval, _ptr(ybar_soa), _ptr(y0bar), _ptr(pbar),
-           N.STRICT if strict else 0, _stream_handle(stream))
+           flags_for(strict, fp32), _stream_handle(stream))
     return y0bar, pbar
 
 

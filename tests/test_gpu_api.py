@@ -124,3 +124,40 @@ def test_engine_aliases_route_to_native(gpu):
     cl = gs.grad_closure(P.DiffEqGPULorenz(), e.y0, e.params, engine="warp_replay",
                          rtol=1e-8, atol=1e-8)
     assert cl.route.actual == "hip_replay"
+
+
+def test_float32_closure_against_float64_replay_of_its_tape(gpu):
+    """precision='float32' (tests/test_f32_replay.py:88-94 of the reference): the FP32 lane's
+    gradient against the FP64 replay adjoint of the same FP32-recorded tape, rel-L2 <= 1e-3
+    and cosine >= 1 - 1e-6; the primal within 1e-4. Lorenz rho = 28, n = 2048, rtol 1e-6,
+    atol 1e-9, y0 = [1,0,0] + 0.01 N(0,1) (seed 0)."""
+    n = 2048
+    rng = np.random.default_rng(0)
+    y0 = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1)) + 0.01 * rng.standard_normal((n, 3))
+    params = np.full((n, 1), 28.0)
+    prob = P.DiffEqGPULorenz()
+    cl = gs.grad_closure(prob, y0, params, wrt="params", rtol=1e-6, atol=1e-9,
+                         precision="float32", device="cuda")
+    p = torch.tensor(params, device="cuda", requires_grad=True)
+    yf = cl(p)
+    (yf * yf).sum().backward()
+    g32 = p.grad.cpu().numpy()[:, 0]
+    tape = cl.dts_padded()
+    oyf = O.tsit5_replay(O.LORENZ, y0, params, 0.0, tape)
+    _, ogp = O.tsit5_adjoint(O.LORENZ, y0, params, 0.0, tape, 2.0 * oyf)
+    g64 = ogp[:, 0]
+    rel_l2 = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    cos = g32 @ g64 / (np.linalg.norm(g32) * np.linalg.norm(g64))
+    prim = np.max(np.abs(yf.detach().cpu().numpy() - oyf) / np.abs(oyf).max(axis=1,
+                                                                          keepdims=True))
+    print(f"float32 lane vs float64 replay of its tape: grad rel-L2 {rel_l2:.2e}, cosine "
+          f"1-{1 - cos:.1e}, primal {prim:.2e}", flush=True)
+    assert rel_l2 <= 1e-3 and cos >= 1 - 1e-6 and prim <= 1e-4
+    res = gs.solve(prob, y0, params, rtol=1e-6, atol=1e-9, precision="float32", device="cuda")
+    assert np.max(np.abs(res.y_final - oyf) / np.abs(oyf).max(axis=1, keepdims=True)) < 1e-4
+
+
+def test_float32_rejected_on_stiff_lane(gpu):
+    e = W.robertson(4, seed=0)
+    with pytest.raises(ValueError, match="float32"):
+        gs.grad_closure(P.Robertson(), e.y0, e.params, precision="float32", device="cuda")
