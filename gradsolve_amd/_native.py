@@ -61,6 +61,13 @@ SIGNATURES = {
                                    _i64, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_tsit5_persist_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                        _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "gsx_kuramoto_fwd": (_i, [_i64, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64, _vp, _i64, _vp,
+                              _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "gsx_kuramoto_replay": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsx_kuramoto_saveat": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                 _vp]),
+    "gsx_kuramoto_adjoint": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                  _vp, _vp, _vp, _vp, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
 }

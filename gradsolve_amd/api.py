@@ -122,10 +122,39 @@ def _solve_rb23(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult
     return R.solve(problem, y0, params, rtol=rtol, atol=atol, device=device)
 
 
+def _is_kuramoto(b) -> bool:
+    from . import _native as N
+    return b is not None and b.field_id == N.FIELD_KURAMOTO
+
+
+def _supports_kuramoto(problem) -> bool:
+    return _is_kuramoto(F.binding_for(problem)) and not problem.is_stiff
+
+
+def _solve_kuramoto(problem, y0, params, *, rtol, atol, device, max_steps=100000,
+                    **kw) -> SolveResult:
+    import torch
+
+    from . import kuramoto as KU
+    b = F.binding_for(problem)
+    dev = _device(device)
+    y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
+    om_soa = torch.as_tensor(np.asarray(params, np.float64), device=dev).t().contiguous()
+    K = torch.as_tensor(b.aux_of(problem), device=dev)
+    yf, na, nr, st = KU.forward_only(K, y0_soa, om_soa, t0=problem.t0, t1=problem.t1,
+                                     rtol=rtol, atol=atol, max_steps=max_steps)
+    y_final = yf.t().cpu().numpy().copy()
+    y_final[st.cpu().numpy() != 0] = np.nan
+    return SolveResult(y_final=y_final, accepted_steps=na.cpu().numpy().astype(np.int64),
+                       rejected_steps=nr.cpu().numpy().astype(np.int64), solver=D.HIP_KURAMOTO)
+
+
 ENGINE_REGISTRY = {
     D.HIP_TSIT5: EngineSpec(D.HIP_TSIT5, _supports_tsit5, _solve_tsit5, None),
     D.HIP_REPLAY: EngineSpec(D.HIP_REPLAY, _supports_tsit5, _solve_tsit5, "tsit5"),
     D.HIP_ROSENBROCK23: EngineSpec(D.HIP_ROSENBROCK23, _supports_rb23, _solve_rb23, "rb23"),
+    D.HIP_KURAMOTO: EngineSpec(D.HIP_KURAMOTO, _supports_kuramoto, _solve_kuramoto,
+                               "kuramoto"),
 }
 
 
@@ -148,6 +177,9 @@ def _resolve(engine, problem, need_grad, batch_n, accuracy_target):
     else:
         raise ValueError(f"unknown engine {engine!r}; known: {sorted(ENGINE_REGISTRY)} + "
                          f"aliases {sorted(D.ALIASES)}")
+    if name in (D.HIP_TSIT5, D.HIP_REPLAY) and _supports_kuramoto(problem):
+        name = D.HIP_KURAMOTO
+        reasons.append("kuramoto-field: wavefront-cooperative MFMA engine")
     if name in _NOT_BUILT:
         raise ValueError(
             f"problem {problem.name!r} (dim {problem.dim}, stiff={problem.is_stiff}) routes to "
@@ -232,6 +264,9 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
     if name == D.HIP_TSIT5:
         name = D.HIP_REPLAY
     b = F.binding_for(problem)
+    if precision == "float32":
+        from . import fp32
+        fp32.reject_incapable(name)
     if not _native_ok(b, problem, params):
         raise ValueError(f"params shape {tuple(params.shape)} does not match the field "
                          f"{b.key!r} ({b.n_params} parameters)")
@@ -247,6 +282,10 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         closure = dense.make_saveat_closure(name, b, problem, y0, params, saveat, wrt=wrt_t,
                                             rtol=rtol, atol=atol, device=dev, aux=aux,
                                             precision=precision)
+    elif name == D.HIP_KURAMOTO:
+        from .closure import KuramotoClosure
+        closure = KuramotoClosure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
+                                  device=dev, aux=aux)
     elif precision == "float32":
         from . import fp32
         closure = fp32.make_closure(name, b, problem, y0, params, wrt=wrt_t, rtol=rtol,

@@ -207,3 +207,79 @@ def make_rb23_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
     lane = _Rb23Lane(binding, aux, strict)
     return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                          device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
+
+
+class _KuramotoFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, om_user, state):
+        from . import kuramoto as KU
+        rec = state.rec
+        y0_soa = z.detach().t().contiguous()
+        om_soa = om_user.detach().t().contiguous()
+        if state.is_recording_point(y0_soa, om_soa):
+            y_final, states = rec.y_final, rec.states
+        else:
+            y_final, states = KU.replay(rec, y0_soa, om_soa)
+        ctx.state, ctx.states, ctx.om_soa = state, states, om_soa
+        if state.dense:
+            return KU.saveat(rec, y0_soa, om_soa, y_final, states)
+        return y_final.t().contiguous()
+
+    @staticmethod
+    def backward(ctx, g):
+        from . import kuramoto as KU
+        st = ctx.state
+        g = g.to(torch.float64)
+        if st.dense:
+            y0bar, ombar = KU.adjoint(st.rec, ysbar=g.contiguous(), om_soa=ctx.om_soa,
+                                      states=ctx.states)
+        else:
+            y0bar, ombar = KU.adjoint(st.rec, yfbar=g.t().contiguous(), om_soa=ctx.om_soa,
+                                      states=ctx.states)
+        g_z = y0bar.t().contiguous() if ctx.needs_input_grad[0] else None
+        g_p = ombar.t().contiguous() if ctx.needs_input_grad[1] else None
+        return g_z, g_p, None
+
+
+class KuramotoClosure(ReplayClosure):
+    """Record-once closure of the Kuramoto engine: ``closure(omega)``, ``closure(theta0)`` or
+    ``closure(theta0, omega)`` returns the final state ``(n, 32)``, or with save times the
+    series ``(n, k, 32)`` (gradsolve/api.py:750-753); the backward is the native adjoint with
+    the save-time cotangents."""
+
+    def __init__(self, binding, problem, y0, params, *, wrt, rtol, atol, device, aux,
+                 ts=None, max_steps=100000):
+        from . import kuramoto as KU
+        self.binding, self.problem, self.wrt, self.device = binding, problem, wrt, device
+        y0_t = _as_device(y0, device)
+        p_t = _as_device(params, device)
+        self._y0, self._params = y0_t, p_t
+        self._y0_soa = y0_t.t().contiguous()
+        self._p_soa = p_t.t().contiguous()
+        self.dense = ts is not None
+        self.ts = None if ts is None else torch.as_tensor(ts, dtype=torch.float64,
+                                                          device=device)
+        self.rec = KU.record(aux, self._y0_soa, self._p_soa, t0=problem.t0, t1=problem.t1,
+                             rtol=rtol, atol=atol, ts=self.ts, max_steps=max_steps)
+        st = self.rec.status
+        bad = torch.nonzero(st != 0).flatten()
+        if bad.numel():
+            i = int(bad[0].item())
+            raise RuntimeError(
+                f"record: {bad.numel()} trajectories (first: {i}) exhausted max_steps="
+                f"{max_steps} before t1={problem.t1} (status {int(st[i].item())}); the "
+                "recorded dt sequence is incomplete and a frozen replay of it would be "
+                "silently wrong. Increase max_steps, loosen the tol, or shorten the horizon.")
+
+    def __call__(self, *args):
+        if self.wrt == ("params",):
+            (p,), z = args, None
+        elif self.wrt == ("y0",):
+            (z,), p = args, None
+        else:
+            z, p = args
+        numpy_in = not any(isinstance(a, torch.Tensor) for a in args)
+        zt = self._y0 if z is None else self._coerce(z)
+        pt = self._params if p is None else self._coerce(p)
+        out = _KuramotoFn.apply(zt, pt, self)
+        return out.detach().cpu().numpy() if numpy_in else out

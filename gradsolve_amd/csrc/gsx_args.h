@@ -81,4 +81,69 @@ struct PersistAdjArgs {
   const int32_t* ckpt_start;
 };
 
+
+// Kuramoto ensemble (gsx_kuramoto.hip describes the layouts). D = P = 32.
+struct KuraFwdArgs {
+  int64_t n;
+  const double* K;       // [32][32] row-major, shared
+  const double* y0;      // [32][n]
+  const double* omega;   // [32][n]
+  double t0, t1, rtol, atol;
+  int64_t max_steps;
+  const double* ts;      // [nsave] sorted save times, or nullptr
+  int64_t nsave;
+  double* y_final;       // [32][n]
+  int32_t* n_acc;
+  int32_t* n_rej;
+  int32_t* status;
+  double* tape;          // [tape_cap][n] or nullptr (forward only)
+  int64_t tape_cap;
+  double* states;        // [tape_cap][n][32] or nullptr
+  int32_t* save_step;    // [nsave][n] or nullptr
+  double* save_t;        // [nsave][n]
+};
+
+struct KuraReplayArgs {
+  int64_t n;
+  const double* K;
+  const double* y0;
+  const double* omega;
+  const double* tape;
+  const int32_t* n_acc;
+  double* y_final;
+  double* states;
+};
+
+struct KuraSaveatArgs {
+  int64_t n;
+  const double* K;
+  const double* y0;
+  const double* omega;
+  const double* y_final;
+  const double* states;
+  const double* ts;
+  int64_t nsave;
+  const int32_t* save_step;
+  const double* save_t;
+  double* ys;            // [n][nsave][32]
+};
+
+struct KuraAdjArgs {
+  int64_t n;
+  const double* K;
+  const double* omega;
+  const double* tape;
+  const int32_t* n_acc;
+  const double* states;
+  const double* ts;
+  int64_t nsave;
+  const int32_t* save_step;
+  const double* save_t;
+  const double* ysbar;   // [n][nsave][32] or nullptr
+  const double* yfbar;   // [32][n] or nullptr
+  double* mu;            // [n][nsave][32] scratch (when ysbar is given)
+  double* y0bar;         // [32][n]
+  double* omega_bar;     // [32][n]
+};
+
 }  // namespace gsx

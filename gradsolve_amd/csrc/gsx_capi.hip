@@ -357,6 +357,126 @@ int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* 
                               static_cast<hipStream_t>(stream));
 }
 
+// ---------------------------------------------------------------------------------------
+// Kuramoto ensemble (D = P = 32, shared K): gsx_kuramoto.hip.
+// ---------------------------------------------------------------------------------------
+static int kura_check_d(int D, const char* what) {
+  if (D != 32)
+    return set_error(GSX_E_UNSUPPORTED, "%s: the Kuramoto kernels are built for D = 32, got %d",
+                     what, D);
+  return GSX_OK;
+}
+
+int gsx_kuramoto_fwd(int64_t n, int D, const double* K, const double* y0, const double* omega,
+                     double t0, double t1, double rtol, double atol, int64_t max_steps,
+                     const double* ts, int64_t nsave, double* y_final, int32_t* n_acc,
+                     int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
+                     double* states, int32_t* save_step, double* save_t, void* stream) {
+  int rc = kura_check_d(D, "kuramoto_fwd");
+  if (rc) return rc;
+  if (n < 0 || n > INT32_MAX || max_steps < 0 || nsave < 0)
+    return set_error(GSX_E_INVALID, "kuramoto_fwd: n in [0, 2^31), max_steps >= 0 and "
+                     "nsave >= 0 required");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  if (tape && (tape_cap < 1 || tape_cap > INT32_MAX))
+    return set_error(GSX_E_INVALID, "kuramoto_fwd: tape_cap must be in [1, 2^31)");
+  if (!tape && (states || nsave > 0))
+    return set_error(GSX_E_INVALID, "kuramoto_fwd: states and save times need the tape");
+  if (n > 0 && (!K || !y0 || !omega || !y_final || !n_acc || !n_rej || !status))
+    return set_error(GSX_E_INVALID, "kuramoto_fwd: null pointer");
+  if (nsave > 0 && (!ts || !save_step || !save_t || !states))
+    return set_error(GSX_E_INVALID, "kuramoto_fwd: save times need ts, save_step, save_t "
+                     "and states");
+  if (n == 0) return GSX_OK;
+  KuraFwdArgs a{};
+  a.n = n;
+  a.K = K;
+  a.y0 = y0;
+  a.omega = omega;
+  a.t0 = t0;
+  a.t1 = t1;
+  a.rtol = rtol;
+  a.atol = atol;
+  a.max_steps = max_steps;
+  a.ts = nsave > 0 ? ts : nullptr;
+  a.nsave = nsave;
+  a.y_final = y_final;
+  a.n_acc = n_acc;
+  a.n_rej = n_rej;
+  a.status = status;
+  a.tape = tape;
+  a.tape_cap = tape ? tape_cap : 0;
+  a.states = states;
+  a.save_step = nsave > 0 ? save_step : nullptr;
+  a.save_t = nsave > 0 ? save_t : nullptr;
+  return kura_fwd_launch(a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_kuramoto_replay(int64_t n, int D, const double* K, const double* y0,
+                        const double* omega, const double* tape, const int32_t* n_acc,
+                        double* y_final, double* states, void* stream) {
+  int rc = kura_check_d(D, "kuramoto_replay");
+  if (rc) return rc;
+  if (n < 0 || n > INT32_MAX) return set_error(GSX_E_INVALID, "kuramoto_replay: bad n");
+  if (n > 0 && (!K || !y0 || !omega || !tape || !n_acc || !y_final))
+    return set_error(GSX_E_INVALID, "kuramoto_replay: null pointer");
+  if (n == 0) return GSX_OK;
+  KuraReplayArgs a{n, K, y0, omega, tape, n_acc, y_final, states};
+  return kura_replay_launch(a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_kuramoto_saveat(int64_t n, int D, const double* K, const double* y0,
+                        const double* omega, const double* y_final, const double* states,
+                        const double* ts, int64_t nsave, const int32_t* save_step,
+                        const double* save_t, double* ys, void* stream) {
+  int rc = kura_check_d(D, "kuramoto_saveat");
+  if (rc) return rc;
+  if (n < 0 || n > INT32_MAX || nsave < 0)
+    return set_error(GSX_E_INVALID, "kuramoto_saveat: bad n or nsave");
+  if (n > 0 && nsave > 0 &&
+      (!K || !y0 || !omega || !y_final || !states || !ts || !save_step || !save_t || !ys))
+    return set_error(GSX_E_INVALID, "kuramoto_saveat: null pointer");
+  if (n == 0 || nsave == 0) return GSX_OK;
+  KuraSaveatArgs a{n, K, y0, omega, y_final, states, ts, nsave, save_step, save_t, ys};
+  return kura_saveat_launch(a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
+                         const double* tape, const int32_t* n_acc, const double* states,
+                         const double* ts, int64_t nsave, const int32_t* save_step,
+                         const double* save_t, const double* ysbar, const double* yfbar,
+                         double* mu, double* y0bar, double* omega_bar, void* stream) {
+  int rc = kura_check_d(D, "kuramoto_adjoint");
+  if (rc) return rc;
+  if (n < 0 || n > INT32_MAX || nsave < 0)
+    return set_error(GSX_E_INVALID, "kuramoto_adjoint: bad n or nsave");
+  if (n > 0 && (!K || !omega || !tape || !n_acc || !states || !y0bar || !omega_bar))
+    return set_error(GSX_E_INVALID, "kuramoto_adjoint: null pointer");
+  if (ysbar && (nsave < 1 || !ts || !save_step || !save_t || !mu))
+    return set_error(GSX_E_INVALID, "kuramoto_adjoint: save cotangents need ts, save_step, "
+                     "save_t and the mu buffer");
+  if (ysbar && mu == ysbar)
+    return set_error(GSX_E_INVALID, "kuramoto_adjoint: mu must not alias ysbar");
+  if (n == 0) return GSX_OK;
+  KuraAdjArgs a{};
+  a.n = n;
+  a.K = K;
+  a.omega = omega;
+  a.tape = tape;
+  a.n_acc = n_acc;
+  a.states = states;
+  a.ts = ts;
+  a.nsave = ysbar ? nsave : 0;
+  a.save_step = save_step;
+  a.save_t = save_t;
+  a.ysbar = ysbar;
+  a.yfbar = yfbar;
+  a.mu = mu;
+  a.y0bar = y0bar;
+  a.omega_bar = omega_bar;
+  return kura_adj_launch(a, static_cast<hipStream_t>(stream));
+}
+
 int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap, const int32_t* n_acc,
                        int64_t S, double* out, void* stream) {
   if (n < 0 || S < 0 || S > tape_cap)

@@ -165,6 +165,54 @@ GSX_HD double norm_div(double a, double b) {
 #endif
 }
 
+// sin(x) and cos(x) together (the Kuramoto field's phases). For |x| < 2^20: k = rint(x * 2/pi)
+// and r = x - k*pi/2 by a three-part split of pi/2 (hi + mid + lo, each product inside a fused
+// multiply-add; the first subtraction is exact because k*hi and x are multiples of 2^-52 in
+// that range), then the Taylor series of sin through r^17 and of cos through r^16 on
+// |r| <= pi/4 (the first omitted terms are below 2^-60 relative), and the quadrant k mod 4
+// swaps and negates them. Within 2 ulp of the correctly rounded values. Larger |x| and
+// non-finite x go to the math library. Every rounding step is an explicit operation, so host
+// and device builds return the same bits.
+GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
+  if (!(std::fabs(x) < 1048576.0)) {
+    *s_out = std::sin(x);
+    *c_out = std::cos(x);
+    return;
+  }
+  constexpr double kTwoOverPi = 0.6366197723675814;
+  constexpr double kPio2Hi = 1.5707963267948966;
+  constexpr double kPio2Mid = 6.123233995736766e-17;
+  constexpr double kPio2Lo = -1.4973849048591698e-33;
+  const double k = std::rint(x * kTwoOverPi);
+  double r = std::fma(-k, kPio2Hi, x);
+  r = std::fma(-k, kPio2Mid, r);
+  r = std::fma(-k, kPio2Lo, r);
+  const double z = r * r;
+  double ps = 1.0 / 355687428096000.0;  // 1/17!
+  ps = std::fma(z, ps, -1.0 / 1307674368000.0);
+  ps = std::fma(z, ps, 1.0 / 6227020800.0);
+  ps = std::fma(z, ps, -1.0 / 39916800.0);
+  ps = std::fma(z, ps, 1.0 / 362880.0);
+  ps = std::fma(z, ps, -1.0 / 5040.0);
+  ps = std::fma(z, ps, 1.0 / 120.0);
+  ps = std::fma(z, ps, -1.0 / 6.0);
+  const double sr = std::fma(r * z, ps, r);
+  double pc = 1.0 / 20922789888000.0;  // 1/16!
+  pc = std::fma(z, pc, -1.0 / 87178291200.0);
+  pc = std::fma(z, pc, 1.0 / 479001600.0);
+  pc = std::fma(z, pc, -1.0 / 3628800.0);
+  pc = std::fma(z, pc, 1.0 / 40320.0);
+  pc = std::fma(z, pc, -1.0 / 720.0);
+  pc = std::fma(z, pc, 1.0 / 24.0);
+  pc = std::fma(z, pc, -0.5);
+  const double cr = std::fma(z, pc, 1.0);
+  const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
+  const double sa = (q & 1) ? cr : sr;
+  const double ca = (q & 1) ? sr : cr;
+  *s_out = (q & 2) ? -sa : sa;
+  *c_out = ((q + 1) & 2) ? -ca : ca;
+}
+
 // Python's builtin min/max argument semantics (min(a, b) is a unless b < a), which the
 // reference controller uses (gradsolve/solvers/tsit5_replay.py:108,124). They propagate a NaN
 // in the first operand, unlike fmin/fmax.

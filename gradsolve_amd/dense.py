@@ -1,0 +1,67 @@
+"""``saveat``: dense output on a recorded mesh (gradsolve/solvers/dense.py, gradsolve/api.py:
+438-535 for ``solve`` and :750-753 for ``grad_closure``).
+
+Semantics as the reference's re-step path: the forward records, for every save time, the
+accepted step that brackets it (``t_s < ts[q] <= t_s + h_s``); the saved state is one Tsit5
+step of size ``ts[q] - t_s`` from that step's start state. Times at or before ``t0`` give
+``y0``; times at or beyond the final time give ``y_final`` (so the last save equals
+``y_final`` bit for bit when ``ts[-1] = t1``). The gradient differentiates the re-steps as well
+as the mesh replay, with the step sizes as data.
+
+This build serves saveat on the Kuramoto engine (the problem the reference's dense lane is
+benchmarked on); a request on another field raises ``ValueError`` naming what is built.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import dispatch as D
+from . import kuramoto as KU
+from .base import SolveResult
+
+
+def _check(name, b, precision):
+    if precision != "float64":
+        raise ValueError(f"precision={precision!r} with saveat: the dense lane is built in "
+                         "float64 only; use precision='float64'")
+    if b.field_id != N.FIELD_KURAMOTO or name != D.HIP_KURAMOTO:
+        raise ValueError(
+            f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
+            "dense output on the Kuramoto engine (hip_kuramoto). Drop saveat for a final-"
+            "state solve or gradient.")
+
+
+def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
+                 precision="float64", max_steps=100000) -> SolveResult:
+    from .api import _device
+    _check(name, b, precision)
+    ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
+    dev = _device(device)
+    y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
+    om_soa = torch.as_tensor(np.asarray(params, np.float64), device=dev).t().contiguous()
+    K = torch.as_tensor(b.aux_of(problem), device=dev)
+    ts_d = torch.as_tensor(ts, device=dev)
+    rec = KU.record(K, y0_soa, om_soa, t0=problem.t0, t1=problem.t1, rtol=rtol, atol=atol,
+                    ts=ts_d, max_steps=max_steps)
+    ys = KU.saveat(rec)
+    status = rec.status.cpu().numpy()
+    y_final = rec.y_final.t().cpu().numpy().copy()
+    y_saved = ys.cpu().numpy()
+    y_final[status != 0] = np.nan
+    y_saved[status != 0] = np.nan
+    return SolveResult(y_final=y_final, accepted_steps=rec.n_acc.cpu().numpy().astype(np.int64),
+                       rejected_steps=rec.n_rej.cpu().numpy().astype(np.int64), solver=name,
+                       y_saved=y_saved, ts_saved=ts)
+
+
+def make_saveat_closure(name, b, problem, y0, params, saveat, *, wrt, rtol, atol, device,
+                        aux=None, precision="float64", max_steps=100000):
+    from .closure import KuramotoClosure
+    _check(name, b, precision)
+    ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
+    if aux is None:
+        aux = torch.as_tensor(b.aux_of(problem), device=device)
+    return KuramotoClosure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                           device=device, aux=aux, ts=ts, max_steps=max_steps)

@@ -7,7 +7,10 @@ The codomain names this build's native engines:
                          adjoint kernel (replaces ``warp_replay``, the record-and-replay
                          gradient lane);
 * ``hip_rosenbrock23`` -- fused Rosenbrock23 forward + record and its adjoint (replaces
-                         ``warp_rosenbrock``/``cuda_rosenbrock23``).
+                         ``warp_rosenbrock``/``cuda_rosenbrock23``);
+* ``hip_kuramoto``     -- Kuramoto-32 forward + record, saveat and adjoint, 16 trajectories
+                         per wavefront with the coupling on MFMA (the reference runs this
+                         problem on its general ``tsit5_replay`` lane).
 
 High-dimensional problems route to the reference's general engines (``tsit5_replay``,
 ``rodas5p_replay``, ``diffrax``), which are not part of this build: the API reports them
@@ -29,6 +32,10 @@ HIP_ROSENBROCK23_ENABLED = True
 HIP_TSIT5 = "hip_tsit5"
 HIP_REPLAY = "hip_replay"
 HIP_ROSENBROCK23 = "hip_rosenbrock23"
+#: The wavefront-cooperative Kuramoto engine (D = 32, coupling on MFMA). Not a router
+#: output: the API substitutes it for the router's Tsit5 engines when the problem's native
+#: field is the Kuramoto kernel, and records why on ``route.reason``.
+HIP_KURAMOTO = "hip_kuramoto"
 TSIT5_REPLAY = "tsit5_replay"
 RODAS5P_REPLAY = "rodas5p_replay"
 DIFFRAX = "diffrax"

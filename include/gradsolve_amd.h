@@ -179,6 +179,54 @@ int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* 
                               const double* ybar, double* y0bar, double* pbar, int flags,
                               void* stream);
 
+/* Kuramoto ensemble, D = P = 32: f = omega + (1/32)((K sin y) cos y - (K cos y) sin y) with one
+ * coupling matrix K[32*32] (row-major) shared by the ensemble; the coupling sums run on FP64
+ * MFMA, 16 trajectories per wavefront. In the reference this problem has no fused kernel
+ * (register_jax_field rejects the closed-over K, gradsolve/warp/jax_field.py:417-421), so
+ * grad_closure(..., saveat=ts) takes the general lane: record_tsit5_jax
+ * (gradsolve/solvers/tsit5_replay.py:127-189) then replay_solve_saveat_jax
+ * (gradsolve/warp/warp_replay.py:58-75 -> gradsolve/solvers/dense.py:106-163, 318-335) and
+ * jax.grad. These four calls replace that lane.
+ *
+ * Layouts: y0, omega, y_final, yfbar, y0bar, omega_bar [32][n]; tape [tape_cap][n];
+ * states [tape_cap][n][32] (the state at the start of accepted step s); save_step, save_t
+ * [nsave][n]; ys, ysbar, mu [n][nsave][32]. ts[nsave] is sorted non-decreasing (validated by
+ * the caller as validate_saveat does, dense.py:81-103).
+ *
+ * gsx_kuramoto_fwd: adaptive Tsit5 forward (as gsx_tsit5_fwd; t0 and t1 shared). With tape
+ * non-NULL it records the accepted steps and, when states is non-NULL, every accepted step's
+ * start state; with nsave > 0 it records for each save time q the step s bracketing it
+ * (t_s < ts[q] <= t_s + h_s) in save_step[q][j] and t_s in save_t[q][j], -1 when
+ * ts[q] <= t0 (the save is y0) and -2 when ts[q] >= the final time (the save is y_final).
+ * status as gsx_tsit5_fwd (3: the tape is full; grow tape_cap and call again). */
+int gsx_kuramoto_fwd(int64_t n, int D, const double* K, const double* y0, const double* omega,
+                     double t0, double t1, double rtol, double atol, int64_t max_steps,
+                     const double* ts, int64_t nsave, double* y_final, int32_t* n_acc,
+                     int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
+                     double* states, int32_t* save_step, double* save_t, void* stream);
+
+/* Frozen-mesh replay from (y0, omega) over a recorded tape: y_final and, when non-NULL, the
+ * states (the save brackets depend on the mesh only and stay valid). */
+int gsx_kuramoto_replay(int64_t n, int D, const double* K, const double* y0,
+                        const double* omega, const double* tape, const int32_t* n_acc,
+                        double* y_final, double* states, void* stream);
+
+/* Saved states ys[j][q]: one Tsit5 step of size ts[q] - save_t from the bracketing step's
+ * start state, y0 for save_step -1, y_final for -2 (dense.py:106-163). */
+int gsx_kuramoto_saveat(int64_t n, int D, const double* K, const double* y0,
+                        const double* omega, const double* y_final, const double* states,
+                        const double* ts, int64_t nsave, const int32_t* save_step,
+                        const double* save_t, double* ys, void* stream);
+
+/* Adjoint of (y_final, ys) with respect to (y0, omega) for cotangents yfbar (may be NULL) and
+ * ysbar (may be NULL): y0bar, omega_bar [32][n]. mu [n][nsave][32] is scratch the caller
+ * provides when ysbar is given (it must not alias ysbar). */
+int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
+                         const double* tape, const int32_t* n_acc, const double* states,
+                         const double* ts, int64_t nsave, const int32_t* save_step,
+                         const double* save_t, const double* ysbar, const double* yfbar,
+                         double* mu, double* y0bar, double* omega_bar, void* stream);
+
 /* Export the tape to the reference's zero-padded trajectory-major form
  * out[n][S] (gradsolve/warp/warp_replay.py:180-182): out[j*S+s] = tape[s*n+j] for
  * s < n_acc[j], else 0. */

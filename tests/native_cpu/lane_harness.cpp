@@ -69,6 +69,10 @@ double lh_ctrl_fac(int which, double err) {
   const double pw = which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
   return py_min(py_max(kSafety * pw, kFacMin), kFacMax);
 }
+void lh_sincos(int64_t n, const double* x, double* s, double* c) {
+  for (int64_t i = 0; i < n; ++i) gsx_sincos(x[i], s + i, c + i);
+}
+
 double lh_ctrl_pow(int which, double x) {
   return which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
 }

@@ -117,3 +117,33 @@ def test_controller_power_fast_variant_accuracy():
         fac = np.array([F.lh_ctrl_fac(which, x) for x in xx])
         rf = np.minimum(np.maximum(0.9 * np.maximum(xx, 1e-10) ** e, 0.2), 5.0)
         assert np.max(np.abs(fac - rf) / np.spacing(rf)) <= 2.0
+
+
+def test_sincos_within_two_ulp():
+    """The device sincos (three-part pi/2 reduction, Taylor polynomials; host build of the
+    same source) against the C library's sin and cos: within 2 ulp over |x| < 2^20, exact
+    library values beyond."""
+    import math
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
+    F = ctypes.CDLL(os.path.join(HERE, "native_cpu", "liblane_harness_fast.so"))
+    F.lh_sincos.restype = None
+    F.lh_sincos.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(11)
+    k = rng.integers(-600000, 600000, 20000).astype(np.float64)
+    near = k * (np.pi / 2) + rng.uniform(-1e-6, 1e-6, k.size)  # close to multiples of pi/2
+    xs = np.concatenate([rng.uniform(-40.0, 40.0, 200000), rng.uniform(-1e6, 1e6, 50000),
+                         near, rng.uniform(-1e-8, 1e-8, 1000), [0.0, -0.0, 1e-300],
+                         rng.uniform(2.0 ** 20, 1e9, 1000)])
+    xs = np.ascontiguousarray(xs)
+    s = np.empty_like(xs)
+    c = np.empty_like(xs)
+    F.lh_sincos(xs.size, xs.ctypes.data, s.ctypes.data, c.ctypes.data)
+    rs = np.array([math.sin(v) for v in xs])
+    rc = np.array([math.cos(v) for v in xs])
+    us = np.abs(s - rs) / np.spacing(np.abs(rs))
+    uc = np.abs(c - rc) / np.spacing(np.abs(rc))
+    print(f"sincos: max ulp sin {us.max():.2f}, cos {uc.max():.2f}; exact "
+          f"{np.mean(us == 0):.4f} / {np.mean(uc == 0):.4f}")
+    assert us.max() <= 2.0 and uc.max() <= 2.0
+    big = np.abs(xs) >= 2.0 ** 20
+    assert np.array_equal(s[big], rs[big]) and np.array_equal(c[big], rc[big])
