@@ -68,6 +68,10 @@ SIGNATURES = {
                                  _vp]),
     "gsx_kuramoto_adjoint": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                   _vp, _vp, _vp, _vp, _vp]),
+    "gsx_plain_tsit5_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "gsx_plain_tsit5_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
 }

@@ -146,4 +146,25 @@ struct KuraAdjArgs {
   double* omega_bar;     // [32][n]
 };
 
+
+// The naive reverse sweep the benchmark compares against (gsx_plain.hip).
+struct PlainArgs {
+  int64_t n;
+  const double* aux;
+  const double* y0;
+  const double* p;
+  double t0, t1, rtol, atol;
+  int64_t max_steps;
+  double* y_final;
+  int32_t* n_acc;
+  int32_t* n_rej;
+  int32_t* status;
+  double* tape;          // [cap][n]
+  double* stages;        // [cap][6][D][n]
+  int64_t cap;
+  const double* ybar;    // adjoint: [D][n]
+  double* y0bar;
+  double* pbar;
+};
+
 }  // namespace gsx

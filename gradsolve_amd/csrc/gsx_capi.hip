@@ -477,6 +477,66 @@ int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
   return kura_adj_launch(a, static_cast<hipStream_t>(stream));
 }
 
+// ---------------------------------------------------------------------------------------
+// The naive reverse sweep (benchmark plain arm): gsx_plain.hip.
+// ---------------------------------------------------------------------------------------
+int gsx_plain_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                        const double* p, double t0, double t1, double rtol, double atol,
+                        int64_t max_steps, double* y_final, int32_t* n_acc, int32_t* n_rej,
+                        int32_t* status, double* tape, double* stages, int64_t cap,
+                        void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff", field);
+  if (n < 0 || n > INT32_MAX || max_steps < 0 || cap < 1 || cap > INT32_MAX)
+    return set_error(GSX_E_INVALID, "plain_tsit5_fwd: bad n, max_steps or cap");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status || !tape || !stages))
+    return set_error(GSX_E_INVALID, "plain_tsit5_fwd: null pointer");
+  PlainArgs a{};
+  a.n = n;
+  a.aux = aux;
+  a.y0 = y0;
+  a.p = p;
+  a.t0 = t0;
+  a.t1 = t1;
+  a.rtol = rtol;
+  a.atol = atol;
+  a.max_steps = max_steps;
+  a.y_final = y_final;
+  a.n_acc = n_acc;
+  a.n_rej = n_rej;
+  a.status = status;
+  a.tape = tape;
+  a.stages = stages;
+  a.cap = cap;
+  return plain_dispatch(field, false, a, static_cast<hipStream_t>(stream));
+}
+
+int gsx_plain_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                            const double* p, const double* tape, const double* stages,
+                            const int32_t* n_acc, const double* ybar, double* y0bar,
+                            double* pbar, void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (n < 0 || n > INT32_MAX) return set_error(GSX_E_INVALID, "plain_tsit5_adjoint: bad n");
+  if (n > 0 && (!p || !tape || !stages || !n_acc || !ybar || !y0bar || !pbar))
+    return set_error(GSX_E_INVALID, "plain_tsit5_adjoint: null pointer");
+  PlainArgs a{};
+  a.n = n;
+  a.aux = aux;
+  a.p = p;
+  a.tape = const_cast<double*>(tape);
+  a.stages = const_cast<double*>(stages);
+  a.n_acc = const_cast<int32_t*>(n_acc);
+  a.ybar = ybar;
+  a.y0bar = y0bar;
+  a.pbar = pbar;
+  return plain_dispatch(field, true, a, static_cast<hipStream_t>(stream));
+}
+
 int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap, const int32_t* n_acc,
                        int64_t S, double* out, void* stream) {
   if (n < 0 || S < 0 || S > tape_cap)

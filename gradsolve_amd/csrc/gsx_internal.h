@@ -39,6 +39,7 @@ int kura_fwd_launch(const KuraFwdArgs& a, hipStream_t s);
 int kura_replay_launch(const KuraReplayArgs& a, hipStream_t s);
 int kura_saveat_launch(const KuraSaveatArgs& a, hipStream_t s);
 int kura_adj_launch(const KuraAdjArgs& a, hipStream_t s);
+int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 
 }  // namespace gsx
 

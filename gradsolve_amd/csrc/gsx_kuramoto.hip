@@ -40,6 +40,9 @@ namespace gsx {
 namespace {
 
 typedef double kd4 __attribute__((ext_vector_type(4)));
+// LDS-qualified double: accesses through it compile to ds_read/ds_write (a generic pointer to
+// LDS becomes FLAT instructions when the access is volatile).
+typedef __attribute__((address_space(3))) double lds_double;
 
 constexpr int kKD = 32;            // phases per trajectory
 constexpr int kSlots = 8;          // phases per lane
@@ -70,7 +73,7 @@ __device__ __forceinline__ void coupling(const double* tab, const double* s, con
                                          double* a, double* b) {
   const int l = static_cast<int>(__lane_id());
   // volatile keeps the operand reads next to their MFMAs instead of 32 hoisted registers
-  const volatile double* tv = tab;
+  const volatile lds_double* tv = (const volatile lds_double*)tab;
   kd4 as0 = {0.0, 0.0, 0.0, 0.0}, as1 = as0, ac0 = as0, ac1 = as0;
   GSX_UNROLL for (int kk = 0; kk < 8; ++kk) {
     const double k0 = tv[kk * 64 + l], k1 = tv[(8 + kk) * 64 + l];
@@ -91,8 +94,6 @@ __device__ __forceinline__ void coupling(const double* tab, const double* s, con
 // lane of the wavefront must call it together (the MFMA spans the wavefront). `aux` is the
 // block's operand tables.
 struct FKuramotoWave {
-  static constexpr int D = kSlots;  // phases of the trajectory this lane holds
-
   __device__ static void f(double, const double* y, const double* om, double* dy,
                            const double* op) {
     double s[kSlots], c[kSlots], a[kSlots], b[kSlots];
@@ -179,16 +180,96 @@ __device__ __forceinline__ LaneId lane_id(int64_t n) {
   return id;
 }
 
+// Stage derivatives live in LDS slots (this thread's column of a [slots][8][kKBlock] block
+// array) and the stage loops are not unrolled, so each kernel holds one copy of the field (its
+// 8 sincos and 32 MFMAs) instead of one per stage: unrolled, the kernels exceeded 110 KB of
+// code, far past the instruction cache. Accesses are volatile: each thread reads back only its
+// own column, and the compiler would otherwise forward the stores into registers.
+struct StageSlots {
+  volatile lds_double* base;
+  __device__ __forceinline__ explicit StageSlots(double* column)
+      : base((volatile lds_double*)column) {}
+  __device__ __forceinline__ volatile lds_double& at(int s, int m) const {
+    return base[(s * kSlots + m) * kKBlock];
+  }
+};
+
+// Row st of the stage weights: rows 1..5 the inputs of k2..k6 (a_{st+1, 1..st}), row 6 the
+// solution weights b_1..b_6, row 7 the error weights e_1..e_7 (b_i - bhat_i).
+__device__ __forceinline__ double stage_w(int row, int r) {
+  using namespace tab;
+  constexpr double W[8][7] = {{0, 0, 0, 0, 0, 0, 0},
+                              {a21, 0, 0, 0, 0, 0, 0},
+                              {a31, a32, 0, 0, 0, 0, 0},
+                              {a41, a42, a43, 0, 0, 0, 0},
+                              {a51, a52, a53, a54, 0, 0, 0},
+                              {a61, a62, a63, a64, a65, 0, 0},
+                              {b1, b2, b3, b4, b5, b6, 0},
+                              {e1, e2, e3, e4, e5, e6, e7}};
+  return W[row][r];
+}
+
+// sum_{r < nk} W[row][r] k_r, summed left to right (nk >= 1).
+__device__ __forceinline__ void stage_sum(const StageSlots& ks, int row, int nk, double* acc) {
+  const double w0 = stage_w(row, 0);
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = w0 * ks.at(0, m);
+  _Pragma("unroll 1") for (int r = 1; r < nk; ++r) {
+    const double w = stage_w(row, r);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = acc[m] + w * ks.at(r, m);
+  }
+}
+
+// Input of stage st (0: y itself; 1..5: y + h * sum_{r<st} a k_r; 6: y5).
+__device__ __forceinline__ void stage_input(const StageSlots& ks, const double* y, double h,
+                                            int st, double* z) {
+  if (st == 0) {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
+    return;
+  }
+  double acc[kSlots];
+  stage_sum(ks, st, st, acc);
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
+}
+
+// Stages first..6 of a trial step: k_st = f(stage input st) into slot st; y5 = input 6.
+__device__ __forceinline__ void trial_stages(const StageSlots& ks, const double* y, double h,
+                                             const double* om, const double* op, int first,
+                                             double* y5) {
+  _Pragma("unroll 1") for (int st = first; st <= 6; ++st) {
+    double z[kSlots], kk[kSlots];
+    stage_input(ks, y, h, st, z);
+    if (st == 6) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y5[m] = z[m];
+    }
+    FKuramotoWave::f(0.0, z, om, kk, op);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+  }
+}
+
+// One replay step y <- y + h * sum b_i k_i (k7 is not formed: its weight is 0).
+__device__ __forceinline__ void kura_advance(double* y, double h, const double* om,
+                                             const double* op, const StageSlots& ks) {
+  _Pragma("unroll 1") for (int st = 0; st <= 5; ++st) {
+    double z[kSlots], kk[kSlots];
+    stage_input(ks, y, h, st, z);
+    FKuramotoWave::f(0.0, z, om, kk, op);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+  }
+  double acc[kSlots];
+  stage_sum(ks, 6, 6, acc);
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = y[m] + h * acc[m];
+}
+
 // Forward + record (gradsolve/solvers/tsit5_replay.py:100-124 per trajectory).
 __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
   __shared__ double op[2 * kOpTable];
+  __shared__ double ks_all[7 * kSlots * kKBlock];
   fill_operands(op, a.K);
-  using namespace tab;
+  const StageSlots ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
   double y[kSlots], om[kSlots], y5[kSlots];
-  double k[7][kSlots];
   load_soa(a.y0, n, j, y);
   load_soa(a.omega, n, j, om);
   double t = a.t0;
@@ -202,15 +283,14 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
     for (; qn < a.nsave && a.ts[qn] <= a.t0; ++qn)
       if (id.valid && id.leader) a.save_step[qn * n + j] = -1;
   }
-  FKuramotoWave::f(t, y, om, k[0], op);
+  int first = 0;  // the first trial also forms k1 = f(y0); later ones carry it (FSAL)
   while (__ballot(active) != 0ull) {
     const double h = active ? py_min(dt, t1 - t) : 0.0;
-    tsit5_stages<FKuramotoWave, double>(t, y, h, om, k, y5, op);
-    FKuramotoWave::f(t + h, y5, om, k[6], op);
+    trial_stages(ks, y, h, om, op, first, y5);
+    first = 1;
     double e[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m)
-      e[m] = h * (e1 * k[0][m] + e2 * k[1][m] + e3 * k[2][m] + e4 * k[3][m] + e5 * k[4][m] +
-                  e6 * k[5][m] + e7 * k[6][m]);
+    stage_sum(ks, 7, 7, e);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) e[m] = h * e[m];
     const double err = kura_wrms(e, y, y5, a.rtol, a.atol);
     if (!active) continue;
     ++it;
@@ -240,7 +320,7 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
       t = tn;
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
         y[m] = y5[m];
-        k[0][m] = k[6][m];
+        ks.at(0, m) = ks.at(6, m);  // k1 <- k7
       }
       ++acc;
     } else {
@@ -268,16 +348,12 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
   }
 }
 
-// One replay step of the wavefront's 16 trajectories (k7 is not formed: its weight is 0).
-__device__ __forceinline__ void kura_advance(double* y, double h, const double* om,
-                                             const double* op) {
-  tsit5_advance<FKuramotoWave>(0.0, y, h, om, op);
-}
-
 // Frozen-mesh replay from (y0, omega): the states at the start of each step and y_final.
 __global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) {
   __shared__ double op[2 * kOpTable];
+  __shared__ double ks_all[6 * kSlots * kKBlock];
   fill_operands(op, a.K);
+  const StageSlots ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
@@ -291,7 +367,7 @@ __global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) 
     if (act && a.states) store_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, y);
     double yn[kSlots];
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) yn[m] = y[m];
-    kura_advance(yn, h, om, op);
+    kura_advance(yn, h, om, op, ks);
     if (act) {
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = yn[m];
     }
@@ -302,7 +378,9 @@ __global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) 
 // ys[j][q] = one step of size ts[q] - t_s from the bracketing step's start state.
 __global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) {
   __shared__ double op[2 * kOpTable];
+  __shared__ double ks_all[6 * kSlots * kKBlock];
   fill_operands(op, a.K);
+  const StageSlots ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
@@ -323,7 +401,7 @@ __global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) 
       } else {
         GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = y0[m];
       }
-      kura_advance(y, h, om, op);
+      kura_advance(y, h, om, op, ks);
       if (need) store_row(row, y);
     }
     if (id.valid && s == -1) store_row(row, y0);
@@ -332,61 +410,45 @@ __global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) 
 }
 
 // lam <- (d y_next / d y)^T lam, pb += (d y_next / d omega)^T lam at the step (y, h): the
-// reverse of tsit5_step_vjp in gsx_core.h with the stage vectors in LDS (zs: this thread's
-// column of a [6][8][kKBlock] block array). Slot s holds the stage input z_s until stage s is
-// reversed and h * zbar_s afterwards, so kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) is
-// summed from the slots (s = 5 down to r + 1, the order of the generic step VJP) and no
-// partial sums stay in registers.
+// reverse of tsit5_step_vjp in gsx_core.h with the stage derivatives in LDS slots 0..5. The
+// forward recompute stores k_1..k_5 in slots 0..4; the reverse rebuilds each stage input
+// z_s from y and slots 0..s-1 (still k's), then overwrites slot s with h * zbar_s, so
+// kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) reads the slots above it (s = 5 down to
+// r + 1, the order of the generic step VJP).
 __device__ __forceinline__ void kura_step_vjp(const double* y, double h, const double* om,
                                               double* lam, double* pb, const double* op,
-                                              double* zs) {
-  using namespace tab;
-  constexpr double A[6][6] = {{0, 0, 0, 0, 0, 0},
-                              {a21, 0, 0, 0, 0, 0},
-                              {a31, a32, 0, 0, 0, 0},
-                              {a41, a42, a43, 0, 0, 0},
-                              {a51, a52, a53, a54, 0, 0},
-                              {a61, a62, a63, a64, a65, 0}};
-  constexpr double B[6] = {b1, b2, b3, b4, b5, b6};
-  // volatile: each thread reads back only its own column, and without it the compiler
-  // forwards the stores into registers and drops the LDS buffer.
-  volatile double* zv = zs;
-  auto zslot = [&](int st, int m) -> volatile double& { return zv[(st * kSlots + m) * kKBlock]; };
-  {
-    double k[5][kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) zslot(0, m) = y[m];
-    FKuramotoWave::f(0.0, y, om, k[0], op);
-    GSX_UNROLL for (int st = 1; st < 6; ++st) {
-      double z[kSlots];
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-        double acc = A[st][0] * k[0][m];
-        GSX_UNROLL for (int r = 1; r < st; ++r) acc = acc + A[st][r] * k[r][m];
-        z[m] = y[m] + h * acc;
-        zslot(st, m) = z[m];
-      }
-      if (st < 5) FKuramotoWave::f(0.0, z, om, k[st], op);
-    }
+                                              const StageSlots& ks) {
+  _Pragma("unroll 1") for (int st = 0; st <= 4; ++st) {
+    double z[kSlots], kk[kSlots];
+    stage_input(ks, y, h, st, z);
+    FKuramotoWave::f(0.0, z, om, kk, op);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
   }
   double hl[kSlots], lnew[kSlots];
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
     hl[m] = h * lam[m];
     lnew[m] = lam[m];
   }
-  // Not unrolled: an unrolled reverse lets the scheduler overlap stages past the register file.
   _Pragma("unroll 1") for (int st = 5; st >= 0; --st) {
     double kb[kSlots], z[kSlots], zb[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-      double part = 0.0;
-      for (int r = 5; r > st; --r)
-        part = (r == 5) ? A[5][st] * zslot(5, m) : part + A[r][st] * zslot(r, m);
-      kb[m] = (st == 5) ? B[5] * hl[m] : B[st] * hl[m] + part;
-      z[m] = zslot(st, m);
-      zb[m] = 0.0;
+    const double bw = stage_w(6, st);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = bw * hl[m];
+    if (st < 5) {
+      double part[kSlots];
+      const double w5 = stage_w(5, st);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = w5 * ks.at(5, m);
+      _Pragma("unroll 1") for (int r = 4; r > st; --r) {
+        const double w = stage_w(r, st);
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = part[m] + w * ks.at(r, m);
+      }
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = kb[m] + part[m];
     }
+    stage_input(ks, y, h, st, z);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) zb[m] = 0.0;
     FKuramotoWave::vjp(0.0, z, om, kb, zb, pb, op);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       lnew[m] += zb[m];
-      if (st > 0) zslot(st, m) = h * zb[m];
+      if (st > 0) ks.at(st, m) = h * zb[m];
     }
   }
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] = lnew[m];
@@ -398,9 +460,9 @@ __device__ __forceinline__ void kura_step_vjp(const double* y, double h, const d
 // iterations are the reverse sweep over the recorded states, adding mu at each bracketing step.
 __global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
   __shared__ double op[2 * kOpTable];
-  __shared__ double zs_all[6 * kSlots * kKBlock];
+  __shared__ double ks_all[6 * kSlots * kKBlock];
   fill_operands(op, a.K);
-  double* zs = zs_all + threadIdx.x;
+  const StageSlots ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
@@ -452,7 +514,7 @@ __global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
     } else {
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) v[m] = lam[m];
     }
-    kura_step_vjp(y, h, om, v, pb, op, zs);
+    kura_step_vjp(y, h, om, v, pb, op, ks);
     if (ph1) {
       if (act) store_row(a.mu + (j * ns + i) * kKD, v);
       continue;

@@ -227,6 +227,21 @@ int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
                          const double* save_t, const double* ysbar, const double* yfbar,
                          double* mu, double* y0bar, double* omega_bar, void* stream);
 
+/* The naive reverse sweep, measured by the benchmark as the plain arm (the fastest "plain"
+ * drop-in: an FP64 HIP forward plus a tape-based reverse sweep with no checkpoints). Not part of
+ * the drop-in boundary; no reference counterpart. The forward is gsx_tsit5_fwd's algorithm
+ * that additionally stores every accepted step's six stage inputs in
+ * stages[cap][6][D][n]; the adjoint reads them back instead of recomputing. Lorenz only. */
+int gsx_plain_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                        const double* p, double t0, double t1, double rtol, double atol,
+                        int64_t max_steps, double* y_final, int32_t* n_acc, int32_t* n_rej,
+                        int32_t* status, double* tape, double* stages, int64_t cap,
+                        void* stream);
+int gsx_plain_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                            const double* p, const double* tape, const double* stages,
+                            const int32_t* n_acc, const double* ybar, double* y0bar,
+                            double* pbar, void* stream);
+
 /* Export the tape to the reference's zero-padded trajectory-major form
  * out[n][S] (gradsolve/warp/warp_replay.py:180-182): out[j*S+s] = tape[s*n+j] for
  * s < n_acc[j], else 0. */

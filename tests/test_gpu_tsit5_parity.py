@@ -244,3 +244,42 @@ def test_linear_decay_closed_form(gpu):
     torch.cuda.synchronize()
     dk = -(y0.sum(axis=1)) * np.exp(-k[:, 0])
     assert np.max(np.abs(pb.cpu().numpy()[0] - dk) / np.abs(dk)) < 1e-7
+
+
+def test_plain_reverse_sweep_matches_oracle(gpu):
+    """The benchmark's plain arm (stage inputs stored for every accepted step, reverse sweep
+    without recomputation): its gradient on its own tape against the oracle adjoint."""
+    from gradsolve_amd import _native as N
+    e = W.lorenz(2048, seed=4)
+    n = e.y0.shape[0]
+    y0 = torch.as_tensor(np.ascontiguousarray(e.y0.T), device=gpu)
+    p = torch.as_tensor(np.ascontiguousarray(e.params.T), device=gpu)
+    cap = 256
+    f64 = dict(dtype=torch.float64, device=gpu)
+    i32 = dict(dtype=torch.int32, device=gpu)
+    yf, na, nr, st = (torch.empty((3, n), **f64), torch.empty(n, **i32),
+                      torch.empty(n, **i32), torch.empty(n, **i32))
+    tape = torch.zeros((cap, n), **f64)
+    stages = torch.empty((cap, 6, 3, n), **f64)
+    s = torch.cuda.current_stream().cuda_stream
+    N.call("gsx_plain_tsit5_fwd", N.FIELD_LORENZ, n, 3, 1, None, y0.data_ptr(), p.data_ptr(),
+           0.0, 1.0, 1e-8, 1e-8, 100000, yf.data_ptr(), na.data_ptr(), nr.data_ptr(),
+           st.data_ptr(), tape.data_ptr(), stages.data_ptr(), cap, s)
+    ybar = (2.0 * yf).contiguous()
+    g0, gp = torch.empty((3, n), **f64), torch.empty((1, n), **f64)
+    N.call("gsx_plain_tsit5_adjoint", N.FIELD_LORENZ, n, 3, 1, None, p.data_ptr(),
+           tape.data_ptr(), stages.data_ptr(), na.data_ptr(), ybar.data_ptr(), g0.data_ptr(),
+           gp.data_ptr(), s)
+    torch.cuda.synchronize()
+    assert int((st != 0).sum().item()) == 0
+    rec = H.record(N.FIELD_LORENZ, y0, p, t0=0.0, t1=1.0, rtol=1e-8, atol=1e-8,
+                   max_steps=100000, tape_cap=256, ckpt_interval=8)
+    torch.cuda.synchronize()
+    assert (rec.n_acc == na).to(torch.float64).mean().item() > 0.999
+    S = int(na.max().item())
+    tp = tape[:S].t().contiguous().cpu().numpy()
+    oyf = O.tsit5_replay(O.LORENZ, e.y0, e.params, 0.0, tp)
+    og0, ogp = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, tp, 2.0 * oyf)
+    rel = np.abs(gp.cpu().numpy()[0] - ogp[:, 0]) / np.abs(ogp[:, 0])
+    print(f"plain arm gradient vs oracle on its tape: max {rel.max():.2e}", flush=True)
+    assert rel.max() < 1e-10
