@@ -297,8 +297,20 @@ def kuramoto(args, dev):
     return out
 
 
+def copy(args, dev):
+    """A device copy of 2^27 float64 (1 GiB read, 1 GiB written), repeated: the calibration
+    kernel for the HBM byte counters (FETCH_SIZE / WRITE_SIZE read for it against its known
+    traffic)."""
+    n = args.n or (1 << 27)
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    t = _best(lambda: dst.copy_(src), args.repeat)
+    return {"workload": "copy", "elements": n, "bytes_read": n * 8, "bytes_written": n * 8,
+            "ms": round(t * 1e3, 3), "GB_per_s": round(2 * n * 8 / t / 1e9, 1)}
+
+
 WORKLOADS = {"uneven": uneven, "robertson": robertson, "lorenz1m": lorenz1m,
-             "kuramoto": kuramoto}
+             "kuramoto": kuramoto, "copy": copy}
 
 
 def main(argv=None):
@@ -309,7 +321,7 @@ def main(argv=None):
     ap.add_argument("--tune-sample", type=int, default=1 << 18)
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
-    names = list(WORKLOADS) if args.workload == "all" else [args.workload]
+    names = [w for w in WORKLOADS if w != "copy"] if args.workload == "all" else [args.workload]
     for name in names:
         _log(f"workload {name}")
         print(json.dumps(WORKLOADS[name](args, dev)), flush=True)
