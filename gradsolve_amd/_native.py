@@ -62,12 +62,12 @@ SIGNATURES = {
     "gsx_tsit5_persist_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                        _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_kuramoto_fwd": (_i, [_i64, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64, _vp, _i64, _vp,
-                              _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
-    "gsx_kuramoto_replay": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+                              _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i, _vp]),
+    "gsx_kuramoto_replay": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_kuramoto_saveat": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
-                                 _vp]),
+                                 _i, _vp]),
     "gsx_kuramoto_adjoint": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
-                                  _vp, _vp, _vp, _vp, _vp]),
+                                  _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_plain_tsit5_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_plain_tsit5_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

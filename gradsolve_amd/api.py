@@ -132,7 +132,7 @@ def _supports_kuramoto(problem) -> bool:
 
 
 def _solve_kuramoto(problem, y0, params, *, rtol, atol, device, max_steps=100000,
-                    **kw) -> SolveResult:
+                    fp32=False, **kw) -> SolveResult:
     import torch
 
     from . import kuramoto as KU
@@ -142,7 +142,7 @@ def _solve_kuramoto(problem, y0, params, *, rtol, atol, device, max_steps=100000
     om_soa = torch.as_tensor(np.asarray(params, np.float64), device=dev).t().contiguous()
     K = torch.as_tensor(b.aux_of(problem), device=dev)
     yf, na, nr, st = KU.forward_only(K, y0_soa, om_soa, t0=problem.t0, t1=problem.t1,
-                                     rtol=rtol, atol=atol, max_steps=max_steps)
+                                     rtol=rtol, atol=atol, max_steps=max_steps, fp32=fp32)
     y_final = yf.t().cpu().numpy().copy()
     y_final[st.cpu().numpy() != 0] = np.nan
     return SolveResult(y_final=y_final, accepted_steps=na.cpu().numpy().astype(np.int64),
@@ -282,7 +282,7 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         closure = dense.make_saveat_closure(name, b, problem, y0, params, saveat, wrt=wrt_t,
                                             rtol=rtol, atol=atol, device=dev, aux=aux,
                                             precision=precision)
-    elif name == D.HIP_KURAMOTO:
+    elif name == D.HIP_KURAMOTO and precision == "float64":
         from .closure import KuramotoClosure
         closure = KuramotoClosure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
                                   device=dev, aux=aux)

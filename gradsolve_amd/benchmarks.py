@@ -213,7 +213,8 @@ def kuramoto(args, dev):
     """Kuramoto-32 with saveat: N = 131,072, K ~ U(0, 2) shared, omega ~ N(0, 1),
     theta0 ~ U(0, 2 pi), t in [0, 10], rtol = atol = 1e-8, 64 save times linspace(0, 10, 64);
     loss mean((ys - target)^2) with target ~ N(0, 1) (torch generator seed 1 on the device);
-    gradient with respect to omega (and theta0)."""
+    gradient with respect to omega (and theta0). FP64 (coupling on FP64 MFMA) and the FP32
+    variant (coupling on FP32 MFMA)."""
     from . import kuramoto as KU
     e = W.kuramoto32(args.n or (1 << 17), seed=0)
     n = e.y0.shape[0]
@@ -221,12 +222,6 @@ def kuramoto(args, dev):
     y0, om = _soa(e.y0, dev), _soa(e.params, dev)
     ts = torch.as_tensor(np.linspace(0.0, 10.0, 64), device=dev)
     k = int(ts.numel())
-    rec = KU.record(K, y0, om, t0=0.0, t1=10.0, rtol=1e-8, atol=1e-8, ts=ts)
-    torch.cuda.synchronize()
-    assert int((rec.status != 0).sum().item()) == 0
-    acc = int(rec.n_acc.to(torch.int64).sum().item())
-    trials = acc + int(rec.n_rej.to(torch.int64).sum().item())
-    resteps = int((rec.save_step >= 0).sum().item())
     g = torch.Generator(device=dev).manual_seed(1)
     target = torch.randn((n, k, 32), generator=g, dtype=torch.float64, device=dev)
     ys = torch.empty((n, k, 32), dtype=torch.float64, device=dev)
@@ -236,64 +231,78 @@ def kuramoto(args, dev):
     ombar = torch.empty_like(y0bar)
     scale = 2.0 / ys.numel()
     st = H._stream_handle(None)
+    out = {"workload": "kuramoto32_saveat", "n": n, "nsave": k}
+    for tag, fp32 in (("fp64", False), ("fp32", True)):
+        rec = KU.record(K, y0, om, t0=0.0, t1=10.0, rtol=1e-8, atol=1e-8, ts=ts, fp32=fp32)
+        torch.cuda.synchronize()
+        assert int((rec.status != 0).sum().item()) == 0
+        acc = int(rec.n_acc.to(torch.int64).sum().item())
+        trials = acc + int(rec.n_rej.to(torch.int64).sum().item())
+        resteps = int((rec.save_step >= 0).sum().item())
 
-    def fwd():
-        N.call("gsx_kuramoto_fwd", n, 32, K.data_ptr(), y0.data_ptr(), om.data_ptr(), 0.0,
-               10.0, 1e-8, 1e-8, 100000, ts.data_ptr(), k, rec.y_final.data_ptr(),
-               rec.n_acc.data_ptr(), rec.n_rej.data_ptr(), rec.status.data_ptr(),
-               rec.tape.data_ptr(), rec.cap, rec.states.data_ptr(), rec.save_step.data_ptr(),
-               rec.save_t.data_ptr(), st)
+        def fwd_record():
+            N.call("gsx_kuramoto_fwd", n, 32, K.data_ptr(), y0.data_ptr(), om.data_ptr(), 0.0,
+                   10.0, 1e-8, 1e-8, 100000, ts.data_ptr(), k, rec.y_final.data_ptr(),
+                   rec.n_acc.data_ptr(), rec.n_rej.data_ptr(), rec.status.data_ptr(),
+                   rec.tape.data_ptr(), rec.cap, rec.states.data_ptr(),
+                   rec.save_step.data_ptr(), rec.save_t.data_ptr(), rec.flags, st)
+
+        def fwd():
+            fwd_record()
+            KU.saveat(rec, out=ys)
+
+        def cot():
+            torch.sub(ys, target, out=ysbar)
+            ysbar.mul_(scale)
+
+        def both():
+            fwd()
+            cot()
+            KU.adjoint(rec, ysbar=ysbar, mu=mu, out=(y0bar, ombar))
+
+        def fwd_only():
+            KU.forward_only(K, y0, om, t0=0.0, t1=10.0, rtol=1e-8, atol=1e-8, fp32=fp32)
+
+        tf0 = _best(fwd_only, args.repeat)
+        tf = _best(fwd, args.repeat)
+        tb = _best(both, args.repeat)
+        # per-kernel times inside one forward + gradient, HIP events on the launch stream
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        fwd_record()
+        ev[1].record()
         KU.saveat(rec, out=ys)
-
-    def both():
-        fwd()
-        torch.sub(ys, target, out=ysbar)
-        ysbar.mul_(scale)
+        ev[2].record()
+        cot()
+        ev[3].record()
         KU.adjoint(rec, ysbar=ysbar, mu=mu, out=(y0bar, ombar))
-
-    def fwd_only():
-        KU.forward_only(K, y0, om, t0=0.0, t1=10.0, rtol=1e-8, atol=1e-8)
-
-    tf0 = _best(fwd_only, args.repeat)
-    tf = _best(fwd, args.repeat)
-    tb = _best(both, args.repeat)
-    # per-kernel times inside one forward + gradient, HIP events on the launch stream
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-    torch.cuda.synchronize()
-    ev[0].record()
-    N.call("gsx_kuramoto_fwd", n, 32, K.data_ptr(), y0.data_ptr(), om.data_ptr(), 0.0, 10.0,
-           1e-8, 1e-8, 100000, ts.data_ptr(), k, rec.y_final.data_ptr(), rec.n_acc.data_ptr(),
-           rec.n_rej.data_ptr(), rec.status.data_ptr(), rec.tape.data_ptr(), rec.cap,
-           rec.states.data_ptr(), rec.save_step.data_ptr(), rec.save_t.data_ptr(), st)
-    ev[1].record()
-    KU.saveat(rec, out=ys)
-    ev[2].record()
-    torch.sub(ys, target, out=ysbar)
-    ysbar.mul_(scale)
-    ev[3].record()
-    KU.adjoint(rec, ysbar=ysbar, mu=mu, out=(y0bar, ombar))
-    ev[4].record()
-    torch.cuda.synchronize()
-    ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
-    fl_fwd = KURA_TRIAL_FLOPS * trials + KURA_RHS_FLOPS * n
-    fl_sav = KURA_RESTEP_FLOPS * resteps
-    fl_adj = KURA_VJP_STEP_FLOPS * (acc + resteps)
-    out = {"workload": "kuramoto32_saveat", "n": n, "nsave": k,
-           "mean_accepted": round(acc / n, 2), "max_accepted": rec.max_steps_taken(),
-           "mean_trials": round(trials / n, 2), "workspace_GB": round(
-               (rec.states.numel() + 4 * ys.numel()) * 8 / 1e9, 2),
-           "fwd_only_ms": round(tf0 * 1e3, 3), "fwd_saveat_ms": round(tf * 1e3, 3),
-           "fwd_adj_ms": round(tb * 1e3, 3),
-           "fwd_traj_per_s": round(n / tf, 1), "fwd_adj_traj_per_s": round(n / tb, 1),
-           "fwd_adj_acc_steps_per_s": round(acc / tb, 1),
-           "kernel_ms": {"kura_fwd_kernel": round(ms[0], 3), "kura_saveat_kernel":
-                         round(ms[1], 3), "cotangent": round(ms[2], 3),
-                         "kura_adj_kernel": round(ms[3], 3)},
-           "mfma_tflops": {"kura_fwd_kernel": round(fl_fwd / ms[0] / 1e9, 2),
-                           "kura_saveat_kernel": round(fl_sav / ms[1] / 1e9, 2),
-                           "kura_adj_kernel": round(fl_adj / ms[3] / 1e9, 2)},
-           "mfma_frac_of_fp64_peak": round((fl_fwd + fl_sav + fl_adj) / tb / 1e12 /
-                                           WC.PEAK_FP64_TFLOPS, 4)}
+        ev[4].record()
+        torch.cuda.synchronize()
+        ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+        fl_fwd = KURA_TRIAL_FLOPS * trials + KURA_RHS_FLOPS * n
+        fl_sav = KURA_RESTEP_FLOPS * resteps
+        fl_adj = KURA_VJP_STEP_FLOPS * (acc + resteps)
+        peak = WC.PEAK_FP64_TFLOPS if not fp32 else 2 * WC.PEAK_FP64_TFLOPS
+        out[tag] = {
+            "mean_accepted": round(acc / n, 2), "max_accepted": rec.max_steps_taken(),
+            "mean_trials": round(trials / n, 2),
+            "workspace_GB": round((rec.states.numel() + 4 * ys.numel()) * 8 / 1e9, 2),
+            "fwd_only_ms": round(tf0 * 1e3, 3), "fwd_saveat_ms": round(tf * 1e3, 3),
+            "fwd_adj_ms": round(tb * 1e3, 3),
+            "fwd_traj_per_s": round(n / tf, 1), "fwd_adj_traj_per_s": round(n / tb, 1),
+            "fwd_adj_acc_steps_per_s": round(acc / tb, 1),
+            "kernel_ms": {"kura_fwd_kernel": round(ms[0], 3),
+                          "kura_saveat_kernel": round(ms[1], 3), "cotangent": round(ms[2], 3),
+                          "kura_adj_kernel": round(ms[3], 3)},
+            "mfma_tflops": {"kura_fwd_kernel": round(fl_fwd / ms[0] / 1e9, 2),
+                            "kura_saveat_kernel": round(fl_sav / ms[1] / 1e9, 2),
+                            "kura_adj_kernel": round(fl_adj / ms[3] / 1e9, 2)},
+            "mfma_peak_tflops": peak,
+            "mfma_frac_of_peak": round((fl_fwd + fl_sav + fl_adj) / tb / 1e12 / peak, 4)}
+        _log(f"kuramoto {tag}: {out[tag]}")
+        del rec
+        torch.cuda.empty_cache()
     return out
 
 

@@ -248,7 +248,7 @@ class KuramotoClosure(ReplayClosure):
     the save-time cotangents."""
 
     def __init__(self, binding, problem, y0, params, *, wrt, rtol, atol, device, aux,
-                 ts=None, max_steps=100000):
+                 ts=None, max_steps=100000, fp32=False):
         from . import kuramoto as KU
         self.binding, self.problem, self.wrt, self.device = binding, problem, wrt, device
         y0_t = _as_device(y0, device)
@@ -260,7 +260,8 @@ class KuramotoClosure(ReplayClosure):
         self.ts = None if ts is None else torch.as_tensor(ts, dtype=torch.float64,
                                                           device=device)
         self.rec = KU.record(aux, self._y0_soa, self._p_soa, t0=problem.t0, t1=problem.t1,
-                             rtol=rtol, atol=atol, ts=self.ts, max_steps=max_steps)
+                             rtol=rtol, atol=atol, ts=self.ts, max_steps=max_steps,
+                             fp32=fp32)
         st = self.rec.status
         bad = torch.nonzero(st != 0).flatten()
         if bad.numel():

@@ -360,6 +360,11 @@ int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* 
 // ---------------------------------------------------------------------------------------
 // Kuramoto ensemble (D = P = 32, shared K): gsx_kuramoto.hip.
 // ---------------------------------------------------------------------------------------
+static int kura_check_flags(int flags) {
+  if (flags & ~GSX_FP32) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  return GSX_OK;
+}
+
 static int kura_check_d(int D, const char* what) {
   if (D != 32)
     return set_error(GSX_E_UNSUPPORTED, "%s: the Kuramoto kernels are built for D = 32, got %d",
@@ -371,8 +376,11 @@ int gsx_kuramoto_fwd(int64_t n, int D, const double* K, const double* y0, const 
                      double t0, double t1, double rtol, double atol, int64_t max_steps,
                      const double* ts, int64_t nsave, double* y_final, int32_t* n_acc,
                      int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
-                     double* states, int32_t* save_step, double* save_t, void* stream) {
+                     double* states, int32_t* save_step, double* save_t, int flags,
+                     void* stream) {
   int rc = kura_check_d(D, "kuramoto_fwd");
+  if (rc) return rc;
+  rc = kura_check_flags(flags);
   if (rc) return rc;
   if (n < 0 || n > INT32_MAX || max_steps < 0 || nsave < 0)
     return set_error(GSX_E_INVALID, "kuramoto_fwd: n in [0, 2^31), max_steps >= 0 and "
@@ -409,27 +417,31 @@ int gsx_kuramoto_fwd(int64_t n, int D, const double* K, const double* y0, const 
   a.states = states;
   a.save_step = nsave > 0 ? save_step : nullptr;
   a.save_t = nsave > 0 ? save_t : nullptr;
-  return kura_fwd_launch(a, static_cast<hipStream_t>(stream));
+  return kura_fwd_launch(a, (flags & GSX_FP32) != 0, static_cast<hipStream_t>(stream));
 }
 
 int gsx_kuramoto_replay(int64_t n, int D, const double* K, const double* y0,
                         const double* omega, const double* tape, const int32_t* n_acc,
-                        double* y_final, double* states, void* stream) {
+                        double* y_final, double* states, int flags, void* stream) {
   int rc = kura_check_d(D, "kuramoto_replay");
+  if (rc) return rc;
+  rc = kura_check_flags(flags);
   if (rc) return rc;
   if (n < 0 || n > INT32_MAX) return set_error(GSX_E_INVALID, "kuramoto_replay: bad n");
   if (n > 0 && (!K || !y0 || !omega || !tape || !n_acc || !y_final))
     return set_error(GSX_E_INVALID, "kuramoto_replay: null pointer");
   if (n == 0) return GSX_OK;
   KuraReplayArgs a{n, K, y0, omega, tape, n_acc, y_final, states};
-  return kura_replay_launch(a, static_cast<hipStream_t>(stream));
+  return kura_replay_launch(a, (flags & GSX_FP32) != 0, static_cast<hipStream_t>(stream));
 }
 
 int gsx_kuramoto_saveat(int64_t n, int D, const double* K, const double* y0,
                         const double* omega, const double* y_final, const double* states,
                         const double* ts, int64_t nsave, const int32_t* save_step,
-                        const double* save_t, double* ys, void* stream) {
+                        const double* save_t, double* ys, int flags, void* stream) {
   int rc = kura_check_d(D, "kuramoto_saveat");
+  if (rc) return rc;
+  rc = kura_check_flags(flags);
   if (rc) return rc;
   if (n < 0 || n > INT32_MAX || nsave < 0)
     return set_error(GSX_E_INVALID, "kuramoto_saveat: bad n or nsave");
@@ -438,15 +450,18 @@ int gsx_kuramoto_saveat(int64_t n, int D, const double* K, const double* y0,
     return set_error(GSX_E_INVALID, "kuramoto_saveat: null pointer");
   if (n == 0 || nsave == 0) return GSX_OK;
   KuraSaveatArgs a{n, K, y0, omega, y_final, states, ts, nsave, save_step, save_t, ys};
-  return kura_saveat_launch(a, static_cast<hipStream_t>(stream));
+  return kura_saveat_launch(a, (flags & GSX_FP32) != 0, static_cast<hipStream_t>(stream));
 }
 
 int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
                          const double* tape, const int32_t* n_acc, const double* states,
                          const double* ts, int64_t nsave, const int32_t* save_step,
                          const double* save_t, const double* ysbar, const double* yfbar,
-                         double* mu, double* y0bar, double* omega_bar, void* stream) {
+                         double* mu, double* y0bar, double* omega_bar, int flags,
+                         void* stream) {
   int rc = kura_check_d(D, "kuramoto_adjoint");
+  if (rc) return rc;
+  rc = kura_check_flags(flags);
   if (rc) return rc;
   if (n < 0 || n > INT32_MAX || nsave < 0)
     return set_error(GSX_E_INVALID, "kuramoto_adjoint: bad n or nsave");
@@ -474,7 +489,7 @@ int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
   a.mu = mu;
   a.y0bar = y0bar;
   a.omega_bar = omega_bar;
-  return kura_adj_launch(a, static_cast<hipStream_t>(stream));
+  return kura_adj_launch(a, (flags & GSX_FP32) != 0, static_cast<hipStream_t>(stream));
 }
 
 // ---------------------------------------------------------------------------------------

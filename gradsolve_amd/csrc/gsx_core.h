@@ -213,6 +213,47 @@ GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
   *c_out = ((q + 1) & 2) ? -ca : ca;
 }
 
+// Single-precision counterpart (the FP32 Kuramoto variant): for |x| < 2^14 the same three-part
+// reduction with an FP32 split of pi/2 (hi = 0x1.921fb6p+0, mid = -0x1.777a5cp-25,
+// lo = -0x1.ee59dap-50; the first subtraction is exact because k*hi and x are multiples of
+// 2^-23 there), then the Taylor series of sin through r^11 and of cos through r^12 on
+// |r| <= pi/4 (the first omitted terms are below 2^-36 relative). Larger |x| and non-finite x
+// go to the math library.
+GSX_HD void gsx_sincosf(float x, float* s_out, float* c_out) {
+  if (!(std::fabs(x) < 16384.0f)) {
+    *s_out = std::sin(x);
+    *c_out = std::cos(x);
+    return;
+  }
+  constexpr float kTwoOverPi = 0.636619746685028f;
+  constexpr float kPio2Hi = 0x1.921fb6p+0f;
+  constexpr float kPio2Mid = -0x1.777a5cp-25f;
+  constexpr float kPio2Lo = -0x1.ee59dap-50f;
+  const float k = std::rint(x * kTwoOverPi);
+  float r = std::fma(-k, kPio2Hi, x);
+  r = std::fma(-k, kPio2Mid, r);
+  r = std::fma(-k, kPio2Lo, r);
+  const float z = r * r;
+  float ps = -1.0f / 39916800.0f;  // -1/11!
+  ps = std::fma(z, ps, 1.0f / 362880.0f);
+  ps = std::fma(z, ps, -1.0f / 5040.0f);
+  ps = std::fma(z, ps, 1.0f / 120.0f);
+  ps = std::fma(z, ps, -1.0f / 6.0f);
+  const float sr = std::fma(r * z, ps, r);
+  float pc = 1.0f / 479001600.0f;  // 1/12!
+  pc = std::fma(z, pc, -1.0f / 3628800.0f);
+  pc = std::fma(z, pc, 1.0f / 40320.0f);
+  pc = std::fma(z, pc, -1.0f / 720.0f);
+  pc = std::fma(z, pc, 1.0f / 24.0f);
+  pc = std::fma(z, pc, -0.5f);
+  const float cr = std::fma(z, pc, 1.0f);
+  const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
+  const float sa = (q & 1) ? cr : sr;
+  const float ca = (q & 1) ? sr : cr;
+  *s_out = (q & 2) ? -sa : sa;
+  *c_out = ((q + 1) & 2) ? -ca : ca;
+}
+
 // Python's builtin min/max argument semantics (min(a, b) is a unless b < a), which the
 // reference controller uses (gradsolve/solvers/tsit5_replay.py:108,124). They propagate a NaN
 // in the first operand, unlike fmin/fmax.

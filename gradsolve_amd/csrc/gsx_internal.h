@@ -35,10 +35,10 @@ int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t
 int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
 int persist_fwd_dispatch(int field, int D, bool fp32, const PersistFwdArgs& pa, hipStream_t s);
 int persist_adj_dispatch(int field, int D, bool fp32, const PersistAdjArgs& pa, hipStream_t s);
-int kura_fwd_launch(const KuraFwdArgs& a, hipStream_t s);
-int kura_replay_launch(const KuraReplayArgs& a, hipStream_t s);
-int kura_saveat_launch(const KuraSaveatArgs& a, hipStream_t s);
-int kura_adj_launch(const KuraAdjArgs& a, hipStream_t s);
+int kura_fwd_launch(const KuraFwdArgs& a, bool fp32, hipStream_t s);
+int kura_replay_launch(const KuraReplayArgs& a, bool fp32, hipStream_t s);
+int kura_saveat_launch(const KuraSaveatArgs& a, bool fp32, hipStream_t s);
+int kura_adj_launch(const KuraAdjArgs& a, bool fp32, hipStream_t s);
 int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 
 }  // namespace gsx

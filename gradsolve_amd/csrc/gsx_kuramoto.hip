@@ -32,6 +32,14 @@
 //                                          the save is y0; -2: ts[q] >= the final time, the
 //                                          save is y_final) and that step's start time
 //   ys, ysbar, mu        : [n][k][32]      (the reference's y_saved layout)
+//
+// FP32 variant (flag GSX_FP32): the field, the stages and the step VJPs run in FP32 with the
+// coupling on v_mfma_f32_16x16x4f32, whose accumulator rows are blocked (D[4(l/16) + r][l%16],
+// against the FP64 instruction's interleaved D[4r + l/16][l%16]), so tile T's row rho holds K
+// row 8*(rho%4) + 2*(rho/4) + T. The state and the adjoint's cotangent and parameter-gradient
+// accumulators are Kahan-compensated FP32 pairs (value hi + lo); time, the step size, the
+// tape, the WRMS norm and the controller stay in FP64, and the HBM buffers stay FP64 (a pair
+// hi + lo is exact in FP64).
 #include <hip/hip_runtime.h>
 
 #include "gsx_internal.h"
@@ -40,9 +48,7 @@ namespace gsx {
 namespace {
 
 typedef double kd4 __attribute__((ext_vector_type(4)));
-// LDS-qualified double: accesses through it compile to ds_read/ds_write (a generic pointer to
-// LDS becomes FLAT instructions when the access is volatile).
-typedef __attribute__((address_space(3))) double lds_double;
+typedef float kf4 __attribute__((ext_vector_type(4)));
 
 constexpr int kKD = 32;            // phases per trajectory
 constexpr int kSlots = 8;          // phases per lane
@@ -52,35 +58,62 @@ constexpr int kOpTable = 16 * 64;  // A operands of one product direction, [T*8 
 constexpr double kInvD = 1.0 / 32.0;
 
 __device__ __forceinline__ int kdim(int g, int m) { return 8 * (m & 3) + 2 * g + (m >> 2); }
-__device__ __forceinline__ int krow(int T, int rho) {
-  return 8 * (rho >> 2) + 2 * (rho & 3) + T;
-}
+
+// Precision policies: the MFMA, its row permutation (see the header), sin/cos, and whether the
+// state is a compensated pair. LDS-qualified element types make the volatile accesses compile
+// to ds_read/ds_write (through a generic pointer they become FLAT instructions).
+template <class R>
+struct Prec;
+template <>
+struct Prec<double> {
+  using acc4 = kd4;
+  using lds_t = __attribute__((address_space(3))) double;
+  static constexpr bool kComp = false;
+  __device__ static acc4 mfma(double a, double b, acc4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int krow(int T, int rho) { return 8 * (rho >> 2) + 2 * (rho & 3) + T; }
+  __device__ static void sincos(double x, double* s, double* c) { gsx_sincos(x, s, c); }
+};
+template <>
+struct Prec<float> {
+  using acc4 = kf4;
+  using lds_t = __attribute__((address_space(3))) float;
+  static constexpr bool kComp = true;
+  __device__ static acc4 mfma(float a, float b, acc4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int krow(int T, int rho) { return 8 * (rho & 3) + 2 * (rho >> 2) + T; }
+  __device__ static void sincos(float x, float* s, float* c) { gsx_sincosf(x, s, c); }
+};
 
 // Fills the block's operand tables: op[e*64 + l] for e = T*8 + kk, forward then transposed.
-__device__ void fill_operands(double* op, const double* K) {
+template <class R>
+__device__ void fill_operands(R* op, const double* K) {
   for (int x = threadIdx.x; x < kOpTable; x += blockDim.x) {
     const int e = x >> 6, l = x & 63;
     const int T = e >> 3, kk = e & 7;
-    const int row = krow(T, l & 15), col = kdim(l >> 4, kk);
-    op[x] = K[row * kKD + col];
-    op[kOpTable + x] = K[col * kKD + row];
+    const int row = Prec<R>::krow(T, l & 15), col = kdim(l >> 4, kk);
+    op[x] = static_cast<R>(K[row * kKD + col]);
+    op[kOpTable + x] = static_cast<R>(K[col * kKD + row]);
   }
   __syncthreads();
 }
 
 // a = (K s) and b = (K c) at the lane's 8 slots (tab: forward or transposed table).
-__device__ __forceinline__ void coupling(const double* tab, const double* s, const double* c,
-                                         double* a, double* b) {
+template <class R>
+__device__ __forceinline__ void coupling(const R* tab, const R* s, const R* c, R* a, R* b) {
+  using P = Prec<R>;
   const int l = static_cast<int>(__lane_id());
   // volatile keeps the operand reads next to their MFMAs instead of 32 hoisted registers
-  const volatile lds_double* tv = (const volatile lds_double*)tab;
-  kd4 as0 = {0.0, 0.0, 0.0, 0.0}, as1 = as0, ac0 = as0, ac1 = as0;
+  const volatile typename P::lds_t* tv = (const volatile typename P::lds_t*)tab;
+  typename P::acc4 as0 = {R(0), R(0), R(0), R(0)}, as1 = as0, ac0 = as0, ac1 = as0;
   GSX_UNROLL for (int kk = 0; kk < 8; ++kk) {
-    const double k0 = tv[kk * 64 + l], k1 = tv[(8 + kk) * 64 + l];
-    as0 = __builtin_amdgcn_mfma_f64_16x16x4f64(k0, s[kk], as0, 0, 0, 0);
-    ac0 = __builtin_amdgcn_mfma_f64_16x16x4f64(k0, c[kk], ac0, 0, 0, 0);
-    as1 = __builtin_amdgcn_mfma_f64_16x16x4f64(k1, s[kk], as1, 0, 0, 0);
-    ac1 = __builtin_amdgcn_mfma_f64_16x16x4f64(k1, c[kk], ac1, 0, 0, 0);
+    const R k0 = tv[kk * 64 + l], k1 = tv[(8 + kk) * 64 + l];
+    as0 = P::mfma(k0, s[kk], as0);
+    ac0 = P::mfma(k0, c[kk], ac0);
+    as1 = P::mfma(k1, s[kk], as1);
+    ac1 = P::mfma(k1, c[kk], ac1);
   }
   GSX_UNROLL for (int r = 0; r < 4; ++r) {
     a[r] = as0[r];
@@ -90,36 +123,33 @@ __device__ __forceinline__ void coupling(const double* tab, const double* s, con
   }
 }
 
-// The field as the per-lane Tsit5 templates see it: D = P = 8 slots of one trajectory; every
-// lane of the wavefront must call it together (the MFMA spans the wavefront). `aux` is the
-// block's operand tables.
+// The field on the lane's 8 slots of one trajectory; every lane of the wavefront must call it
+// together (the MFMA spans the wavefront). `op` is the block's operand tables.
+template <class R>
 struct FKuramotoWave {
-  __device__ static void f(double, const double* y, const double* om, double* dy,
-                           const double* op) {
-    double s[kSlots], c[kSlots], a[kSlots], b[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) gsx_sincos(y[m], &s[m], &c[m]);
-    coupling(op, s, c, a, b);
+  __device__ static void f(const R* y, const R* om, R* dy, const R* op) {
+    R s[kSlots], c[kSlots], a[kSlots], b[kSlots];
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) Prec<R>::sincos(y[m], &s[m], &c[m]);
+    coupling<R>(op, s, c, a, b);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m)
-      dy[m] = om[m] + kInvD * (a[m] * c[m] - b[m] * s[m]);
+      dy[m] = om[m] + R(kInvD) * (a[m] * c[m] - b[m] * s[m]);
   }
 
   // yb += J^T v, pb += v: (J^T v)_m = (1/32) [c_m (K^T (v c))_m + s_m (K^T (v s))_m
   //                                            - v_m (a_m s_m + b_m c_m)].
-  __device__ static void vjp(double, const double* y, const double* om, const double* v,
-                             double* yb, double* pb, const double* op) {
-    (void)om;
-    double s[kSlots], c[kSlots], a[kSlots], b[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) gsx_sincos(y[m], &s[m], &c[m]);
-    coupling(op, s, c, a, b);
-    double dg[kSlots], vc[kSlots], vs[kSlots], u[kSlots], w[kSlots];
+  __device__ static void vjp(const R* y, const R* v, R* yb, R* pb, const R* op) {
+    R s[kSlots], c[kSlots], a[kSlots], b[kSlots];
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) Prec<R>::sincos(y[m], &s[m], &c[m]);
+    coupling<R>(op, s, c, a, b);
+    R dg[kSlots], vc[kSlots], vs[kSlots], u[kSlots], w[kSlots];
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       dg[m] = a[m] * s[m] + b[m] * c[m];
       vc[m] = v[m] * c[m];
       vs[m] = v[m] * s[m];
     }
-    coupling(op + kOpTable, vc, vs, u, w);
+    coupling<R>(op + kOpTable, vc, vs, u, w);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-      yb[m] += kInvD * (c[m] * u[m] + s[m] * w[m] - v[m] * dg[m]);
+      yb[m] += R(kInvD) * (c[m] * u[m] + s[m] * w[m] - v[m] * dg[m]);
       pb[m] += v[m];
     }
   }
@@ -165,6 +195,49 @@ __device__ __forceinline__ void store_soa(double* a, int64_t n, int64_t j, const
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) a[kdim(g, m) * n + j] = v[m];
 }
 
+// A value of the state or of an accumulator: hi alone in FP64, the compensated pair hi + lo
+// in FP32 (lo stays 0 in FP64).
+template <class R>
+struct Vec {
+  R hi[kSlots], lo[kSlots];
+  __device__ __forceinline__ void set(const double* v) {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+      hi[m] = static_cast<R>(v[m]);
+      lo[m] = Prec<R>::kComp ? static_cast<R>(v[m] - static_cast<double>(hi[m])) : R(0);
+    }
+  }
+  __device__ __forceinline__ void zero() {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) hi[m] = lo[m] = R(0);
+  }
+  __device__ __forceinline__ void get(double* v) const {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m)
+      v[m] = Prec<R>::kComp ? static_cast<double>(hi[m]) + static_cast<double>(lo[m])
+                            : static_cast<double>(hi[m]);
+  }
+  // += d (R increments): Kahan in FP32, plain in FP64
+  __device__ __forceinline__ void add(const R* d) {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+      if constexpr (Prec<R>::kComp) {
+        kahan_add(hi[m], lo[m], d[m]);
+      } else {
+        hi[m] += d[m];
+      }
+    }
+  }
+  // += v (FP64 values): both halves of the value in FP32
+  __device__ __forceinline__ void add_f64(const double* v) {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+      if constexpr (Prec<R>::kComp) {
+        const float vh = static_cast<float>(v[m]);
+        kahan_add(hi[m], lo[m], vh);
+        kahan_add(hi[m], lo[m], static_cast<float>(v[m] - static_cast<double>(vh)));
+      } else {
+        hi[m] += v[m];
+      }
+    }
+  }
+};
+
 struct LaneId {
   int64_t j;
   bool valid, leader;  // leader: lane group 0 writes the per-trajectory scalars
@@ -185,93 +258,131 @@ __device__ __forceinline__ LaneId lane_id(int64_t n) {
 // 8 sincos and 32 MFMAs) instead of one per stage: unrolled, the kernels exceeded 110 KB of
 // code, far past the instruction cache. Accesses are volatile: each thread reads back only its
 // own column, and the compiler would otherwise forward the stores into registers.
+template <class R>
 struct StageSlots {
-  volatile lds_double* base;
-  __device__ __forceinline__ explicit StageSlots(double* column)
-      : base((volatile lds_double*)column) {}
-  __device__ __forceinline__ volatile lds_double& at(int s, int m) const {
+  using lds_t = typename Prec<R>::lds_t;
+  volatile lds_t* base;
+  __device__ __forceinline__ explicit StageSlots(R* column) : base((volatile lds_t*)column) {}
+  __device__ __forceinline__ volatile lds_t& at(int s, int m) const {
     return base[(s * kSlots + m) * kKBlock];
   }
 };
 
 // Row st of the stage weights: rows 1..5 the inputs of k2..k6 (a_{st+1, 1..st}), row 6 the
 // solution weights b_1..b_6, row 7 the error weights e_1..e_7 (b_i - bhat_i).
-__device__ __forceinline__ double stage_w(int row, int r) {
+template <class R>
+__device__ __forceinline__ R stage_w(int row, int r) {
   using namespace tab;
-  constexpr double W[8][7] = {{0, 0, 0, 0, 0, 0, 0},
-                              {a21, 0, 0, 0, 0, 0, 0},
-                              {a31, a32, 0, 0, 0, 0, 0},
-                              {a41, a42, a43, 0, 0, 0, 0},
-                              {a51, a52, a53, a54, 0, 0, 0},
-                              {a61, a62, a63, a64, a65, 0, 0},
-                              {b1, b2, b3, b4, b5, b6, 0},
-                              {e1, e2, e3, e4, e5, e6, e7}};
+  constexpr R W[8][7] = {{0, 0, 0, 0, 0, 0, 0},
+                         {R(a21), 0, 0, 0, 0, 0, 0},
+                         {R(a31), R(a32), 0, 0, 0, 0, 0},
+                         {R(a41), R(a42), R(a43), 0, 0, 0, 0},
+                         {R(a51), R(a52), R(a53), R(a54), 0, 0, 0},
+                         {R(a61), R(a62), R(a63), R(a64), R(a65), 0, 0},
+                         {R(b1), R(b2), R(b3), R(b4), R(b5), R(b6), 0},
+                         {R(e1), R(e2), R(e3), R(e4), R(e5), R(e6), R(e7)}};
   return W[row][r];
 }
 
 // sum_{r < nk} W[row][r] k_r, summed left to right (nk >= 1).
-__device__ __forceinline__ void stage_sum(const StageSlots& ks, int row, int nk, double* acc) {
-  const double w0 = stage_w(row, 0);
+template <class R>
+__device__ __forceinline__ void stage_sum(const StageSlots<R>& ks, int row, int nk, R* acc) {
+  const R w0 = stage_w<R>(row, 0);
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = w0 * ks.at(0, m);
   _Pragma("unroll 1") for (int r = 1; r < nk; ++r) {
-    const double w = stage_w(row, r);
+    const R w = stage_w<R>(row, r);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = acc[m] + w * ks.at(r, m);
   }
 }
 
-// Input of stage st (0: y itself; 1..5: y + h * sum_{r<st} a k_r; 6: y5).
-__device__ __forceinline__ void stage_input(const StageSlots& ks, const double* y, double h,
-                                            int st, double* z) {
+// Input of stage st (0: y itself; 1..5: y + h * sum_{r<st} a k_r). In FP32, y is the
+// compensated state's leading half.
+template <class R>
+__device__ __forceinline__ void stage_input(const StageSlots<R>& ks, const R* y, R h, int st,
+                                            R* z) {
   if (st == 0) {
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
     return;
   }
-  double acc[kSlots];
-  stage_sum(ks, st, st, acc);
+  R acc[kSlots];
+  stage_sum<R>(ks, st, st, acc);
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
 }
 
-// Stages first..6 of a trial step: k_st = f(stage input st) into slot st; y5 = input 6.
-__device__ __forceinline__ void trial_stages(const StageSlots& ks, const double* y, double h,
-                                             const double* om, const double* op, int first,
-                                             double* y5) {
-  _Pragma("unroll 1") for (int st = first; st <= 6; ++st) {
-    double z[kSlots], kk[kSlots];
-    stage_input(ks, y, h, st, z);
-    if (st == 6) {
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y5[m] = z[m];
+// The step's increment h * sum b_i k_i (FP32) or y5 = y + h * sum b_i k_i (FP64), from slots
+// 0..5.
+template <class R>
+__device__ __forceinline__ void step_update(const StageSlots<R>& ks, const R* y, R h, R* y5,
+                                            R* inc) {
+  R acc[kSlots];
+  stage_sum<R>(ks, 6, 6, acc);
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+    if constexpr (Prec<R>::kComp) {
+      inc[m] = h * acc[m];
+      y5[m] = y[m] + inc[m];
+    } else {
+      y5[m] = y[m] + h * acc[m];
     }
-    FKuramotoWave::f(0.0, z, om, kk, op);
+  }
+}
+
+// Stages first..6 of a trial step: k_st = f(stage input st) into slot st, with the stage-6
+// input y5 (and, in FP32, the increment).
+template <class R>
+__device__ __forceinline__ void trial_stages(const StageSlots<R>& ks, const R* y, R h,
+                                             const R* om, const R* op, int first, R* y5,
+                                             R* inc) {
+  _Pragma("unroll 1") for (int st = first; st <= 6; ++st) {
+    R z[kSlots], kk[kSlots];
+    if (st == 6) {
+      step_update<R>(ks, y, h, y5, inc);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y5[m];
+    } else {
+      stage_input<R>(ks, y, h, st, z);
+    }
+    FKuramotoWave<R>::f(z, om, kk, op);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
   }
 }
 
-// One replay step y <- y + h * sum b_i k_i (k7 is not formed: its weight is 0).
-__device__ __forceinline__ void kura_advance(double* y, double h, const double* om,
-                                             const double* op, const StageSlots& ks) {
+// One replay step of the state (k7 is not formed: its weight is 0).
+template <class R>
+__device__ __forceinline__ void kura_advance(Vec<R>& y, R h, const R* om, const R* op,
+                                             const StageSlots<R>& ks) {
   _Pragma("unroll 1") for (int st = 0; st <= 5; ++st) {
-    double z[kSlots], kk[kSlots];
-    stage_input(ks, y, h, st, z);
-    FKuramotoWave::f(0.0, z, om, kk, op);
+    R z[kSlots], kk[kSlots];
+    stage_input<R>(ks, y.hi, h, st, z);
+    FKuramotoWave<R>::f(z, om, kk, op);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
   }
-  double acc[kSlots];
-  stage_sum(ks, 6, 6, acc);
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = y[m] + h * acc[m];
+  R y5[kSlots], inc[kSlots];
+  step_update<R>(ks, y.hi, h, y5, inc);
+  if constexpr (Prec<R>::kComp) {
+    y.add(inc);
+  } else {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) y.hi[m] = y5[m];
+  }
 }
 
 // Forward + record (gradsolve/solvers/tsit5_replay.py:100-124 per trajectory).
+template <class R>
 __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
-  __shared__ double op[2 * kOpTable];
-  __shared__ double ks_all[7 * kSlots * kKBlock];
-  fill_operands(op, a.K);
-  const StageSlots ks(ks_all + threadIdx.x);
+  __shared__ R op[2 * kOpTable];
+  __shared__ R ks_all[7 * kSlots * kKBlock];
+  fill_operands<R>(op, a.K);
+  const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
-  double y[kSlots], om[kSlots], y5[kSlots];
-  load_soa(a.y0, n, j, y);
-  load_soa(a.omega, n, j, om);
+  Vec<R> y;
+  R om[kSlots], y5[kSlots], inc[kSlots];
+  {
+    double v[kSlots];
+    load_soa(a.y0, n, j, v);
+    y.set(v);
+    load_soa(a.omega, n, j, v);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) om[m] = static_cast<R>(v[m]);
+  }
   double t = a.t0;
   const double t1 = a.t1;
   double dt = (t1 - t) / 100.0;
@@ -286,12 +397,20 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
   int first = 0;  // the first trial also forms k1 = f(y0); later ones carry it (FSAL)
   while (__ballot(active) != 0ull) {
     const double h = active ? py_min(dt, t1 - t) : 0.0;
-    trial_stages(ks, y, h, om, op, first, y5);
+    const R hr = static_cast<R>(h);
+    trial_stages<R>(ks, y.hi, hr, om, op, first, y5, inc);
     first = 1;
-    double e[kSlots];
-    stage_sum(ks, 7, 7, e);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) e[m] = h * e[m];
-    const double err = kura_wrms(e, y, y5, a.rtol, a.atol);
+    double e[kSlots], yd[kSlots], y5d[kSlots];
+    {
+      R er[kSlots];
+      stage_sum<R>(ks, 7, 7, er);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        e[m] = static_cast<double>(hr * er[m]);
+        y5d[m] = static_cast<double>(y5[m]);
+      }
+      y.get(yd);
+    }
+    const double err = kura_wrms(e, yd, y5d, a.rtol, a.atol);
     if (!active) continue;
     ++it;
     if (err != err) {
@@ -307,7 +426,7 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
         continue;
       }
       const double tn = t + h;
-      if (a.states) store_row(a.states + (static_cast<int64_t>(acc) * n + j) * kKD, y);
+      if (a.states) store_row(a.states + (static_cast<int64_t>(acc) * n + j) * kKD, yd);
       if (a.tape && id.leader) a.tape[static_cast<int64_t>(acc) * n + j] = h;
       if (a.save_step) {
         for (; qn < a.nsave && a.ts[qn] <= tn; ++qn) {
@@ -318,10 +437,12 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
         }
       }
       t = tn;
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-        y[m] = y5[m];
-        ks.at(0, m) = ks.at(6, m);  // k1 <- k7
+      if constexpr (Prec<R>::kComp) {
+        y.add(inc);
+      } else {
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) y.hi[m] = y5[m];
       }
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(0, m) = ks.at(6, m);  // k1 <- k7
       ++acc;
     } else {
       ++rej;
@@ -337,7 +458,11 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
   }
   if (!id.valid) return;
   if (status == kReached && !(t >= t1)) status = kExhausted;
-  store_soa(a.y_final, n, j, y);
+  {
+    double v[kSlots];
+    y.get(v);
+    store_soa(a.y_final, n, j, v);
+  }
   if (id.leader) {
     a.n_acc[j] = acc;
     a.n_rej[j] = rej;
@@ -349,60 +474,81 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
 }
 
 // Frozen-mesh replay from (y0, omega): the states at the start of each step and y_final.
+template <class R>
 __global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) {
-  __shared__ double op[2 * kOpTable];
-  __shared__ double ks_all[6 * kSlots * kKBlock];
-  fill_operands(op, a.K);
-  const StageSlots ks(ks_all + threadIdx.x);
+  __shared__ R op[2 * kOpTable];
+  __shared__ R ks_all[6 * kSlots * kKBlock];
+  fill_operands<R>(op, a.K);
+  const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
-  double y[kSlots], om[kSlots];
-  load_soa(a.y0, n, j, y);
-  load_soa(a.omega, n, j, om);
+  Vec<R> y;
+  R om[kSlots];
+  {
+    double v[kSlots];
+    load_soa(a.y0, n, j, v);
+    y.set(v);
+    load_soa(a.omega, n, j, v);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) om[m] = static_cast<R>(v[m]);
+  }
   const int32_t na = id.valid ? a.n_acc[j] : 0;
   for (int32_t s = 0; __ballot(s < na) != 0ull; ++s) {
     const bool act = s < na;
     const double h = act ? a.tape[static_cast<int64_t>(s) * n + j] : 0.0;
-    if (act && a.states) store_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, y);
-    double yn[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) yn[m] = y[m];
-    kura_advance(yn, h, om, op, ks);
-    if (act) {
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = yn[m];
+    if (act && a.states) {
+      double v[kSlots];
+      y.get(v);
+      store_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, v);
     }
+    Vec<R> yn = y;
+    kura_advance<R>(yn, static_cast<R>(h), om, op, ks);
+    if (act) y = yn;
   }
-  if (id.valid) store_soa(a.y_final, n, j, y);
+  if (id.valid) {
+    double v[kSlots];
+    y.get(v);
+    store_soa(a.y_final, n, j, v);
+  }
 }
 
 // ys[j][q] = one step of size ts[q] - t_s from the bracketing step's start state.
+template <class R>
 __global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) {
-  __shared__ double op[2 * kOpTable];
-  __shared__ double ks_all[6 * kSlots * kKBlock];
-  fill_operands(op, a.K);
-  const StageSlots ks(ks_all + threadIdx.x);
+  __shared__ R op[2 * kOpTable];
+  __shared__ R ks_all[6 * kSlots * kKBlock];
+  fill_operands<R>(op, a.K);
+  const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
-  double y0[kSlots], yf[kSlots], om[kSlots];
+  double y0[kSlots], yf[kSlots];
+  R om[kSlots];
   load_soa(a.y0, n, j, y0);
   load_soa(a.y_final, n, j, yf);
-  load_soa(a.omega, n, j, om);
+  {
+    double v[kSlots];
+    load_soa(a.omega, n, j, v);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) om[m] = static_cast<R>(v[m]);
+  }
   for (int64_t q = 0; q < ns; ++q) {
     const int32_t s = id.valid ? a.save_step[q * n + j] : -1;
     const bool need = id.valid && s >= 0;
     double* row = a.ys + (j * ns + q) * kKD;
     if (__ballot(need) != 0ull) {
-      double y[kSlots];
+      double v[kSlots];
       double h = 0.0;
       if (need) {
-        load_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, y);
+        load_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, v);
         h = a.ts[q] - a.save_t[q * n + j];
       } else {
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = y0[m];
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) v[m] = y0[m];
       }
-      kura_advance(y, h, om, op, ks);
-      if (need) store_row(row, y);
+      Vec<R> y;
+      y.set(v);
+      kura_advance<R>(y, static_cast<R>(h), om, op, ks);
+      y.get(v);
+      if (need) store_row(row, v);
     }
     if (id.valid && s == -1) store_row(row, y0);
     if (id.valid && s == -2) store_row(row, yf);
@@ -414,65 +560,82 @@ __global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) 
 // forward recompute stores k_1..k_5 in slots 0..4; the reverse rebuilds each stage input
 // z_s from y and slots 0..s-1 (still k's), then overwrites slot s with h * zbar_s, so
 // kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) reads the slots above it (s = 5 down to
-// r + 1, the order of the generic step VJP).
-__device__ __forceinline__ void kura_step_vjp(const double* y, double h, const double* om,
-                                              double* lam, double* pb, const double* op,
-                                              const StageSlots& ks) {
+// r + 1, the order of the generic step VJP). In FP32 the step's contributions to lam and pb
+// are summed separately and added to the compensated pairs once.
+template <class R>
+__device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<R>& lam,
+                                              Vec<R>& pb, const R* op,
+                                              const StageSlots<R>& ks) {
   _Pragma("unroll 1") for (int st = 0; st <= 4; ++st) {
-    double z[kSlots], kk[kSlots];
-    stage_input(ks, y, h, st, z);
-    FKuramotoWave::f(0.0, z, om, kk, op);
+    R z[kSlots], kk[kSlots];
+    stage_input<R>(ks, y, h, st, z);
+    FKuramotoWave<R>::f(z, om, kk, op);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
   }
-  double hl[kSlots], lnew[kSlots];
+  constexpr bool comp = Prec<R>::kComp;
+  R hl[kSlots], dl[kSlots], dp[kSlots];
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-    hl[m] = h * lam[m];
-    lnew[m] = lam[m];
+    hl[m] = h * lam.hi[m];
+    dl[m] = comp ? R(0) : lam.hi[m];
+    dp[m] = R(0);
   }
   _Pragma("unroll 1") for (int st = 5; st >= 0; --st) {
-    double kb[kSlots], z[kSlots], zb[kSlots];
-    const double bw = stage_w(6, st);
+    R kb[kSlots], z[kSlots], zb[kSlots];
+    const R bw = stage_w<R>(6, st);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = bw * hl[m];
     if (st < 5) {
-      double part[kSlots];
-      const double w5 = stage_w(5, st);
+      R part[kSlots];
+      const R w5 = stage_w<R>(5, st);
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = w5 * ks.at(5, m);
       _Pragma("unroll 1") for (int r = 4; r > st; --r) {
-        const double w = stage_w(r, st);
+        const R w = stage_w<R>(r, st);
         GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = part[m] + w * ks.at(r, m);
       }
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = kb[m] + part[m];
     }
-    stage_input(ks, y, h, st, z);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) zb[m] = 0.0;
-    FKuramotoWave::vjp(0.0, z, om, kb, zb, pb, op);
+    stage_input<R>(ks, y, h, st, z);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) zb[m] = R(0);
+    FKuramotoWave<R>::vjp(z, kb, zb, comp ? dp : pb.hi, op);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-      lnew[m] += zb[m];
+      dl[m] += zb[m];
       if (st > 0) ks.at(st, m) = h * zb[m];
     }
   }
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] = lnew[m];
+  if constexpr (comp) {
+    lam.add(dl);
+    pb.add(dp);
+  } else {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam.hi[m] = dl[m];
+  }
 }
 
 // Adjoint with saveat cotangents. One loop with a single step-VJP site: its first nsave
 // iterations take, for every save time bracketed by step s, the VJP of the re-step
 // (mu = (d ys[q] / d y_s)^T ysbar[q] into the mu buffer; omega_bar accumulates); the remaining
 // iterations are the reverse sweep over the recorded states, adding mu at each bracketing step.
+template <class R>
 __global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
-  __shared__ double op[2 * kOpTable];
-  __shared__ double ks_all[6 * kSlots * kKBlock];
-  fill_operands(op, a.K);
-  const StageSlots ks(ks_all + threadIdx.x);
+  __shared__ R op[2 * kOpTable];
+  __shared__ R ks_all[6 * kSlots * kKBlock];
+  fill_operands<R>(op, a.K);
+  const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
   const int64_t j = id.j;
-  double om[kSlots], pb[kSlots], lam[kSlots];
-  load_soa(a.omega, n, j, om);
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) pb[m] = 0.0;
+  R om[kSlots];
+  Vec<R> pb, lam;
+  {
+    double v[kSlots];
+    load_soa(a.omega, n, j, v);
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) om[m] = static_cast<R>(v[m]);
+  }
+  pb.zero();
   if (a.yfbar) {
-    load_soa(a.yfbar, n, j, lam);
+    double v[kSlots];
+    load_soa(a.yfbar, n, j, v);
+    lam.set(v);
   } else {
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] = 0.0;
+    lam.zero();
   }
   const int64_t nq = a.ysbar ? ns : 0;
   int64_t qtop = ns - 1;  // highest save index bracketed by a recorded step
@@ -490,36 +653,45 @@ __global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
       if (id.valid && s == -2) {
         double v[kSlots];
         load_row(a.ysbar + (j * ns + i) * kKD, v);
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] += v[m];
+        lam.add_f64(v);
       }
       if (__ballot(act) == 0ull) continue;
     } else {
       s = na - 1 - static_cast<int32_t>(i - nq);
       act = s >= 0;
     }
-    double y[kSlots], v[kSlots];
+    R y[kSlots];
+    Vec<R> v;
     double h = 0.0;
     if (act) {
-      load_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, y);
+      double st[kSlots];
+      load_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, st);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = static_cast<R>(st[m]);
       h = ph1 ? a.ts[i] - a.save_t[i * n + j] : a.tape[static_cast<int64_t>(s) * n + j];
     } else {
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = 0.0;
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) y[m] = R(0);
     }
     if (ph1) {
       if (act) {
-        load_row(a.ysbar + (j * ns + i) * kKD, v);
+        double w[kSlots];
+        load_row(a.ysbar + (j * ns + i) * kKD, w);
+        v.set(w);
       } else {
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) v[m] = 0.0;
+        v.zero();
       }
     } else {
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) v[m] = lam[m];
+      v = lam;
     }
-    kura_step_vjp(y, h, om, v, pb, op, ks);
+    kura_step_vjp<R>(y, static_cast<R>(h), om, v, pb, op, ks);
     if (ph1) {
-      if (act) store_row(a.mu + (j * ns + i) * kKD, v);
+      if (act) {
+        double w[kSlots];
+        v.get(w);
+        store_row(a.mu + (j * ns + i) * kKD, w);
+      }
       continue;
     }
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] = v[m];
+    lam = v;
     if (act && a.ysbar) {
       // the saves bracketed by step s, added in increasing q as the oracle does
       int64_t lo = qtop;
@@ -527,26 +699,29 @@ __global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
       for (int64_t q = lo + 1; q <= qtop; ++q) {
         double w[kSlots];
         load_row(a.mu + (j * ns + q) * kKD, w);
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] += w[m];
+        lam.add_f64(w);
       }
       qtop = lo;
     }
   }
   if (!id.valid) return;
+  double out[kSlots];
+  lam.get(out);
   if (a.ysbar) {
     // saves at or before t0 are y0 itself: their cotangents, summed in increasing q
     double x[kSlots];
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) x[m] = 0.0;
     for (int64_t q = 0; q < ns; ++q) {
       if (a.save_step[q * n + j] != -1) continue;
-      double v[kSlots];
-      load_row(a.ysbar + (j * ns + q) * kKD, v);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) x[m] += v[m];
+      double w[kSlots];
+      load_row(a.ysbar + (j * ns + q) * kKD, w);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) x[m] += w[m];
     }
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) lam[m] += x[m];
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) out[m] += x[m];
   }
-  store_soa(a.y0bar, n, j, lam);
-  store_soa(a.omega_bar, n, j, pb);
+  store_soa(a.y0bar, n, j, out);
+  pb.get(out);
+  store_soa(a.omega_bar, n, j, out);
 }
 
 unsigned grid_for(int64_t n) {
@@ -556,20 +731,36 @@ unsigned grid_for(int64_t n) {
 
 }  // namespace
 
-int kura_fwd_launch(const KuraFwdArgs& a, hipStream_t s) {
-  kura_fwd_kernel<<<grid_for(a.n), kKBlock, 0, s>>>(a);
+int kura_fwd_launch(const KuraFwdArgs& a, bool fp32, hipStream_t s) {
+  if (fp32) {
+    kura_fwd_kernel<float><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  } else {
+    kura_fwd_kernel<double><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  }
   return check_launch("kura_fwd_kernel");
 }
-int kura_replay_launch(const KuraReplayArgs& a, hipStream_t s) {
-  kura_replay_kernel<<<grid_for(a.n), kKBlock, 0, s>>>(a);
+int kura_replay_launch(const KuraReplayArgs& a, bool fp32, hipStream_t s) {
+  if (fp32) {
+    kura_replay_kernel<float><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  } else {
+    kura_replay_kernel<double><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  }
   return check_launch("kura_replay_kernel");
 }
-int kura_saveat_launch(const KuraSaveatArgs& a, hipStream_t s) {
-  kura_saveat_kernel<<<grid_for(a.n), kKBlock, 0, s>>>(a);
+int kura_saveat_launch(const KuraSaveatArgs& a, bool fp32, hipStream_t s) {
+  if (fp32) {
+    kura_saveat_kernel<float><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  } else {
+    kura_saveat_kernel<double><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  }
   return check_launch("kura_saveat_kernel");
 }
-int kura_adj_launch(const KuraAdjArgs& a, hipStream_t s) {
-  kura_adj_kernel<<<grid_for(a.n), kKBlock, 0, s>>>(a);
+int kura_adj_launch(const KuraAdjArgs& a, bool fp32, hipStream_t s) {
+  if (fp32) {
+    kura_adj_kernel<float><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  } else {
+    kura_adj_kernel<double><<<grid_for(a.n), kKBlock, 0, s>>>(a);
+  }
   return check_launch("kura_adj_kernel");
 }
 

@@ -23,9 +23,6 @@ from .base import SolveResult
 
 
 def _check(name, b, precision):
-    if precision != "float64":
-        raise ValueError(f"precision={precision!r} with saveat: the dense lane is built in "
-                         "float64 only; use precision='float64'")
     if b.field_id != N.FIELD_KURAMOTO or name != D.HIP_KURAMOTO:
         raise ValueError(
             f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
@@ -44,7 +41,7 @@ def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
     K = torch.as_tensor(b.aux_of(problem), device=dev)
     ts_d = torch.as_tensor(ts, device=dev)
     rec = KU.record(K, y0_soa, om_soa, t0=problem.t0, t1=problem.t1, rtol=rtol, atol=atol,
-                    ts=ts_d, max_steps=max_steps)
+                    ts=ts_d, max_steps=max_steps, fp32=precision == "float32")
     ys = KU.saveat(rec)
     status = rec.status.cpu().numpy()
     y_final = rec.y_final.t().cpu().numpy().copy()
@@ -64,4 +61,5 @@ def make_saveat_closure(name, b, problem, y0, params, saveat, *, wrt, rtol, atol
     if aux is None:
         aux = torch.as_tensor(b.aux_of(problem), device=device)
     return KuramotoClosure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
-                           device=device, aux=aux, ts=ts, max_steps=max_steps)
+                           device=device, aux=aux, ts=ts, max_steps=max_steps,
+                           fp32=precision == "float32")

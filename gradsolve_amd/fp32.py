@@ -21,20 +21,24 @@ from . import hip_tsit5 as H
 from .base import SolveResult
 from .closure import make_tsit5_f32_closure
 
-F32_CAPABLE = frozenset({D.HIP_TSIT5, D.HIP_REPLAY})
+F32_CAPABLE = frozenset({D.HIP_TSIT5, D.HIP_REPLAY, D.HIP_KURAMOTO})
 
 
 def reject_incapable(name: str) -> None:
     if name not in F32_CAPABLE:
         raise ValueError(
             f"precision='float32' is not supported on lane {name!r}: the FP32 variant is built "
-            "for the non-stiff Tsit5 lane only. Use precision='float64'.")
+            "for the non-stiff Tsit5 lanes (hip_tsit5, hip_replay, hip_kuramoto) only. Use "
+            "precision='float64'.")
 
 
 def solve(b, problem, y0, params, *, rtol, atol, device, name=D.HIP_TSIT5,
           max_steps=100000) -> SolveResult:
-    from .api import _device
+    from .api import _device, _solve_kuramoto
     reject_incapable(name)
+    if name == D.HIP_KURAMOTO:
+        return _solve_kuramoto(problem, y0, params, rtol=rtol, atol=atol, device=device,
+                               max_steps=max_steps, fp32=True)
     dev = _device(device)
     y0_t = torch.as_tensor(np.asarray(y0, np.float64), device=dev) \
         if not isinstance(y0, torch.Tensor) else y0.to(dev, torch.float64)
@@ -57,5 +61,9 @@ def solve(b, problem, y0, params, *, rtol, atol, device, name=D.HIP_TSIT5,
 def make_closure(name, b, problem, y0, params, *, wrt, rtol, atol, device, aux=None,
                  ckpt_interval=None):
     reject_incapable(name)
+    if name == D.HIP_KURAMOTO:
+        from .closure import KuramotoClosure
+        return KuramotoClosure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                               device=device, aux=aux, fp32=True)
     return make_tsit5_f32_closure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                                   device=device, ckpt_interval=ckpt_interval, aux=aux)

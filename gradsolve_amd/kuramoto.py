@@ -62,6 +62,7 @@ class KuramotoRecord:
     states: torch.Tensor     # [cap, n, 32]
     save_step: Optional[torch.Tensor]
     save_t: Optional[torch.Tensor]
+    flags: int = 0           # 0 (FP64) or N.FP32
 
     @property
     def nsave(self) -> int:
@@ -79,7 +80,8 @@ class KuramotoRecord:
                            torch.zeros((), dtype=t.dtype, device=t.device)).t().contiguous()
 
 
-def forward_only(K, y0_soa, om_soa, *, t0, t1, rtol, atol, max_steps=100000, stream=None):
+def forward_only(K, y0_soa, om_soa, *, t0, t1, rtol, atol, max_steps=100000, stream=None,
+                 fp32=False):
     """Forward without a record: (y_final [32, n], n_acc, n_rej, status)."""
     n = y0_soa.shape[1]
     dev = y0_soa.device
@@ -89,11 +91,13 @@ def forward_only(K, y0_soa, om_soa, *, t0, t1, rtol, atol, max_steps=100000, str
     status = torch.empty(n, dtype=torch.int32, device=dev)
     N.call("gsx_kuramoto_fwd", n, D, _ptr(K), _ptr(y0_soa), _ptr(om_soa), float(t0),
            float(t1), float(rtol), float(atol), int(max_steps), None, 0, _ptr(y_final),
-           _ptr(n_acc), _ptr(n_rej), _ptr(status), None, 0, None, None, None, _stream(stream))
+           _ptr(n_acc), _ptr(n_rej), _ptr(status), None, 0, None, None, None,
+           N.FP32 if fp32 else 0, _stream(stream))
     return y_final, n_acc, n_rej, status
 
 
-def estimate_cap(K, y0_soa, om_soa, *, t0, t1, rtol, atol, sample=2048, seed=0) -> int:
+def estimate_cap(K, y0_soa, om_soa, *, t0, t1, rtol, atol, sample=2048, seed=0,
+                 fp32=False) -> int:
     """Tape rows for the ensemble: 1.25 x the largest accepted-step count on a random
     subsample, plus 16 (a trajectory that still does not fit grows the tape)."""
     n = y0_soa.shape[1]
@@ -101,18 +105,21 @@ def estimate_cap(K, y0_soa, om_soa, *, t0, t1, rtol, atol, sample=2048, seed=0) 
     g = torch.Generator(device="cpu").manual_seed(seed)
     idx = torch.randperm(n, generator=g)[:m].to(y0_soa.device)
     _, na, _, _ = forward_only(K, y0_soa[:, idx].contiguous(), om_soa[:, idx].contiguous(),
-                               t0=t0, t1=t1, rtol=rtol, atol=atol)
+                               t0=t0, t1=t1, rtol=rtol, atol=atol, fp32=fp32)
     return int(1.25 * int(na.max().item())) + 16 if m else 16
 
 
 def record(K, y0_soa, om_soa, *, t0, t1, rtol, atol, ts=None, max_steps=100000,
-           tape_cap=None, grow=True, stream=None) -> KuramotoRecord:
+           tape_cap=None, grow=True, stream=None, fp32=False) -> KuramotoRecord:
     """Forward + record (tape, states, save brackets). ``ts`` is a device float64 tensor of
-    validated save times or None. Grows the tape on status 3."""
+    validated save times or None. Grows the tape on status 3. ``fp32`` selects the FP32-MFMA
+    variant (FP32 stages and coupling, compensated state, FP64 t, h, norm and controller)."""
+    flags = N.FP32 if fp32 else 0
     n = y0_soa.shape[1]
     dev = y0_soa.device
     if tape_cap is None:
-        tape_cap = estimate_cap(K, y0_soa, om_soa, t0=t0, t1=t1, rtol=rtol, atol=atol)
+        tape_cap = estimate_cap(K, y0_soa, om_soa, t0=t0, t1=t1, rtol=rtol, atol=atol,
+                                fp32=fp32)
     cap = int(max(1, min(tape_cap, max_steps)))
     k = 0 if ts is None else int(ts.numel())
     while True:
@@ -127,9 +134,9 @@ def record(K, y0_soa, om_soa, *, t0, t1, rtol, atol, ts=None, max_steps=100000,
         N.call("gsx_kuramoto_fwd", n, D, _ptr(K), _ptr(y0_soa), _ptr(om_soa), float(t0),
                float(t1), float(rtol), float(atol), int(max_steps), _ptr(ts), k,
                _ptr(y_final), _ptr(n_acc), _ptr(n_rej), _ptr(status), _ptr(tape), cap,
-               _ptr(states), _ptr(save_step), _ptr(save_t), _stream(stream))
+               _ptr(states), _ptr(save_step), _ptr(save_t), flags, _stream(stream))
         rec = KuramotoRecord(n, K, y0_soa, om_soa, float(t0), float(t1), ts, y_final, n_acc,
-                             n_rej, status, tape, cap, states, save_step, save_t)
+                             n_rej, status, tape, cap, states, save_step, save_t, flags)
         if not grow or cap >= max_steps or n == 0:
             return rec
         if not bool((status == 3).any().item()):
@@ -143,7 +150,8 @@ def replay(rec: KuramotoRecord, y0_soa, om_soa, stream=None):
     y_final = torch.empty_like(rec.y_final)
     states = torch.empty_like(rec.states)
     N.call("gsx_kuramoto_replay", rec.n, D, _ptr(rec.K), _ptr(y0_soa), _ptr(om_soa),
-           _ptr(rec.tape), _ptr(rec.n_acc), _ptr(y_final), _ptr(states), _stream(stream))
+           _ptr(rec.tape), _ptr(rec.n_acc), _ptr(y_final), _ptr(states), rec.flags,
+           _stream(stream))
     return y_final, states
 
 
@@ -159,7 +167,7 @@ def saveat(rec: KuramotoRecord, y0_soa=None, om_soa=None, y_final=None, states=N
                                                  device=rec.y0.device)
     N.call("gsx_kuramoto_saveat", rec.n, D, _ptr(rec.K), _ptr(y0_soa), _ptr(om_soa),
            _ptr(y_final), _ptr(states), _ptr(rec.ts), k, _ptr(rec.save_step),
-           _ptr(rec.save_t), _ptr(ys), _stream(stream))
+           _ptr(rec.save_t), _ptr(ys), rec.flags, _stream(stream))
     return ys
 
 
@@ -184,5 +192,5 @@ def adjoint(rec: KuramotoRecord, *, ysbar=None, yfbar=None, om_soa=None, states=
     N.call("gsx_kuramoto_adjoint", rec.n, D, _ptr(rec.K), _ptr(om_soa), _ptr(rec.tape),
            _ptr(rec.n_acc), _ptr(states), _ptr(rec.ts), rec.nsave, _ptr(rec.save_step),
            _ptr(rec.save_t), _ptr(ysbar), _ptr(yfbar), _ptr(mu), _ptr(y0bar), _ptr(ombar),
-           _stream(stream))
+           rec.flags, _stream(stream))
     return y0bar, ombar

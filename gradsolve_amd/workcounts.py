@@ -20,3 +20,17 @@ LORENZ_ADJ_IO_BYTES = 8 + 24 + 24 + 24 + 8   # rho, ybar in (+ checkpoint 0 = y0
 # MI355X published peaks.
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+
+
+# HBM bytes per launch of the headline workload's kernels (N = 2^20 Lorenz, rtol 1e-8), by
+# checkpoint interval: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same
+# counters read for a 1 GiB device copy (FETCH_SIZE 0.5 GiB -> x2.0; WRITE_SIZE 1 GiB -> x1.0).
+# Algorithmic bytes: the forward reads y0 and p and writes the tape (8 B per accepted step), a
+# checkpoint (24 B) at the start and every C steps, y_final and the counts; the adjoint reads
+# them back with the cotangent and writes the two gradients.
+HEADLINE_TRAFFIC_BYTES = {
+    # C: {kernel: (measured bytes, algorithmic bytes)}
+    4: {"tsit5_fwd_kernel": (1.806e9, 1.738e9), "tsit5_adj_kernel": (4.640e9, 1.738e9)},
+    8: {"tsit5_fwd_kernel": (1.421e9, 1.386e9), "tsit5_adj_kernel": (2.172e9, 1.386e9)},
+    16: {"tsit5_fwd_kernel": (1.239e9, 1.211e9), "tsit5_adj_kernel": (1.665e9, 1.211e9)},
+}

@@ -193,6 +193,10 @@ int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* 
  * [nsave][n]; ys, ysbar, mu [n][nsave][32]. ts[nsave] is sorted non-decreasing (validated by
  * the caller as validate_saveat does, dense.py:81-103).
  *
+ * flags: 0 (FP64, coupling on FP64 MFMA) or GSX_FP32 (FP32 stages and coupling on FP32 MFMA,
+ * Kahan-compensated state and adjoint accumulators, FP64 t, h, norm, controller and buffers);
+ * the four calls on one record must use the same flags.
+ *
  * gsx_kuramoto_fwd: adaptive Tsit5 forward (as gsx_tsit5_fwd; t0 and t1 shared). With tape
  * non-NULL it records the accepted steps and, when states is non-NULL, every accepted step's
  * start state; with nsave > 0 it records for each save time q the step s bracketing it
@@ -203,20 +207,21 @@ int gsx_kuramoto_fwd(int64_t n, int D, const double* K, const double* y0, const 
                      double t0, double t1, double rtol, double atol, int64_t max_steps,
                      const double* ts, int64_t nsave, double* y_final, int32_t* n_acc,
                      int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
-                     double* states, int32_t* save_step, double* save_t, void* stream);
+                     double* states, int32_t* save_step, double* save_t, int flags,
+                     void* stream);
 
 /* Frozen-mesh replay from (y0, omega) over a recorded tape: y_final and, when non-NULL, the
  * states (the save brackets depend on the mesh only and stay valid). */
 int gsx_kuramoto_replay(int64_t n, int D, const double* K, const double* y0,
                         const double* omega, const double* tape, const int32_t* n_acc,
-                        double* y_final, double* states, void* stream);
+                        double* y_final, double* states, int flags, void* stream);
 
 /* Saved states ys[j][q]: one Tsit5 step of size ts[q] - save_t from the bracketing step's
  * start state, y0 for save_step -1, y_final for -2 (dense.py:106-163). */
 int gsx_kuramoto_saveat(int64_t n, int D, const double* K, const double* y0,
                         const double* omega, const double* y_final, const double* states,
                         const double* ts, int64_t nsave, const int32_t* save_step,
-                        const double* save_t, double* ys, void* stream);
+                        const double* save_t, double* ys, int flags, void* stream);
 
 /* Adjoint of (y_final, ys) with respect to (y0, omega) for cotangents yfbar (may be NULL) and
  * ysbar (may be NULL): y0bar, omega_bar [32][n]. mu [n][nsave][32] is scratch the caller
@@ -225,7 +230,8 @@ int gsx_kuramoto_adjoint(int64_t n, int D, const double* K, const double* omega,
                          const double* tape, const int32_t* n_acc, const double* states,
                          const double* ts, int64_t nsave, const int32_t* save_step,
                          const double* save_t, const double* ysbar, const double* yfbar,
-                         double* mu, double* y0bar, double* omega_bar, void* stream);
+                         double* mu, double* y0bar, double* omega_bar, int flags,
+                         void* stream);
 
 /* The naive reverse sweep, measured by the benchmark as the plain arm (the fastest "plain"
  * drop-in: an FP64 HIP forward plus a tape-based reverse sweep with no checkpoints). Not part of

@@ -73,6 +73,10 @@ void lh_sincos(int64_t n, const double* x, double* s, double* c) {
   for (int64_t i = 0; i < n; ++i) gsx_sincos(x[i], s + i, c + i);
 }
 
+void lh_sincosf(int64_t n, const float* x, float* s, float* c) {
+  for (int64_t i = 0; i < n; ++i) gsx_sincosf(x[i], s + i, c + i);
+}
+
 double lh_ctrl_pow(int which, double x) {
   return which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
 }

@@ -182,3 +182,53 @@ def test_ragged_sizes_and_save_edges(gpu, n):
                                        aux=e.aux)
     assert _rowrel(gom.t().cpu().numpy(), ogom).max() < GRAD_REL
     assert _rowrel(g0.t().cpu().numpy(), og0).max() < GRAD_REL
+
+
+def test_fp32_mfma_variant_error_against_fp64(gpu):
+    """FP32 variant (FP32 stages and coupling on FP32 MFMA, compensated state, FP64 t, h, norm
+    and controller): final states and saves against the FP64 oracle's own solution, and the
+    gradient against the FP64 oracle adjoint on the FP32-recorded mesh (the reference's f32
+    acceptance, tests/test_f32_replay.py:88-94: rel-L2 <= 1e-3, cosine >= 1 - 1e-6). The errors
+    are printed; the FP64 oracle's discretization error (rtol 1e-8 against rtol 1e-11) is
+    printed beside them as the scale."""
+    n = 256
+    print(f"case kuramoto fp32 n={n}", flush=True)
+    e, K, y0, om = _setup(n, 4, gpu)
+    tsd = torch.as_tensor(TS, device=gpu)
+    rec = KU.record(K, y0, om, t0=0.0, t1=10.0, rtol=1e-8, atol=1e-8, ts=tsd, fp32=True)
+    ys = KU.saveat(rec)
+    torch.cuda.synchronize()
+    assert int((rec.status != 0).sum().item()) == 0
+    na = rec.n_acc.cpu().numpy()
+    oyf, otape, ona, _, _ = O.tsit5_record(O.KURAMOTO, e.y0, e.params, 0.0, 10.0, rtol=1e-8,
+                                           atol=1e-8, aux=e.aux)
+    _, oys = O.tsit5_replay_saveat(O.KURAMOTO, e.y0, e.params, 0.0, otape, TS, aux=e.aux)
+    fyf, ftape, _, _, _ = O.tsit5_record(O.KURAMOTO, e.y0, e.params, 0.0, 10.0, rtol=1e-11,
+                                         atol=1e-11, aux=e.aux)
+    yf32 = rec.y_final.t().cpu().numpy()
+    r_state = _rowrel(yf32, oyf)
+    r_saves = _rowrel(ys.cpu().numpy(), oys)
+    r_disc = _rowrel(oyf, fyf)
+    print(f"FP32 vs FP64 oracle: final state max {r_state.max():.2e} (median "
+          f"{np.median(r_state):.2e}), saves max {r_saves.max():.2e}; FP64 rtol 1e-8 vs 1e-11: "
+          f"max {r_disc.max():.2e}; accepted steps FP32 {na.mean():.1f} vs FP64 {ona.mean():.1f}",
+          flush=True)
+    assert r_state.max() < 1e-4 and r_saves.max() < 1e-4
+    rng = np.random.default_rng(6)
+    ysn = ys.cpu().numpy()
+    ysbar = 2.0 * (ysn - rng.standard_normal(ysn.shape)) / ysn.size
+    g0, gom = KU.adjoint(rec, ysbar=torch.as_tensor(ysbar, device=gpu))
+    tape = rec.padded_tape().cpu().numpy()
+    _, ogom = O.tsit5_adjoint_saveat(O.KURAMOTO, e.y0, e.params, 0.0, tape, TS, ysbar,
+                                     aux=e.aux)
+    g = gom.t().cpu().numpy().ravel()
+    og = ogom.ravel()
+    rel_l2 = np.linalg.norm(g - og) / np.linalg.norm(og)
+    cos = g @ og / (np.linalg.norm(g) * np.linalg.norm(og))
+    print(f"FP32 omega gradient vs FP64 adjoint on the FP32 mesh: rel-L2 {rel_l2:.2e}, cosine "
+          f"1-{1 - cos:.1e}", flush=True)
+    assert rel_l2 <= 1e-3 and cos >= 1 - 1e-6
+    # the public route
+    cl = gs.grad_closure(P.Kuramoto(e.aux), e.y0[:8], e.params[:8], wrt="params", saveat=TS,
+                         rtol=1e-8, atol=1e-8, precision="float32", device="cuda")
+    assert cl.route.actual == "hip_kuramoto" and cl.rec.flags != 0

@@ -132,6 +132,20 @@ def test_fp32_error_against_fp64_oracle(gpu):
     # tolerance, in the bulk and in the tail.
     assert np.median(r32) < np.median(r64)
     assert np.percentile(r32, 90) < np.percentile(r64, 90)
+    # Gradients: the FP32 adjoint against the FP64 oracle adjoint on the FP32-recorded mesh
+    # (same steps, so only the arithmetic differs), cotangent 2 * y_final.
+    torch.mul(eng.y_final, 2.0, out=eng.ybar)
+    _, gp = eng.adjoint(p, eng.ybar, t0=0.0)
+    torch.cuda.synchronize()
+    tape = eng.padded_tape().cpu().numpy()
+    _, ogp = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, tape, 2.0 * yf32)
+    g32 = gp.cpu().numpy()[0]
+    rg = np.abs(g32 - ogp[:, 0]) / np.abs(ogp[:, 0])
+    short = e.t1_per < 2.0
+    print(f"FP32 gradient vs FP64 oracle on the FP32 mesh: median {np.median(rg):.2e} p90 "
+          f"{np.percentile(rg, 90):.2e} max {rg.max():.2e}; t1<2: max {rg[short].max():.2e}",
+          flush=True)
+    assert np.median(rg) < 1e-4 and rg[short].max() < 1e-2
 
 
 def test_capacity_growth(gpu):

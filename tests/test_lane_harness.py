@@ -147,3 +147,28 @@ def test_sincos_within_two_ulp():
     assert us.max() <= 2.0 and uc.max() <= 2.0
     big = np.abs(xs) >= 2.0 ** 20
     assert np.array_equal(s[big], rs[big]) and np.array_equal(c[big], rc[big])
+
+
+def test_sincosf_within_two_ulp():
+    """The FP32 device sincos (host build of the same source) against sin and cos computed in
+    FP64 and rounded to FP32: within 2 FP32 ulp over |x| < 2^14."""
+    import math
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
+    F = ctypes.CDLL(os.path.join(HERE, "native_cpu", "liblane_harness_fast.so"))
+    F.lh_sincosf.restype = None
+    F.lh_sincosf.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(12)
+    k = rng.integers(-10000, 10000, 20000).astype(np.float64)
+    near = (k * (np.pi / 2) + rng.uniform(-1e-3, 1e-3, k.size))
+    xs = np.concatenate([rng.uniform(-40.0, 40.0, 200000), rng.uniform(-16000, 16000, 50000),
+                         near, rng.uniform(-1e-6, 1e-6, 1000), [0.0, -0.0]]).astype(np.float32)
+    xs = np.ascontiguousarray(xs)
+    s = np.empty_like(xs)
+    c = np.empty_like(xs)
+    F.lh_sincosf(xs.size, xs.ctypes.data, s.ctypes.data, c.ctypes.data)
+    rs = np.array([math.sin(float(v)) for v in xs]).astype(np.float32)
+    rc = np.array([math.cos(float(v)) for v in xs]).astype(np.float32)
+    us = np.abs(s.astype(np.float64) - rs) / np.spacing(np.abs(rs))
+    uc = np.abs(c.astype(np.float64) - rc) / np.spacing(np.abs(rc))
+    print(f"sincosf: max ulp sin {us.max():.2f}, cos {uc.max():.2f}")
+    assert us.max() <= 2.0 and uc.max() <= 2.0
