@@ -119,8 +119,8 @@ def test_fp32_error_against_fp64_oracle(gpu):
     yf32 = eng.y_final.cpu().numpy().T
     oyf, otape, ona, onr, _ = O.tsit5_record(O.LORENZ, e.y0, e.params, 0.0, e.t1_per,
                                              rtol=1e-6, atol=1e-6)
-    fine, *_ = O.tsit5_record(O.LORENZ, e.y0, e.params, 0.0, e.t1_per, rtol=1e-9, atol=1e-9,
-                              max_steps=10 ** 6, keep_tape=False)
+    fine, ftape, *_ = O.tsit5_record(O.LORENZ, e.y0, e.params, 0.0, e.t1_per, rtol=1e-9,
+                                     atol=1e-9, max_steps=10 ** 6, cap=4000)
     r32 = np.max(np.abs(yf32 - oyf) / np.maximum(np.abs(oyf), 1e-12), axis=1)
     r64 = np.max(np.abs(oyf - fine) / np.maximum(np.abs(fine), 1e-12), axis=1)
     short = e.t1_per < 2.0
@@ -132,20 +132,27 @@ def test_fp32_error_against_fp64_oracle(gpu):
     # tolerance, in the bulk and in the tail.
     assert np.median(r32) < np.median(r64)
     assert np.percentile(r32, 90) < np.percentile(r64, 90)
-    # Gradients: the FP32 adjoint against the FP64 oracle adjoint on the FP32-recorded mesh
-    # (same steps, so only the arithmetic differs), cotangent 2 * y_final.
+    # Gradients of sum(y_final^2) w.r.t. rho: the FP32 adjoint against the FP64 oracle adjoint
+    # on the FP32-recorded mesh (same steps: only the arithmetic differs), beside the FP64
+    # gradient's own discretization error (rtol 1e-6 against 1e-9 meshes). The trajectories are
+    # chaotic up to t1 = 10 with rho up to 60, so both tails are large.
     torch.mul(eng.y_final, 2.0, out=eng.ybar)
     _, gp = eng.adjoint(p, eng.ybar, t0=0.0)
     torch.cuda.synchronize()
     tape = eng.padded_tape().cpu().numpy()
     _, ogp = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, tape, 2.0 * yf32)
+    _, ogp64 = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, otape, 2.0 * oyf)
+    _, ogpf = O.tsit5_adjoint(O.LORENZ, e.y0, e.params, 0.0, ftape, 2.0 * fine)
     g32 = gp.cpu().numpy()[0]
     rg = np.abs(g32 - ogp[:, 0]) / np.abs(ogp[:, 0])
-    short = e.t1_per < 2.0
+    rg64 = np.abs(ogp64[:, 0] - ogpf[:, 0]) / np.abs(ogpf[:, 0])
     print(f"FP32 gradient vs FP64 oracle on the FP32 mesh: median {np.median(rg):.2e} p90 "
           f"{np.percentile(rg, 90):.2e} max {rg.max():.2e}; t1<2: max {rg[short].max():.2e}",
           flush=True)
-    assert np.median(rg) < 1e-4 and rg[short].max() < 1e-2
+    print(f"FP64 gradient rtol 1e-6 vs rtol 1e-9 (discretization): median "
+          f"{np.median(rg64):.2e} p90 {np.percentile(rg64, 90):.2e} max {rg64.max():.2e}",
+          flush=True)
+    assert np.median(rg) < np.median(rg64) and np.percentile(rg, 90) < np.percentile(rg64, 90)
 
 
 def test_capacity_growth(gpu):
