@@ -114,15 +114,18 @@ GSX_HD float fast_log2f(float x) { return std::log2(x); }
 GSX_HD float fast_exp2f(float x) { return std::exp2(x); }
 #endif
 
-// x^(-1/m) * exp(-c ln x) for positive finite x, m in {3, 5}: see above.
+// x^(-1/m) * exp(-c ln x) for positive finite x, m in {3, 5, 10}: see above. For m = 10 the
+// input is a squared norm and the clamp is squared with it.
 template <int M>
 GSX_HD double ctrl_pow_newton(double x, double c) {
-  const double xc = x < 1e-5 ? 1e-5 : (x > 1e4 ? 1e4 : x);
+  constexpr double lo = M == 10 ? 1e-10 : 1e-5, hi = M == 10 ? 1e8 : 1e4;
+  const double xc = x < lo ? lo : (x > hi ? hi : x);
   const float lf = fast_log2f(static_cast<float>(xc));
   double y = static_cast<double>(fast_exp2f(lf * (-1.0f / M)));
   GSX_UNROLL for (int it = 0; it < 2; ++it) {
     const double y2 = y * y;
-    const double ym = (M == 5) ? y2 * y2 * y : y2 * y;
+    const double y5 = y2 * y2 * y;
+    const double ym = (M == 10) ? y5 * y5 : (M == 5) ? y5 : y2 * y;
     const double r = std::fma(xc, ym, -1.0);
     y = std::fma(y * (-1.0 / M), r, y);
   }
@@ -139,6 +142,13 @@ GSX_HD double ctrl_pow_tsit5(double x) {
   if (!(x < 1e300)) return std::pow(x, -0.2);  // inf (and NaN) keep the library's result
   return ctrl_pow_newton<5>(x, 1.1102230246251565e-17);
 #endif
+}
+
+// m^(-0.1) for the squared norm m = err^2, equal to err^(-0.2) with -0.2 in binary64:
+// m^(-1/10) * exp(-5.551115123125783e-18 * ln m) (half the correction above).
+GSX_HD double ctrl_pow_tsit5_sq(double m) {
+  if (!(m < 1e300)) return std::pow(m, -0.1);  // inf (and NaN) keep the library's result
+  return ctrl_pow_newton<10>(m, 5.551115123125783e-18);
 }
 
 // x^(-1/3) with -1/3 in binary64: the double nearest 1/3 falls short of it by
@@ -454,6 +464,51 @@ GSX_HD double wrms(const double* e, const double* y, const double* ynew, double 
 }
 
 // ---------------------------------------------------------------------------------------
+// Tsit5 error measure and step factor (gradsolve/solvers/method.py:85-108). The strict
+// variant follows the reference: err = sqrt(mean q^2) (the WRMS norm), accept iff err <= 1,
+// factor 0.9 * max(err, 1e-10)^-0.2. The default variant carries m = mean q^2 = err^2
+// instead and never forms the square root: m <= 1 accepts the same steps (the root is
+// monotone and 1 at 1; the one exception is the double just above 1, whose rounded root is
+// 1), and the factor m^(-0.1) (ctrl_pow_tsit5_sq) equals err^(-0.2) to within the
+// controller power's own rounding. The mean divides by D with one Markstein correction
+// (q = s/D to within an ulp) instead of the IEEE division sequence.
+// ---------------------------------------------------------------------------------------
+template <int D>
+GSX_HD double wrms_sq(const double* e, const double* y, const double* ynew, double rtol,
+                      double atol) {
+  double q[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) {
+    const double sc = atol + rtol * py_max(std::fabs(y[i]), std::fabs(ynew[i]));
+    const double r = norm_div(e[i], sc);
+    q[i] = r * r;
+  }
+  const double s = np_sum<D>(q);
+  constexpr double inv = 1.0 / D;
+  if constexpr ((D & (D - 1)) == 0) return s * inv;
+  const double m = s * inv;
+  return std::fma(std::fma(-m, double(D), s), inv, m);
+}
+
+template <int D>
+GSX_HD double tsit5_err(const double* e, const double* y, const double* ynew, double rtol,
+                        double atol) {
+#if defined(GSX_STRICT_ARITH)
+  return wrms<D>(e, y, ynew, rtol, atol);
+#else
+  return wrms_sq<D>(e, y, ynew, rtol, atol);
+#endif
+}
+
+// Step-size factor 0.9 * err^-0.2 from tsit5_err's measure (clamped by the caller).
+GSX_HD double tsit5_fac(double em) {
+#if defined(GSX_STRICT_ARITH)
+  return kSafety * ctrl_pow_tsit5(py_max(em, kErrFloor));
+#else
+  return kSafety * ctrl_pow_tsit5_sq(py_max(em, kErrFloor * kErrFloor));
+#endif
+}
+
+// ---------------------------------------------------------------------------------------
 // Tsit5 stage block. Given k1 (FSAL) computes z2..z6 / k2..k6 and y5 = y + h*sum b_i k_i.
 // Stage inputs follow tsit5_step.py:88-99: z_i = y + h*(a_i1 k1 + ... ).
 // ---------------------------------------------------------------------------------------
@@ -505,17 +560,23 @@ GSX_HD void tsit5_stages(T t, const T* y, T h, const T* p, T (&k)[7][F::D], T* y
 template <class F>
 GSX_HD constexpr int ckpt_rows() { return F::D + (F::autonomous ? 0 : 1); }
 
+// One checkpoint at `base` = &ck[slot][0][j] (rows n apart).
 template <class F>
-GSX_HD void ckpt_store(double* ck, int64_t slot, int64_t n, int64_t j, const double* y,
-                       double t) {
-  constexpr int DCK = ckpt_rows<F>();
-  double* base = ck + slot * DCK * n + j;
+GSX_HD void ckpt_put(double* base, int64_t n, const double* y, double t) {
   GSX_UNROLL for (int d = 0; d < F::D; ++d) base[d * n] = y[d];
   if constexpr (!F::autonomous) base[F::D * n] = t;
   (void)t;
 }
 
 template <class F>
+GSX_HD void ckpt_store(double* ck, int64_t slot, int64_t n, int64_t j, const double* y,
+                       double t) {
+  ckpt_put<F>(ck + slot * ckpt_rows<F>() * n + j, n, y, t);
+}
+
+// kRecCk = true is the recording launch (tape and checkpoints both given): the two flags
+// become constants and the accept path carries no runtime tests of them.
+template <class F, bool kRecCk = false>
 GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
   constexpr int D = F::D, P = F::P;
   using namespace tab;
@@ -529,10 +590,17 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
   double dt = (t1 - t) / 100.0;  // dt0 = (t1 - t0)/100 (gradsolve/warp/warp_replay.py:157)
   F::f(t, y, p, k[0], a.aux);
   int32_t acc = 0, rej = 0, status = kReached;
-  const bool rec = a.tape != nullptr;
-  const bool ck = a.ckpt != nullptr;
+  const bool rec = kRecCk || a.tape != nullptr;
+  const bool ck = kRecCk || a.ckpt != nullptr;
+  // Tape rows available (the tape-full test is one 32-bit compare; no tape: unlimited).
+  const int32_t cap = rec ? static_cast<int32_t>(py_min<int64_t>(a.tape_cap, INT32_MAX))
+                          : INT32_MAX;
   int32_t ck_left = a.ckpt_interval;
-  if (ck) ckpt_store<F>(a.ckpt, 0, n, j, y, t);
+  // Running write cursors: tape row acc and checkpoint slot acc / C of this lane.
+  double* tp = rec ? a.tape + j : nullptr;
+  double* cp = ck ? a.ckpt + j : nullptr;
+  const int64_t cstride = static_cast<int64_t>(ckpt_rows<F>()) * n;
+  if (ck) ckpt_put<F>(cp, n, y, t);
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
     const double h = py_min(dt, t1 - t);
@@ -542,7 +610,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
     GSX_UNROLL for (int i = 0; i < D; ++i)
       e[i] = h * (e1 * k[0][i] + e2 * k[1][i] + e3 * k[2][i] + e4 * k[3][i] + e5 * k[4][i] +
                   e6 * k[5][i] + e7 * k[6][i]);
-    const double err = wrms<D>(e, y, y5, a.rtol, a.atol);
+    const double err = tsit5_err<D>(e, y, y5, a.rtol, a.atol);
     if (err != err) {
       // A NaN error makes the next step size NaN; the reference loop then rejects every
       // remaining attempt (tsit5_replay.py:108-124) and ends with status 1 at the same state.
@@ -550,7 +618,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
       break;
     }
     if (err <= 1.0) {
-      if (rec && acc >= a.tape_cap) {
+      if (acc >= cap) {
         status = kTapeFull;
         break;
       }
@@ -559,16 +627,20 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
         y[d] = y5[d];
         k[0][d] = k[6][d];
       }
-      if (rec) a.tape[static_cast<int64_t>(acc) * n + j] = h;
+      if (rec) {
+        *tp = h;
+        tp += n;
+      }
       ++acc;
       if (ck && --ck_left == 0) {
-        ckpt_store<F>(a.ckpt, acc / a.ckpt_interval, n, j, y, t);
+        cp += cstride;
+        ckpt_put<F>(cp, n, y, t);
         ck_left = a.ckpt_interval;
       }
     } else {
       ++rej;
     }
-    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
+    const double fac = tsit5_fac(err);
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;
@@ -598,15 +670,14 @@ GSX_HD void tsit5_advance(double t, double* y, double h, const double* p, const 
   GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = y5[d];
 }
 
-// lam <- (d y_next / d y)^T lam ; pb += (d y_next / d p)^T lam, at the step (t, y, h).
+// Stage inputs z1..z6 of the step (t, y, h) (z1 = y), and with ynext non-null the step's
+// result y + h*(b1 k1 + ... + b6 k6), which equals tsit5_advance's.
 template <class F>
-GSX_HD void tsit5_step_vjp(double t, const double* y, double h, const double* p, double* lam,
-                           double* pb, const double* aux) {
+GSX_HD void tsit5_stage_inputs(double t, const double* y, double h, const double* p,
+                               double (&z)[6][F::D], double* ynext, const double* aux) {
   constexpr int D = F::D;
   using namespace tab;
-  // Recompute the stage inputs z1..z6 (z1 = y).
-  double z[6][D];
-  double k[5][D];
+  double k[6][D];
   GSX_UNROLL for (int i = 0; i < D; ++i) z[0][i] = y[i];
   F::f(t, z[0], p, k[0], aux);
   GSX_UNROLL for (int i = 0; i < D; ++i) z[1][i] = y[i] + h * (a21 * k[0][i]);
@@ -622,7 +693,22 @@ GSX_HD void tsit5_step_vjp(double t, const double* y, double h, const double* p,
   GSX_UNROLL for (int i = 0; i < D; ++i)
     z[5][i] = y[i] + h * (a61 * k[0][i] + a62 * k[1][i] + a63 * k[2][i] + a64 * k[3][i] +
                           a65 * k[4][i]);
-  // Reverse: kbar_s = h*b_s*lam + sum_{r>s} a_rs * (h*zbar_r); zbar_s = J(z_s)^T kbar_s.
+  if (ynext) {
+    F::f(t + c6 * h, z[5], p, k[5], aux);
+    GSX_UNROLL for (int i = 0; i < D; ++i)
+      ynext[i] = y[i] + h * (b1 * k[0][i] + b2 * k[1][i] + b3 * k[2][i] + b4 * k[3][i] +
+                             b5 * k[4][i] + b6 * k[5][i]);
+  }
+}
+
+// The reverse half of the step VJP given the stage inputs z: lam <- (d y_next / d y)^T lam,
+// pb += (d y_next / d p)^T lam.
+template <class F>
+GSX_HD void tsit5_step_vjp_z(double t, const double (&z)[6][F::D], double h, const double* p,
+                             double* lam, double* pb, const double* aux) {
+  constexpr int D = F::D;
+  using namespace tab;
+  // kbar_s = h*b_s*lam + sum_{r>s} a_rs * (h*zbar_r); zbar_s = J(z_s)^T kbar_s.
   constexpr double A[6][6] = {{0, 0, 0, 0, 0, 0},
                               {a21, 0, 0, 0, 0, 0},
                               {a31, a32, 0, 0, 0, 0},
@@ -651,6 +737,16 @@ GSX_HD void tsit5_step_vjp(double t, const double* y, double h, const double* p,
     }
   }
   GSX_UNROLL for (int i = 0; i < D; ++i) lam[i] = lam_new[i];
+}
+
+// lam <- (d y_next / d y)^T lam ; pb += (d y_next / d p)^T lam, at the step (t, y, h): the
+// stage inputs are recomputed from y, then reversed.
+template <class F>
+GSX_HD void tsit5_step_vjp(double t, const double* y, double h, const double* p, double* lam,
+                           double* pb, const double* aux) {
+  double z[6][F::D];
+  tsit5_stage_inputs<F>(t, y, h, p, z, nullptr, aux);
+  tsit5_step_vjp_z<F>(t, z, h, p, lam, pb, aux);
 }
 
 // Frozen-mesh replay (the forward of gradsolve/warp/warp_replay.py:30-55) from possibly new
@@ -728,6 +824,38 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
+}
+
+// The adjoint with checkpoints every 2 accepted steps and no re-integration overhead: for
+// segment m (steps 2m, 2m+1) one advance from checkpoint m forms step 2m's stage inputs and the
+// state y_{2m+1}; step 2m+1's stage inputs are formed for its own VJP, and step 2m's VJP reuses
+// the ones kept from the advance, so each step's stages are computed once, as the minimal
+// adjoint needs. A missing second step (odd n_acc) runs with h = 0, which leaves lam and pb
+// unchanged exactly, so every lane follows the same control flow. `m` is wavefront-uniform in
+// the kernel: all active lanes read checkpoint row m and tape rows 2m, 2m+1 together.
+template <class F>
+GSX_HD void tsit5_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32_t na,
+                                   const double* p, double* lam, double* pb) {
+  constexpr int D = F::D;
+  constexpr int DCK = ckpt_rows<F>();
+  const int64_t n = a.n;
+  const int32_t s0 = 2 * m;
+  const double* base = a.ckpt + static_cast<int64_t>(m) * DCK * n + j;
+  double y[D];
+  GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = base[d * n];
+  double t = 0.0;
+  if constexpr (!F::autonomous) t = base[D * n];
+  const double h0 = a.tape[static_cast<int64_t>(s0) * n + j];
+  double h1 = 0.0;
+  if (s0 + 1 < na) h1 = a.tape[static_cast<int64_t>(s0 + 1) * n + j];
+  double z0[6][D], y1[D];
+  tsit5_stage_inputs<F>(t, y, h0, p, z0, y1, a.aux);
+  {
+    double z1[6][D];
+    tsit5_stage_inputs<F>(t + h0, y1, h1, p, z1, nullptr, a.aux);
+    tsit5_step_vjp_z<F>(t + h0, z1, h1, p, lam, pb, a.aux);
+  }
+  tsit5_step_vjp_z<F>(t, z0, h0, p, lam, pb, a.aux);
 }
 
 }  // namespace GSX_NS

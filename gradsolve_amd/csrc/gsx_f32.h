@@ -21,15 +21,20 @@ GSX_HD void kahan_add(float& s, float& lo, float d) {
   s = t;
 }
 
+// One FP32 checkpoint (value rows, then compensation rows) at `base` = &ck[slot][0][j].
 template <class F>
-GSX_HD void ckpt_store_f32(double* ck, int64_t slot, int64_t n, int64_t j, const float* y,
-                           const float* lo) {
+GSX_HD void ckpt_put_f32(float* base, int64_t n, const float* y, const float* lo) {
   constexpr int D = F::D;
-  float* base = reinterpret_cast<float*>(ck) + slot * (2 * D) * n + j;
   GSX_UNROLL for (int d = 0; d < D; ++d) {
     base[d * n] = y[d];
     base[(D + d) * n] = lo[d];
   }
+}
+
+template <class F>
+GSX_HD void ckpt_store_f32(double* ck, int64_t slot, int64_t n, int64_t j, const float* y,
+                           const float* lo) {
+  ckpt_put_f32<F>(reinterpret_cast<float*>(ck) + slot * (2 * F::D) * n + j, n, y, lo);
 }
 
 template <class F>
@@ -54,7 +59,10 @@ GSX_HD void tsit5_fwd_lane_f32(const FwdArgs& a, int64_t j) {
   const bool rec = a.tape != nullptr;
   const bool ck = a.ckpt != nullptr;
   int32_t ck_left = a.ckpt_interval;
-  if (ck) ckpt_store_f32<F>(a.ckpt, 0, n, j, y, lo);
+  double* tp = rec ? a.tape + j : nullptr;
+  float* cp = ck ? reinterpret_cast<float*>(a.ckpt) + j : nullptr;
+  const int64_t cstride = static_cast<int64_t>(2 * D) * n;
+  if (ck) ckpt_put_f32<F>(cp, n, y, lo);
   const float E1 = float(e1), E2 = float(e2), E3 = float(e3), E4 = float(e4), E5 = float(e5),
               E6 = float(e6), E7 = float(e7);
   for (int64_t it = 0; it < a.max_steps; ++it) {
@@ -71,7 +79,7 @@ GSX_HD void tsit5_fwd_lane_f32(const FwdArgs& a, int64_t j) {
       yd[i] = static_cast<double>(y[i]) + static_cast<double>(lo[i]);
       y5d[i] = static_cast<double>(y5[i]);
     }
-    const double err = wrms<D>(e, yd, y5d, a.rtol, a.atol);
+    const double err = tsit5_err<D>(e, yd, y5d, a.rtol, a.atol);
     if (err != err) {
       rej += static_cast<int32_t>(a.max_steps - it);
       break;
@@ -86,16 +94,20 @@ GSX_HD void tsit5_fwd_lane_f32(const FwdArgs& a, int64_t j) {
         kahan_add(y[d], lo[d], inc[d]);
         k[0][d] = k[6][d];
       }
-      if (rec) a.tape[static_cast<int64_t>(acc) * n + j] = h;
+      if (rec) {
+        *tp = h;
+        tp += n;
+      }
       ++acc;
       if (ck && --ck_left == 0) {
-        ckpt_store_f32<F>(a.ckpt, acc / a.ckpt_interval, n, j, y, lo);
+        cp += cstride;
+        ckpt_put_f32<F>(cp, n, y, lo);
         ck_left = a.ckpt_interval;
       }
     } else {
       ++rej;
     }
-    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
+    const double fac = tsit5_fac(err);
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;

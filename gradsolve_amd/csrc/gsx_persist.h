@@ -66,7 +66,7 @@ struct StateF64 {
     GSX_UNROLL for (int i = 0; i < D; ++i)
       e[i] = h * (e1 * k[0][i] + e2 * k[1][i] + e3 * k[2][i] + e4 * k[3][i] + e5 * k[4][i] +
                   e6 * k[5][i] + e7 * k[6][i]);
-    return wrms<D>(e, y, y5, rtol, atol);
+    return tsit5_err<D>(e, y, y5, rtol, atol);
   }
   GSX_HD void accept() {
     GSX_UNROLL for (int d = 0; d < D; ++d) {
@@ -110,7 +110,7 @@ struct StateF32 {
       yd[i] = static_cast<double>(y[i]) + static_cast<double>(lo[i]);
       y5d[i] = static_cast<double>(y5[i]);
     }
-    return wrms<D>(e, yd, y5d, rtol, atol);
+    return tsit5_err<D>(e, yd, y5d, rtol, atol);
   }
   GSX_HD void accept() {
     GSX_UNROLL for (int d = 0; d < D; ++d) {

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
     } else {
       ++rej;
     }
-    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
+    const double fac = tsit5_fac(err);
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
     if (!(t < t1)) {
       finish(kReached);

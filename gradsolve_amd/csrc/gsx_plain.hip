@@ -54,7 +54,7 @@ __device__ void plain_fwd_lane(const PlainArgs& a, int64_t j) {
     GSX_UNROLL for (int i = 0; i < D; ++i)
       e[i] = h * (e1 * k[0][i] + e2 * k[1][i] + e3 * k[2][i] + e4 * k[3][i] + e5 * k[4][i] +
                   e6 * k[5][i] + e7 * k[6][i]);
-    const double err = wrms<D>(e, y, y5, a.rtol, a.atol);
+    const double err = tsit5_err<D>(e, y, y5, a.rtol, a.atol);
     if (err != err) {
       rej += static_cast<int32_t>(a.max_steps - it);
       break;
@@ -78,7 +78,7 @@ __device__ void plain_fwd_lane(const PlainArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = kSafety * ctrl_pow_tsit5(py_max(err, kErrFloor));
+    const double fac = tsit5_fac(err);
     dt = h * py_min(py_max(fac, kFacMin), kFacMax);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;

@@ -217,7 +217,10 @@ GSX_HD void rb23_fwd_lane(const FwdArgs& a, int64_t j) {
   const bool rec = a.tape != nullptr;
   const bool ck = a.ckpt != nullptr;
   int32_t ck_left = a.ckpt_interval;
-  if (ck) ckpt_store<F>(a.ckpt, 0, n, j, y, t);
+  double* tp = rec ? a.tape + j : nullptr;
+  double* cp = ck ? a.ckpt + j : nullptr;
+  const int64_t cstride = static_cast<int64_t>(ckpt_rows<F>()) * n;
+  if (ck) ckpt_put<F>(cp, n, y, t);
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
     const double h = py_min(dt, t1 - t);
@@ -233,10 +236,14 @@ GSX_HD void rb23_fwd_lane(const FwdArgs& a, int64_t j) {
       }
       t = t + h;
       GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ynew[d];
-      if (rec) a.tape[static_cast<int64_t>(acc) * n + j] = h;
+      if (rec) {
+        *tp = h;
+        tp += n;
+      }
       ++acc;
       if (ck && --ck_left == 0) {
-        ckpt_store<F>(a.ckpt, acc / a.ckpt_interval, n, j, y, t);
+        cp += cstride;
+        ckpt_put<F>(cp, n, y, t);
         ck_left = a.ckpt_interval;
       }
     } else {

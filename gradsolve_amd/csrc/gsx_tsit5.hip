@@ -19,11 +19,12 @@ namespace {
 constexpr int kFwdBlock = 256;
 constexpr int kAdjBlock = 64;  // one wavefront per workgroup: LDS is allocated per wave
 
-template <class F>
+// kRecCk: the launch records both the tape and the checkpoints (see tsit5_fwd_lane).
+template <class F, bool kRecCk>
 __global__ __launch_bounds__(kFwdBlock) void tsit5_fwd_kernel(FwdArgs a) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
   if (j >= a.n) return;
-  tsit5_fwd_lane<F>(a, j);
+  tsit5_fwd_lane<F, kRecCk>(a, j);
 }
 
 // The segment buffer holds, for each lane, the states at the start of the (at most CMAX)
@@ -78,8 +79,48 @@ template <class F>
 int launch_fwd(const FwdArgs& a, hipStream_t s) {
   if (a.n == 0) return GSX_OK;
   const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
-  tsit5_fwd_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  if (a.tape != nullptr && a.ckpt != nullptr)
+    tsit5_fwd_kernel<F, true><<<grid, kFwdBlock, 0, s>>>(a);
+  else
+    tsit5_fwd_kernel<F, false><<<grid, kFwdBlock, 0, s>>>(a);
   return check_launch("tsit5_fwd_kernel");
+}
+
+// Checkpoint interval 2: the pair adjoint (tsit5_adj_pair_segment; no LDS). The segment loop
+// runs in step across the wavefront, from its largest segment count down, so the active lanes
+// read the same checkpoint and tape rows; a lane idles above its own count. Fields with at
+// most three states and one parameter are held to 128 VGPRs (four waves per SIMD; Lorenz:
+// 136 -> 128 VGPRs with two spilled, measured 1.3% faster at N = 2^20); other fields keep
+// the compiler's choice (Lorenz with three parameters would spill 112 bytes per lane).
+template <class F>
+constexpr int pair_waves_per_eu() { return (F::D <= 3 && F::P == 1) ? 4 : 1; }
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<F>())))
+void tsit5_adj_pair_kernel(AdjArgs a) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  const bool valid = j < a.n;
+  const int32_t na = valid ? a.n_acc[j] : 0;
+  const int32_t nseg = (na + 1) / 2;
+  int32_t mmax = nseg;
+  GSX_UNROLL for (int off = 32; off >= 1; off >>= 1) {
+    const int32_t o = __shfl_xor(mmax, off);
+    mmax = o > mmax ? o : mmax;
+  }
+  if (!valid) return;
+  const int64_t n = a.n;
+  double p[P], pb[P], lam[D];
+  GSX_UNROLL for (int q = 0; q < P; ++q) {
+    p[q] = a.p[q * n + j];
+    pb[q] = 0.0;
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  for (int32_t m = mmax - 1; m >= 0; --m) {
+    if (m < nseg) tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb);
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
+  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }
 
 template <class F, int CMAX>
@@ -94,6 +135,11 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
   if (a.n == 0) return GSX_OK;
   const int C = a.ckpt_interval;
   if (C <= 1) return launch_adj_c<F, 1>(a, s);
+  if (C == 2) {
+    const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+    tsit5_adj_pair_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+    return check_launch("tsit5_adj_pair_kernel");
+  }
   if (C <= 4) return launch_adj_c<F, 4>(a, s);
   if (C <= 8) return launch_adj_c<F, 8>(a, s);
   if (C <= 16) return launch_adj_c<F, 16>(a, s);

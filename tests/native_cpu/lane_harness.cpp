@@ -29,6 +29,24 @@ int run_adj(const AdjArgs& a) {
   return 0;
 }
 template <class F>
+int run_adj_pair(const AdjArgs& a) {
+  constexpr int D = F::D, P = F::P;
+  for (int64_t j = 0; j < a.n; ++j) {
+    double p[P], pb[P], lam[D];
+    for (int q = 0; q < P; ++q) {
+      p[q] = a.p[q * a.n + j];
+      pb[q] = 0.0;
+    }
+    for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * a.n + j];
+    const int32_t na = a.n_acc[j];
+    for (int32_t m = (na + 1) / 2 - 1; m >= 0; --m)
+      tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb);
+    for (int d = 0; d < D; ++d) a.y0bar[d * a.n + j] = lam[d];
+    for (int q = 0; q < P; ++q) a.pbar[q * a.n + j] = pb[q];
+  }
+  return 0;
+}
+template <class F>
 int run_rb_fwd(const FwdArgs& a) {
   for (int64_t j = 0; j < a.n; ++j) rb23_fwd_lane<F>(a, j);
   return 0;
@@ -77,6 +95,15 @@ void lh_sincosf(int64_t n, const float* x, float* s, float* c) {
   for (int64_t i = 0; i < n; ++i) gsx_sincosf(x[i], s + i, c + i);
 }
 
+// The default variant's Tsit5 factor from the squared norm m = err^2, clamped.
+double lh_ctrl_fac_sq(double m) {
+  return py_min(py_max(tsit5_fac(m), kFacMin), kFacMax);
+}
+// (q0^2 + q1^2 + q2^2) / 3 as the default variant's squared norm forms it (unit scales).
+double lh_mean_sq3(const double* q) {
+  const double z[3] = {0.0, 0.0, 0.0};
+  return wrms_sq<3>(q, z, z, 0.0, 1.0);
+}
 double lh_ctrl_pow(int which, double x) {
   return which == 0 ? ctrl_pow_tsit5(x) : ctrl_pow_rb23(x);
 }
@@ -126,6 +153,19 @@ int lh_tsit5_replay(int field, int64_t n, const double* y0, const double* p, dou
   if (field == 6) return run_replay<FLorenz3>(a);
   if (field == 5) return run_replay<FVdp>(a);
   if (field == 2) return run_replay<FLinearDecay<2>>(a);
+  return -1;
+}
+// The checkpoint-interval-2 pair adjoint (the kernel's per-lane body, segments walked from
+// each trajectory's last; the kernel walks them in step across the wavefront, which only
+// changes which lanes idle).
+int lh_tsit5_adjoint_pair(int field, int64_t n, const double* p, double t0, const double* tape,
+                          int64_t cap, const int32_t* n_acc, const double* ckpt,
+                          const double* ybar, double* y0bar, double* pbar) {
+  AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, 2, ybar, y0bar, pbar};
+  if (field == 1) return run_adj_pair<FLorenz>(a);
+  if (field == 6) return run_adj_pair<FLorenz3>(a);
+  if (field == 5) return run_adj_pair<FVdp>(a);
+  if (field == 2) return run_adj_pair<FLinearDecay<2>>(a);
   return -1;
 }
 int lh_tsit5_adjoint(int field, int64_t n, const double* p, double t0, const double* tape,

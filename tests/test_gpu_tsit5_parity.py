@@ -107,7 +107,7 @@ def test_forward_record_matches_oracle_small(gpu, strict):
     assert rel_fr.max() < FREE_RUNNING_REL
 
 
-@pytest.mark.parametrize("C", [1, 4, 8, 16, 32])
+@pytest.mark.parametrize("C", [1, 2, 4, 8, 16, 32])
 def test_adjoint_matches_oracle_on_same_tape(gpu, C):
     ens = W.lorenz_small(seed=1)
     print(f"case adjoint C={C}", flush=True)

@@ -119,6 +119,36 @@ def test_controller_power_fast_variant_accuracy():
         assert np.max(np.abs(fac - rf) / np.spacing(rf)) <= 2.0
 
 
+def test_squared_norm_factor_and_mean():
+    """The default variant's squared-norm controller: m^(-0.1) against err^(-0.2) with
+    err = sqrt(m) evaluated in extended precision (within 2 ulp after the 0.9 factor and
+    the [0.2, 5] clamp), and the mean of three squares with one Markstein correction against
+    the IEEE quotient (within 1 ulp)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
+    F = ctypes.CDLL(os.path.join(HERE, "native_cpu", "liblane_harness_fast.so"))
+    F.lh_ctrl_fac_sq.restype = ctypes.c_double
+    F.lh_ctrl_fac_sq.argtypes = [ctypes.c_double]
+    F.lh_mean_sq3.restype = ctypes.c_double
+    F.lh_mean_sq3.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(3)
+    ms = np.exp(rng.uniform(np.log(1e-24), np.log(1e20), 20000))
+    got = np.array([F.lh_ctrl_fac_sq(m) for m in ms])
+    ld = np.longdouble
+    err = np.sqrt(ms.astype(ld))
+    ref = np.clip(ld(0.9) * np.maximum(err, ld(1e-10)) ** ld(-0.2), ld(0.2), ld(5.0))
+    ref = ref.astype(np.float64)
+    assert np.max(np.abs(got - ref) / np.spacing(ref)) <= 2.0
+    # The accept tests m <= 1 and sqrt(m) <= 1 part only at the double just above 1.
+    above = np.nextafter(1.0, 2.0)
+    assert np.sqrt(above) == 1.0 and not above <= 1.0
+    assert np.sqrt(np.nextafter(above, 2.0)) > 1.0 and np.sqrt(np.nextafter(1.0, 0.0)) < 1.0
+    qs = rng.standard_normal((20000, 3)) * np.exp(rng.uniform(-20, 20, (20000, 1)))
+    mean = np.array([F.lh_mean_sq3(np.ascontiguousarray(q).ctypes.data) for q in qs])
+    s = (qs[:, 0] * qs[:, 0] + qs[:, 1] * qs[:, 1]) + qs[:, 2] * qs[:, 2]
+    ieee = s / 3.0
+    assert np.max(np.abs(mean - ieee) / np.spacing(ieee)) <= 1.0
+
+
 def test_sincos_within_two_ulp():
     """The device sincos (three-part pi/2 reduction, Taylor polynomials; host build of the
     same source) against the C library's sin and cos: within 2 ulp over |x| < 2^20, exact
@@ -172,3 +202,30 @@ def test_sincosf_within_two_ulp():
     uc = np.abs(c.astype(np.float64) - rc) / np.spacing(np.abs(rc))
     print(f"sincosf: max ulp sin {us.max():.2f}, cos {uc.max():.2f}")
     assert us.max() <= 2.0 and uc.max() <= 2.0
+
+
+def test_pair_adjoint_equals_generic_adjoint(lh):
+    """Checkpoint interval 2: the pair adjoint (each step's stages formed once, the advance's
+    kept for the second VJP, a zero-size step padding odd step counts) returns the generic
+    segment adjoint's gradients bit for bit, on a tape with odd and even step counts."""
+    e = W.lorenz(300, seed=21)
+    n, cap = 300, 200
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape, ck = Guarded((cap, n)), Guarded((cap // 2 + 1, 3, n))
+    lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 100000,
+                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, 2)
+    assert np.all(st.a == 0) and len(set(na.a % 2)) == 2
+    ybar = np.ascontiguousarray(2.0 * yf.a)
+    g0, gp = Guarded((3, n)), Guarded((1, n))
+    lh.lh_tsit5_adjoint(1, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, 2,
+                        ybar.ctypes.data, g0.ptr, gp.ptr)
+    q0, qp = Guarded((3, n)), Guarded((1, n))
+    lh.lh_tsit5_adjoint_pair.argtypes = [ctypes.c_int, ctypes.c_int64] + [ctypes.c_void_p] + \
+        [ctypes.c_double] + [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    lh.lh_tsit5_adjoint_pair(1, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr,
+                             ybar.ctypes.data, q0.ptr, qp.ptr)
+    assert q0.intact() and qp.intact()
+    assert np.array_equal(q0.a, g0.a) and np.array_equal(qp.a, gp.a)
