@@ -106,6 +106,24 @@ constexpr double kTsit5Exp = -0.2;
 // [1e-5, 1e4]: outside that range the controller factor 0.9 x^e saturates at the
 // [0.2, 5] clamp for both exponents, so the step size is unchanged.
 // ---------------------------------------------------------------------------------------
+// min/max of operands that are never NaN (step sizes, error scales, controller factors):
+// the hardware instructions in the default variant, Python's comparison form (b < a ? b : a)
+// in the strict one; for non-NaN operands of equal zero sign both give the same value.
+GSX_HD double nn_min(double a, double b) {
+#if defined(GSX_STRICT_ARITH)
+  return b < a ? b : a;
+#else
+  return std::fmin(a, b);
+#endif
+}
+GSX_HD double nn_max(double a, double b) {
+#if defined(GSX_STRICT_ARITH)
+  return b > a ? b : a;
+#else
+  return std::fmax(a, b);
+#endif
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
 GSX_HD float fast_log2f(float x) { return __builtin_amdgcn_logf(x); }
 GSX_HD float fast_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -119,7 +137,7 @@ GSX_HD float fast_exp2f(float x) { return std::exp2(x); }
 template <int M>
 GSX_HD double ctrl_pow_newton(double x, double c) {
   constexpr double lo = M == 10 ? 1e-10 : 1e-5, hi = M == 10 ? 1e8 : 1e4;
-  const double xc = x < lo ? lo : (x > hi ? hi : x);
+  const double xc = nn_min(nn_max(x, lo), hi);
   const float lf = fast_log2f(static_cast<float>(xc));
   double y = static_cast<double>(fast_exp2f(lf * (-1.0f / M)));
   GSX_UNROLL for (int it = 0; it < 2; ++it) {
@@ -163,15 +181,22 @@ GSX_HD double ctrl_pow_rb23(double x) {
 }
 
 // a / b for the error norm (b >= atol > 0): the IEEE quotient in the strict variant; in the
-// default variant a times the hardware reciprocal refined by two Newton steps.
+// default variant the hardware reciprocal refined by one Newton step, the product a r, and
+// one Markstein correction q + r (a - b q) of the quotient.
+// (Host builds seed with the single-precision reciprocal, of similar accuracy, so that the
+// refinement can be tested on the CPU.)
 GSX_HD double norm_div(double a, double b) {
-#if defined(GSX_STRICT_ARITH) || !defined(__HIP_DEVICE_COMPILE__)
+#if defined(GSX_STRICT_ARITH)
   return a / b;
 #else
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(b);
+#else
+  double r = static_cast<double>(1.0f / static_cast<float>(b));
+#endif
   r = std::fma(r, std::fma(-b, r, 1.0), r);
-  r = std::fma(r, std::fma(-b, r, 1.0), r);
-  return a * r;
+  const double q = a * r;
+  return std::fma(std::fma(-b, q, a), r, q);
 #endif
 }
 
@@ -478,7 +503,7 @@ GSX_HD double wrms_sq(const double* e, const double* y, const double* ynew, doub
                       double atol) {
   double q[D];
   GSX_UNROLL for (int i = 0; i < D; ++i) {
-    const double sc = atol + rtol * py_max(std::fabs(y[i]), std::fabs(ynew[i]));
+    const double sc = atol + rtol * nn_max(std::fabs(y[i]), std::fabs(ynew[i]));
     const double r = norm_div(e[i], sc);
     q[i] = r * r;
   }
@@ -499,13 +524,18 @@ GSX_HD double tsit5_err(const double* e, const double* y, const double* ynew, do
 #endif
 }
 
-// Step-size factor 0.9 * err^-0.2 from tsit5_err's measure (clamped by the caller).
+// Step-size factor 0.9 * err^-0.2 from tsit5_err's measure.
 GSX_HD double tsit5_fac(double em) {
 #if defined(GSX_STRICT_ARITH)
   return kSafety * ctrl_pow_tsit5(py_max(em, kErrFloor));
 #else
-  return kSafety * ctrl_pow_tsit5_sq(py_max(em, kErrFloor * kErrFloor));
+  return kSafety * ctrl_pow_tsit5_sq(nn_max(em, kErrFloor * kErrFloor));
 #endif
+}
+
+// Next step size after a trial of size h with measure em: h * clamp(fac, 0.2, 5).
+GSX_HD double tsit5_next_dt(double h, double em) {
+  return h * nn_min(nn_max(tsit5_fac(em), kFacMin), kFacMax);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -603,7 +633,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
   if (ck) ckpt_put<F>(cp, n, y, t);
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
-    const double h = py_min(dt, t1 - t);
+    const double h = nn_min(dt, t1 - t);
     tsit5_stages<F, double>(t, y, h, p, k, y5, a.aux);
     F::f(t + h, y5, p, k[6], a.aux);
     double e[D];
@@ -640,8 +670,7 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = tsit5_fac(err);
-    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+    dt = tsit5_next_dt(h, err);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];

@@ -67,7 +67,7 @@ GSX_HD void tsit5_fwd_lane_f32(const FwdArgs& a, int64_t j) {
               E6 = float(e6), E7 = float(e7);
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
-    const double h = py_min(dt, t1 - t);
+    const double h = nn_min(dt, t1 - t);
     const float hf = static_cast<float>(h);
     tsit5_stages<F, float>(0.0f, y, hf, p, k, y5, a.aux, inc);
     F::f(0.0f, y5, p, k[6], a.aux);
@@ -107,8 +107,7 @@ GSX_HD void tsit5_fwd_lane_f32(const FwdArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = tsit5_fac(err);
-    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+    dt = tsit5_next_dt(h, err);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;
   GSX_UNROLL for (int d = 0; d < D; ++d)

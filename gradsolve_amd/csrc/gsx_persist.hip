@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
     if (!active) continue;
 
     // One trial step (gradsolve/solvers/tsit5_replay.py:107-124).
-    const double h = py_min(dt, t1 - t);
+    const double h = nn_min(dt, t1 - t);
     const double err = st.trial(h, a.rtol, a.atol, a.aux);
     ++it;
     if (err != err) {
@@ -146,8 +146,7 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
     } else {
       ++rej;
     }
-    const double fac = tsit5_fac(err);
-    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+    dt = tsit5_next_dt(h, err);
     if (!(t < t1)) {
       finish(kReached);
     } else if (it >= a.max_steps) {

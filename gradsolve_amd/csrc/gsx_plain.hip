@@ -28,7 +28,7 @@ __device__ void plain_fwd_lane(const PlainArgs& a, int64_t j) {
   int32_t acc = 0, rej = 0, status = kReached;
   for (int64_t it = 0; it < a.max_steps; ++it) {
     if (t >= t1) break;
-    const double h = py_min(dt, t1 - t);
+    const double h = nn_min(dt, t1 - t);
     GSX_UNROLL for (int i = 0; i < D; ++i) {
       z[0][i] = y[i];
       z[1][i] = y[i] + h * (a21 * k[0][i]);
@@ -78,8 +78,7 @@ __device__ void plain_fwd_lane(const PlainArgs& a, int64_t j) {
     } else {
       ++rej;
     }
-    const double fac = tsit5_fac(err);
-    dt = h * py_min(py_max(fac, kFacMin), kFacMax);
+    dt = tsit5_next_dt(h, err);
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];

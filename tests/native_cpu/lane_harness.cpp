@@ -99,6 +99,7 @@ void lh_sincosf(int64_t n, const float* x, float* s, float* c) {
 double lh_ctrl_fac_sq(double m) {
   return py_min(py_max(tsit5_fac(m), kFacMin), kFacMax);
 }
+double lh_norm_div(double a, double b) { return norm_div(a, b); }
 // (q0^2 + q1^2 + q2^2) / 3 as the default variant's squared norm forms it (unit scales).
 double lh_mean_sq3(const double* q) {
   const double z[3] = {0.0, 0.0, 0.0};

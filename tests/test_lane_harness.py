@@ -142,6 +142,14 @@ def test_squared_norm_factor_and_mean():
     above = np.nextafter(1.0, 2.0)
     assert np.sqrt(above) == 1.0 and not above <= 1.0
     assert np.sqrt(np.nextafter(above, 2.0)) > 1.0 and np.sqrt(np.nextafter(1.0, 0.0)) < 1.0
+    # The error-norm quotient: refined reciprocal and one Markstein correction, against the
+    # IEEE quotient (within 1 ulp), over the scales the norm sees.
+    F.lh_norm_div.restype = ctypes.c_double
+    F.lh_norm_div.argtypes = [ctypes.c_double, ctypes.c_double]
+    a = rng.standard_normal(20000) * np.exp(rng.uniform(-30, 10, 20000))
+    b = np.exp(rng.uniform(np.log(1e-12), np.log(1e6), 20000))
+    got = np.array([F.lh_norm_div(x, y) for x, y in zip(a, b)])
+    assert np.max(np.abs(got - a / b) / np.spacing(np.abs(a / b))) <= 1.0
     qs = rng.standard_normal((20000, 3)) * np.exp(rng.uniform(-20, 20, (20000, 1)))
     mean = np.array([F.lh_mean_sq3(np.ascontiguousarray(q).ctypes.data) for q in qs])
     s = (qs[:, 0] * qs[:, 0] + qs[:, 1] * qs[:, 1]) + qs[:, 2] * qs[:, 2]
