@@ -756,7 +756,7 @@ GSX_HD void tsit5_step_vjp_z(double t, const double (&z)[6][F::D], double h, con
   GSX_UNROLL for (int i = 0; i < D; ++i) lam_new[i] = lam[i];
   GSX_UNROLL for (int s = 5; s >= 0; --s) {
     double zb[D];
-    GSX_UNROLL for (int i = 0; i < D; ++i) zb[i] = 0.0;
+    GSX_UNROLL for (int i = 0; i < D; ++i) zb[i] = -0.0;  // x + (-0.0) == x: the first add folds
     F::vjp(t + Cn[s] * h, z[s], p, kb[s], zb, pb, aux);
     GSX_UNROLL for (int i = 0; i < D; ++i) lam_new[i] += zb[i];
     double hz[D];

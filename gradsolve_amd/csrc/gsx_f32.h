@@ -194,11 +194,12 @@ GSX_HD void tsit5_step_vjp_f32(const float* y, float h, const float* p, float* l
   }
   float dlam[D];
   float dp[P];
-  GSX_UNROLL for (int i = 0; i < D; ++i) dlam[i] = 0.0f;
-  GSX_UNROLL for (int q = 0; q < P; ++q) dp[q] = 0.0f;
+  // Accumulators start at -0.0, the exact additive identity, so their first adds fold away.
+  GSX_UNROLL for (int i = 0; i < D; ++i) dlam[i] = -0.0f;
+  GSX_UNROLL for (int q = 0; q < P; ++q) dp[q] = -0.0f;
   GSX_UNROLL for (int s = 5; s >= 0; --s) {
     float zb[D];
-    GSX_UNROLL for (int i = 0; i < D; ++i) zb[i] = 0.0f;
+    GSX_UNROLL for (int i = 0; i < D; ++i) zb[i] = -0.0f;
     F::vjp(0.0f, z[s], p, kb[s], zb, dp, aux);
     GSX_UNROLL for (int i = 0; i < D; ++i) dlam[i] += zb[i];
     float hz[D];

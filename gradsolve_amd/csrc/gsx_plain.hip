@@ -121,7 +121,7 @@ __device__ void plain_adj_lane(const PlainArgs& a, int64_t j) {
       double z[D], zb[D];
       GSX_UNROLL for (int d = 0; d < D; ++d) {
         z[d] = row[(st * D + d) * n];
-        zb[d] = 0.0;
+        zb[d] = -0.0;  // x + (-0.0) == x: the first add folds
       }
       F::vjp(0.0, z, p, kb[st], zb, pb, a.aux);
       GSX_UNROLL for (int i = 0; i < D; ++i) {

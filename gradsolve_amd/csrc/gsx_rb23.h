@@ -183,7 +183,7 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
   }
   GSX_UNROLL for (int i = 0; i < D; ++i) {
     k1b[i] -= r2b[i];
-    ub[i] = 0.0;
+    ub[i] = -0.0;  // x + (-0.0) == x: the first add folds
   }
   F::vjp(t, u, p, r2b, ub, pb, aux);
   GSX_UNROLL for (int i = 0; i < D; ++i) {
