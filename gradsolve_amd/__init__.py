@@ -7,13 +7,13 @@ The compute path is the gfx950 HIP library behind the C ABI in include/gradsolve
 from .api import ENGINE_REGISTRY, EngineSpec, grad_closure, solve
 from .base import Backend, Problem, Route, SolveResult
 from .dispatch import DECISION_MAP, choose_engine, choose_remat
-from .fields import UnsupportedRHS, register_jax_field
+from .fields import UnsupportedRHS, register_field, register_jax_field
 from . import problems
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "solve", "grad_closure", "register_jax_field", "Problem", "Route", "SolveResult",
-    "Backend", "ENGINE_REGISTRY", "EngineSpec", "choose_engine", "choose_remat",
+    "solve", "grad_closure", "register_jax_field", "register_field", "Problem", "Route",
+    "SolveResult", "Backend", "ENGINE_REGISTRY", "EngineSpec", "choose_engine", "choose_remat",
     "DECISION_MAP", "UnsupportedRHS", "problems", "__version__",
 ]

@@ -26,6 +26,9 @@ FIELD_ROBERTSON = 3
 FIELD_KURAMOTO = 4
 FIELD_VDP = 5
 FIELD_LORENZ3 = 6
+FIELD_RUNTIME_BASE = 1000  # ids of fields registered at run time start here
+FIELD_STIFF = 1            # gsx_field_register flags
+FIELD_TIME_DEPENDENT = 2
 
 _lock = threading.Lock()
 _lib = None
@@ -74,6 +77,8 @@ SIGNATURES = {
                                      _vp, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "gsx_field_register": (_i, [ctypes.c_char_p, _i, _i, _i, ctypes.POINTER(_i)]),
+    "gsx_field_release": (_i, [_i]),
 }
 
 
@@ -119,3 +124,12 @@ def call(name: str, *args) -> int:
             raise ValueError(f"{name}: {msg}")
         raise NativeError(name, rc, msg)
     return rc
+
+
+def field_register(source: str, dim: int, n_params: int, flags: int) -> int:
+    """Compile a field's C++ source into the fused kernels; returns its field id. A compile
+    error raises ValueError carrying the compiler's log."""
+    fid = _i(-1)
+    call("gsx_field_register", source.encode(), int(dim), int(n_params), int(flags),
+         ctypes.byref(fid))
+    return int(fid.value)

@@ -158,8 +158,8 @@ ENGINE_REGISTRY = {
 }
 
 
-def _resolve(engine, problem, need_grad, batch_n, accuracy_target):
-    reasons = []
+def _resolve(engine, problem, need_grad, batch_n, accuracy_target, pre=None):
+    reasons = list(pre or [])
     if engine == "auto":
         name = D.choose_engine(problem.dim, problem.is_stiff, need_grad, batch_n=batch_n,
                                accuracy_target=accuracy_target)
@@ -186,11 +186,35 @@ def _resolve(engine, problem, need_grad, batch_n, accuracy_target):
             f"the reference's general engine {name!r}, which this build does not include")
     spec = ENGINE_REGISTRY[name]
     if not spec.supports(problem):
+        why = "".join(f" ({r})" for r in reasons if r.startswith("fused translation"))
         raise ValueError(
-            f"no native field serves problem {problem.name!r}: the reference would fall back "
-            "to its general-RHS engines, which this build does not include. Register the "
-            "field with register_jax_field or use one of " + str(F.registered_names()))
+            f"no native field serves problem {problem.name!r}{why}: the reference would fall "
+            "back to its general-RHS engines, which this build does not include. Register the "
+            "field with register_jax_field / register_field (or pass fused='auto' with a "
+            "traceable f_jax) or use one of " + str(F.registered_names()))
     return name, reasons
+
+
+def _translate(problem, params, fused, reasons) -> None:
+    """``fused='auto'|True``: a problem with no registered field whose ``f_jax`` traces gets
+    one compiled on first use, as the reference translates on first use
+    (gradsolve/api.py:291-400 with gradsolve/warp/jax_field.py). ``fused=True`` re-raises an
+    untraceable right-hand side; ``'auto'`` records why and lets routing report that no
+    native field serves the problem (this build has no general-RHS engine to fall back to)."""
+    if fused is False or F.binding_for(problem) is not None:
+        return
+    f = getattr(problem, "f_jax", None)
+    if f is None:
+        return
+    n_params = int(np.asarray(params).shape[1]) if np.ndim(params) == 2 else 0
+    try:
+        F.register_jax_field(problem.name, f, int(problem.dim), n_params,
+                             stiff=bool(problem.is_stiff))
+        reasons.append("fused: f_jax compiled into the native kernels")
+    except F.UnsupportedRHS as exc:
+        if fused is True:
+            raise
+        reasons.append(f"fused translation fell back ({exc.primitive_name}): {exc}")
 
 
 def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
@@ -211,8 +235,10 @@ def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
     if precision not in ("float64", "float32"):
         raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
     _check_shapes(problem, y0, params)
+    pre = []
+    _translate(problem, params, fused, pre)
     name, reasons = _resolve(engine, problem, need_grad=saveat is not None, batch_n=batch_n,
-                             accuracy_target=accuracy_target)
+                             accuracy_target=accuracy_target, pre=pre)
     b = F.binding_for(problem)
     if not _native_ok(b, problem, params):
         raise ValueError(f"params shape {tuple(params.shape)} does not match the field "
@@ -259,8 +285,10 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
     if precision not in ("float64", "float32"):
         raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
     _check_shapes(problem, y0, params)
+    pre = []
+    _translate(problem, params, fused, pre)
     name, reasons = _resolve(engine, problem, need_grad=True, batch_n=batch_n,
-                             accuracy_target=accuracy_target)
+                             accuracy_target=accuracy_target, pre=pre)
     if name == D.HIP_TSIT5:
         name = D.HIP_REPLAY
     b = F.binding_for(problem)

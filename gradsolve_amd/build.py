@@ -56,18 +56,55 @@ def sources() -> list[tuple[str, str, list[str]]]:
     return out
 
 
+# Headers embedded into the library for fields compiled at run time (gsx_runtime.hip hands
+# them to hiprtc by name, so the library needs no header files where it runs).
+RT_HEADERS = ["gsx_platform.h", "gsx_args.h", "gsx_core.h", "gsx_f32.h", "gsx_rb23.h",
+              "gsx_tsit5_kernels.h", "gsx_rb23_kernels.h"]
+RT_DELIM = "GSXRTHDR"
+
+
+def write_rt_headers(gendir: str) -> str:
+    """Generate gsx_rt_headers.inc: the RT_HEADERS texts as raw string literals."""
+    os.makedirs(gendir, exist_ok=True)
+    parts = ["// generated by gradsolve_amd/build.py from csrc/ -- do not edit", ""]
+    names, srcs = [], []
+    for h in RT_HEADERS:
+        with open(os.path.join(CSRC, h)) as fh:
+            text = fh.read()
+        if ")" + RT_DELIM + '"' in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter {RT_DELIM}")
+        names.append(f'"{h}"')
+        srcs.append(f'R"{RT_DELIM}({text}){RT_DELIM}"')
+    parts.append(f"static const int kRtHeaderCount = {len(RT_HEADERS)};")
+    parts.append("static const char* const kRtHeaderNames[] = {" + ", ".join(names) + "};")
+    parts.append("static const char* const kRtHeaderSources[] = {")
+    parts.extend(f"    {x}," for x in srcs)
+    parts.append("};")
+    out = os.path.join(gendir, "gsx_rt_headers.inc")
+    text = "\n".join(parts) + "\n"
+    old = None
+    if os.path.exists(out):
+        with open(out) as fh:
+            old = fh.read()
+    if old != text:
+        with open(out, "w") as fh:
+            fh.write(text)
+    return gendir
+
+
 def build(verbose: bool = False, jobs: int = 8) -> str:
     """Compile each translation unit to an object (in parallel), then link the .so."""
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
+    gendir = write_rt_headers(os.path.join(objdir, "gen"))
     procs = []
     objs = []
     for src, suffix, extra in sources():
         name = os.path.basename(src) + (f".{suffix}" if suffix else "") + ".o"
         obj = os.path.join(objdir, name)
         objs.append(obj)
-        cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-c", src, "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra, f"-I{gendir}", "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
@@ -76,7 +113,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
             _drain(procs)
     _drain(procs)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

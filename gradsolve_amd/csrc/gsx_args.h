@@ -1,7 +1,7 @@
 // gsx_args.h -- argument blocks and status codes shared by every kernel variant.
 #pragma once
 
-#include <cstdint>
+#include "gsx_platform.h"
 
 namespace gsx {
 

@@ -11,7 +11,7 @@
 namespace gsx {
 
 namespace {
-thread_local char g_err[512] = "";
+thread_local char g_err[8192] = "";  // room for a field compiler log
 }
 
 int set_error(int code, const char* fmt, ...) {
@@ -29,6 +29,7 @@ int check_launch(const char* what) {
 }
 
 bool field_info(int field, int D, FieldInfo* out) {
+  if (rt::is_runtime(field)) return rt::info(field, out) && out->D == D;
   switch (field) {
     case GSX_FIELD_LORENZ: *out = {3, 1, true, false}; return D == 3;
     case GSX_FIELD_LINEAR_DECAY: *out = {D, 1, true, false}; return D >= 1 && D <= 3;

@@ -14,18 +14,7 @@
 //                              (DCK = D, plus one row holding t for non-autonomous fields)
 #pragma once
 
-#include <cmath>
-#include <cstdint>
-
-#if defined(__HIPCC__)
-#include <hip/hip_runtime.h>
-#define GSX_HD __host__ __device__ __forceinline__
-#define GSX_UNROLL _Pragma("unroll")
-#else
-#define GSX_HD inline
-#define GSX_UNROLL
-#endif
-
+#include "gsx_platform.h"
 #include "gsx_args.h"
 
 // The computational code lives in GSX_NS. The library compiles it twice: once in namespace
@@ -85,6 +74,11 @@ constexpr double e5 = -3.2900695154360806799010475857113638501156832908949361585
 constexpr double e6 = 2.324710524099773982415355918398765796109060233222962411944060046314465391054716027841 - 1.866628418170587035753719399566211498666255505244122593996591602841258328965767580089;
 constexpr double e7 = -1.0 / 66.0;
 }  // namespace tab
+
+// Workgroup sizes: forward and replay kernels run 256 lanes per workgroup; the segment
+// adjoints run one wavefront per workgroup so that their LDS is allocated per wave.
+constexpr int kFwdBlock = 256;
+constexpr int kAdjBlock = 64;
 
 // Step-size controller constants (Hairer-Norsett-Wanner II.4; gradsolve/solvers/tsit5_step.py:26).
 constexpr double kSafety = 0.9;

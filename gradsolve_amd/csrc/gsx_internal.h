@@ -41,6 +41,20 @@ int kura_saveat_launch(const KuraSaveatArgs& a, bool fp32, hipStream_t s);
 int kura_adj_launch(const KuraAdjArgs& a, bool fp32, hipStream_t s);
 int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 
+// Fields registered at run time (gsx_runtime.hip): ids from GSX_FIELD_RUNTIME_BASE up,
+// kernels compiled with hiprtc per family and launched with the built-in argument blocks.
+namespace rt {
+enum Family { kTsit5, kTsit5Strict, kTsit5F32, kRb23, kRb23Strict, kNumFamilies };
+bool is_runtime(int field);
+bool info(int field, FieldInfo* out);
+int tsit5_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
+int tsit5_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
+int tsit5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
+int rb23_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
+int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
+int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
+}  // namespace rt
+
 }  // namespace gsx
 
 namespace gsx_strict {

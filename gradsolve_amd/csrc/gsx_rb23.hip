@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gsx_internal.h"
-#include "gsx_rb23.h"
+#include "gsx_rb23_kernels.h"
 
 namespace GSX_NS {
 
@@ -13,31 +13,6 @@ using ::gsx::kMaxCkptInterval;
 using ::gsx::set_error;
 
 namespace {
-
-constexpr int kFwdBlock = 256;
-constexpr int kAdjBlock = 64;
-
-template <class F>
-__global__ __launch_bounds__(kFwdBlock) void rb23_fwd_kernel(FwdArgs a) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
-  if (j >= a.n) return;
-  rb23_fwd_lane<F>(a, j);
-}
-
-template <class F>
-__global__ __launch_bounds__(kFwdBlock) void rb23_replay_kernel(ReplayArgs a) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
-  if (j >= a.n) return;
-  rb23_replay_lane<F>(a, j);
-}
-
-template <class F, int CMAX>
-__global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
-  __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
-  if (j >= a.n) return;
-  rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
-}
 
 template <class F>
 int launch_fwd(const FwdArgs& a, hipStream_t s) {
@@ -75,6 +50,12 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
                    kMaxCkptInterval);
 }
 
+#if defined(GSX_STRICT_ARITH)
+constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23Strict;
+#else
+constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23;
+#endif
+
 }  // namespace
 
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
@@ -85,7 +66,9 @@ int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
       if (D == 2) return launch_fwd<FLinearDecay<2>>(a, s);
       if (D == 3) return launch_fwd<FLinearDecay<3>>(a, s);
       break;
-    default: break;
+    default:
+      if (::gsx::rt::is_runtime(field)) return ::gsx::rt::rb23_fwd(field, kRtFamily, a, s);
+      break;
   }
   return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 forward: field %d with D=%d has no "
                    "analytic Jacobian in this build", field, D);
@@ -99,7 +82,9 @@ int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
       if (D == 2) return launch_replay<FLinearDecay<2>>(a, s);
       if (D == 3) return launch_replay<FLinearDecay<3>>(a, s);
       break;
-    default: break;
+    default:
+      if (::gsx::rt::is_runtime(field)) return ::gsx::rt::rb23_replay(field, kRtFamily, a, s);
+      break;
   }
   return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 replay: field %d with D=%d has no "
                    "analytic Jacobian in this build", field, D);
@@ -113,7 +98,9 @@ int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
       if (D == 2) return launch_adj<FLinearDecay<2>>(a, s);
       if (D == 3) return launch_adj<FLinearDecay<3>>(a, s);
       break;
-    default: break;
+    default:
+      if (::gsx::rt::is_runtime(field)) return ::gsx::rt::rb23_adj(field, kRtFamily, a, s);
+      break;
   }
   return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 adjoint: field %d with D=%d has no "
                    "analytic Jacobian in this build", field, D);

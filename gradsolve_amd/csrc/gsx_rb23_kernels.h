@@ -1,0 +1,32 @@
+// gsx_rb23_kernels.h -- the Rosenbrock23 kernels (one trajectory per lane) as templates over
+// the field, shared like gsx_tsit5_kernels.h by the library and by fields compiled at run time.
+#pragma once
+
+#include "gsx_core.h"
+#include "gsx_rb23.h"
+
+namespace GSX_NS {
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_fwd_kernel(FwdArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_fwd_lane<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_replay_kernel(ReplayArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_replay_lane<F>(a, j);
+}
+
+template <class F, int CMAX>
+__global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
+  __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+}
+
+}  // namespace GSX_NS

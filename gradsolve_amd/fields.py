@@ -35,12 +35,20 @@ class FieldBinding:
     param_cols: Optional[Tuple[int, ...]]  # user columns feeding the field (None: all)
     numpy_f: Callable                    # (y (dim,), p (n_params,), aux) -> dy, for probing
     aux_of: Optional[Callable] = None   # problem -> shared field data (K for Kuramoto)
+    source: Optional[str] = None        # generated C++ of a field compiled at run time
+    time_dependent: bool = False
 
     def native_params(self, params):
         """(n, P_user) -> (n, n_params) columns the field reads (numpy or torch)."""
-        if self.param_cols is None:
-            return params
-        return params[:, list(self.param_cols)]
+        if self.param_cols is not None:
+            return params[:, list(self.param_cols)]
+        if params.shape[1] < self.n_params:  # a field without parameters takes one unused
+            pad = self.n_params - params.shape[1]
+            if isinstance(params, np.ndarray):
+                return np.concatenate([params, np.zeros((params.shape[0], pad))], axis=1)
+            import torch
+            return torch.cat([params, params.new_zeros((params.shape[0], pad))], dim=1)
+        return params
 
 
 def _lorenz1(y, p, aux):
@@ -114,15 +122,53 @@ def registered_names():
     return sorted(set(_BUILTIN) | set(_REGISTERED))
 
 
+def _generate(name: str, f, dim: int, n_params: int, stiff: bool) -> FieldBinding:
+    """Trace ``f``, emit its C++ field with generated derivatives, compile it with hiprtc."""
+    from . import fieldgen as FG
+    tr = FG.trace(f, int(dim), int(n_params))
+    src = FG.emit_field(tr, stiff=bool(stiff))
+    flags = (N.FIELD_STIFF if stiff else 0) | (N.FIELD_TIME_DEPENDENT if tr.time_dependent
+                                               else 0)
+    P = max(int(n_params), 1)
+    fid = N.field_register(src, int(dim), P, flags)
+
+    def numpy_f(y, p, aux, _f=f, _n=int(n_params)):
+        return np.asarray(_f(0.0, np.asarray(y, np.float64), np.asarray(p, np.float64)[:_n]),
+                          dtype=np.float64)
+
+    return FieldBinding(name, fid, int(dim), P, bool(stiff), None, numpy_f, source=src,
+                        time_dependent=tr.time_dependent)
+
+
+def register_field(name: str, f, dim: int, n_params: int, *, stiff: bool = False):
+    """Compile the right-hand side ``f(t, y, p)`` into the fused kernels and route problems
+    named ``name`` to it.
+
+    ``f`` is traced once on symbolic NumPy arrays (``gradsolve_amd.fieldgen``); its C++ form,
+    with reverse-mode generated VJP (and, for ``stiff=True``, the analytic Jacobian and its
+    VJP), is compiled for gfx950 through ``gsx_field_register``. This is what the reference's
+    ``register_jax_field`` does with a jaxpr (gradsolve/warp/jax_field.py:516-564). Raises
+    ``UnsupportedRHS`` for right-hand sides outside the traceable subset and ``ValueError``
+    (with the compiler log) if the emitted source does not compile. Idempotent per name.
+    """
+    with _lock:
+        if name in _REGISTERED:
+            return _REGISTERED[name]
+        b = _generate(name, f, dim, n_params, stiff)
+        _REGISTERED[name] = b
+        return b
+
+
 def register_jax_field(name: str, f_jax, dim: int, n_params: int, *, stiff: bool = False):
-    """Route problems named ``name`` to a native field computing the same ``f_jax``.
+    """Route problems named ``name`` to a native field computing ``f_jax``.
 
     The reference translates the jaxpr of ``f_jax`` into a fused-kernel field
-    (gradsolve/__init__.py:31-42, gradsolve/warp/jax_field.py:516-564). This build matches
-    ``f_jax`` numerically against the native fields of the same dimension, parameter count
-    and stiffness (evaluated at random states and parameters with NumPy arrays); on a match
-    the name routes to that field. A right-hand side no native field computes raises
-    ``UnsupportedRHS``. Idempotent: an already-registered name is left untouched.
+    (gradsolve/__init__.py:31-42, gradsolve/warp/jax_field.py:516-564). JAX is absent here, so
+    ``f_jax`` is called on NumPy values: first it is matched numerically against the
+    hand-written native fields of the same dimension, parameter count and stiffness (random
+    states and parameters); on a match the name routes to that field. Otherwise it is traced
+    and compiled as a new field (``register_field``). A right-hand side outside the traceable
+    subset raises ``UnsupportedRHS``. Idempotent: an already-registered name is left untouched.
     """
     with _lock:
         if name in _REGISTERED:
@@ -152,7 +198,6 @@ def register_jax_field(name: str, f_jax, dim: int, n_params: int, *, stiff: bool
                                   b.numpy_f, b.aux_of)
                 _REGISTERED[name] = nb
                 return nb
-        raise UnsupportedRHS(
-            "field", f"register_jax_field({name!r}): no native field of dim {dim}, "
-                     f"{n_params} parameters, stiff={stiff} computes this f_jax (native "
-                     f"fields: {registered_names()})")
+        nb = _generate(name, f_jax, dim, n_params, stiff)
+        _REGISTERED[name] = nb
+        return nb

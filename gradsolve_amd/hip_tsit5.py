@@ -46,6 +46,18 @@ def flags_for(strict: bool = False, fp32: bool = False) -> int:
     return N.STRICT if strict else (N.FP32 if fp32 else 0)
 
 
+def max_segment_interval(rows: int) -> int:
+    """Largest checkpoint interval whose segment adjoint fits its LDS ring. The segment
+    kernels are built for at most 1, 4, 8, 16 or 32 steps per segment and hold that many
+    checkpoint rows per lane: CMAX * rows * 64 lanes * 8 B <= 64 KiB (C = 2 runs the pair
+    adjoint, which needs no LDS)."""
+    best = 2
+    for cmax in (4, 8, 16, 32):
+        if cmax * rows * 64 * 8 <= 65536:
+            best = cmax
+    return best
+
+
 def default_ckpt_interval(expected_steps: int) -> int:
     """sqrt(S)-spaced checkpoints: the interval that balances checkpoint storage against
     the re-integration a segment needs."""
@@ -114,9 +126,12 @@ def record(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, *, t0: float, 
     dev = y0_soa.device
     cap = int(min(max_steps, tape_cap if tape_cap else DEFAULT_TAPE_CAP))
     cap = max(cap, 1)
+    rows = int(N.call("gsx_ckpt_rows", field, D))
     while True:
         C = ckpt_interval if ckpt_interval else default_ckpt_interval(cap // 2)
         C = max(1, min(MAX_CKPT_INTERVAL, int(C)))
+        if not ckpt_interval:
+            C = min(C, max(1, max_segment_interval(rows)))
         y_final = torch.empty((D, n), dtype=torch.float64, device=dev)
         n_acc = torch.empty(n, dtype=torch.int32, device=dev)
         n_rej = torch.empty(n, dtype=torch.int32, device=dev)

@@ -47,6 +47,14 @@ enum gsx_field {
   GSX_FIELD_LORENZ3 = 6       /* D=3, P=3 (sigma, beta, rho) */
 };
 
+/* Fields registered at run time (gsx_field_register) take ids from GSX_FIELD_RUNTIME_BASE. */
+enum { GSX_FIELD_RUNTIME_BASE = 1000 };
+
+/* gsx_field_register flags: a stiff field (served by the Rosenbrock23 entry points; must be
+ * autonomous) and a field whose right-hand side reads t (Tsit5 only; checkpoints then carry
+ * t in an extra row, see gsx_ckpt_rows). */
+enum gsx_field_flags { GSX_FIELD_STIFF = 1, GSX_FIELD_TIME_DEPENDENT = 2 };
+
 /* Arithmetic flags. Default (0): the kernels contract a*b + c into fused multiply-adds.
  * GSX_STRICT: contraction off, so every operation rounds once in the reference loop's
  * textual order (gradsolve/solvers/tsit5_replay.py:100-124); the recorded tape then matches
@@ -56,6 +64,36 @@ enum gsx_field {
  * stay FP64; checkpoints hold the compensated pair in the bytes of an FP64 checkpoint.
  * The two flags are exclusive. */
 enum gsx_flags { GSX_STRICT = 1, GSX_FP32 = 2 };
+
+/* Registers a field given as HIP C++ source and returns its id in *field_id. The id is
+ * accepted wherever a built-in field id is: gsx_tsit5_fwd / _replay / _adjoint in every
+ * arithmetic variant (GSX_FP32 for autonomous fields), gsx_rb23_* for stiff fields, and the
+ * layout queries. The source defines `struct Field` (the library places it in a namespace of
+ * its own, where the gsx_core.h helpers are visible) with the members of the built-in fields:
+ *   static constexpr int D, P;  static constexpr bool autonomous;
+ *   template <class T> GSX_HD static void f(T t, const T* y, const T* p, T* dy,
+ *                                           const double* aux);
+ *   template <class T> GSX_HD static void vjp(T t, const T* y, const T* p, const T* v,
+ *                                             T* yb, T* pb, const double* aux);
+ * (vjp adds v^T df/dy to yb and v^T df/dp to pb), and for stiff fields
+ *   template <class T> GSX_HD static void jac(T t, const T* y, const T* p, T* J,
+ *                                             const double* aux);       (J row-major D x D)
+ *   template <class T> GSX_HD static void jac_vjp(const T* y, const T* p, const T* Jb,
+ *                                                 T* yb, T* pb, const double* aux);
+ * D, P and autonomy must agree with the arguments and flags. The kernels are compiled for
+ * gfx950 with hiprtc, one per program: the recording forward (Tsit5, or Rosenbrock23 for a
+ * stiff field) now, so that a compile error returns GSX_E_INVALID with the compiler's log in
+ * gsx_last_error(); every other kernel (replay, adjoints, strict and FP32 variants) on its
+ * first launch. A segment adjoint holds C checkpoint rows per lane in LDS, so for wide fields
+ * checkpoint intervals whose kernel would need more than 64 KiB per wavefront are rejected
+ * (GSX_E_INVALID); C = 2 (the pair adjoint) needs none. Replaces gradsolve's register_jax_field and the
+ * fused-kernel builds it triggers (gradsolve/warp/jax_field.py:516-564 registering into
+ * gradsolve/warp/_warp_kernel.py and gradsolve/warp/warp_rosenbrock.py). Thread-safe. */
+int gsx_field_register(const char* source, int D, int P, int flags, int* field_id);
+
+/* Unloads a registered field's kernels from every device they were loaded on; the id is not
+ * reused and later calls with it fail with GSX_E_INVALID. */
+int gsx_field_release(int field_id);
 
 /* Library version (major*10000 + minor*100 + patch). */
 int gsx_version(void);

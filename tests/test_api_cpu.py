@@ -54,9 +54,14 @@ def test_register_jax_field_identifies_native_field():
     assert gs.register_jax_field(MyLorenz.name, MyLorenz().f_jax, 3, 1) is b  # idempotent
 
 
-def test_register_jax_field_rejects_unknown_rhs():
+def test_register_jax_field_compiles_unknown_rhs_and_rejects_untraceable():
+    """A right-hand side no hand-written field computes is traced and compiled (hiprtc, no GPU
+    needed) into a field of its own; one that branches on the state raises UnsupportedRHS."""
+    b = gs.register_jax_field("cubic_for_test", lambda t, y, p: -p[0] * y ** 3, 2, 1)
+    assert b.field_id >= 1000 and b.source is not None and "struct Field" in b.source
     with pytest.raises(gs.UnsupportedRHS):
-        gs.register_jax_field("cubic_for_test", lambda t, y, p: y ** 3, 2, 1)
+        gs.register_jax_field("branchy_for_test",
+                              lambda t, y, p: y if y[0] > 0.0 else -y, 2, 1)
 
 
 def test_problem_f_jax_shapes():
@@ -77,7 +82,8 @@ def test_validation_errors():
         gs.solve(P.DiffEqGPULorenz(), e_y0, np.ones((4, 1)), engine="nope")
     with pytest.raises(ValueError, match="does not include"):
         gs.solve(P.DiffEqGPULorenz(), e_y0, np.ones((4, 1)), engine="diffrax")
+    # fused=False skips compiling the problem's f_jax, so nothing serves dim 7
     with pytest.raises(ValueError, match="no native field"):
-        gs.solve(P.LinearDecay(7), np.zeros((4, 7)), np.ones((4, 1)))
+        gs.solve(P.LinearDecay(7), np.zeros((4, 7)), np.ones((4, 1)), fused=False)
     with pytest.raises(RuntimeError, match="no CPU engine"):
         gs.solve(P.DiffEqGPULorenz(), e_y0, np.ones((4, 1)), device="cpu")

@@ -1,0 +1,250 @@
+"""Fields compiled from Python right-hand sides (gradsolve_amd.fieldgen, gsx_field_register).
+
+CPU tests. The emitted C++ field is compiled for the host (the same source the library hands
+to hiprtc) and checked:
+  * f against the Python function evaluated on float64 NumPy values: bitwise for arithmetic
+    fields (same operation order, contraction off); rel 1e-14 where transcendental functions
+    enter (NumPy's and the C library's implementations may differ in the last place, and
+    cancellation between terms can magnify that relative to the result);
+  * vjp against v^T J with J from complex-step differentiation of the Python function
+    (exact to rounding: Im f(y + i h e_j) / h with h = 1e-30), rel 1e-13;
+  * for stiff fields, jac against the complex-step Jacobian and jac_vjp against central
+    differences of the generated jac (exact to rounding for the polynomial Robertson field);
+  * the translatable subset: branches and comparisons on traced values raise UnsupportedRHS
+    naming the construct, as the reference's translator does for `cond`
+    (gradsolve tests/test_jax_field.py:112-118).
+The library side registers fields through hiprtc (no GPU needed to compile): errors carry the
+compiler log, and the registered id answers the layout queries. GPU parity of the kernels
+built from these fields is in test_gpu_runtime_field.py.
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from gradsolve_amd import _native as N
+from gradsolve_amd import fieldgen as FG
+from gradsolve_amd.fields import UnsupportedRHS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "gradsolve_amd", "csrc")
+
+
+def lorenz(t, y, p):
+    sigma, beta, rho = 10.0, 8.0 / 3.0, p[0]
+    return np.array([sigma * (y[1] - y[0]), rho * y[0] - y[1] - y[0] * y[2],
+                     y[0] * y[1] - beta * y[2]])
+
+
+def vdp(t, y, p):
+    x, v, mu = y[0], y[1], p[0]
+    return np.array([v, mu * ((1.0 - x * x) * v - x)])
+
+
+def forced_pendulum(t, y, p):
+    return np.array([y[1], -p[0] * np.sin(y[0]) - p[1] * y[1] + 0.5 * np.cos(t)])
+
+
+def mixed(t, y, p):
+    a, b = p[0], p[1]
+    return np.array([np.tanh(a * y[0]) - np.exp(-y[1] * y[1]) + y[2] ** 3,
+                     np.sqrt(1.0 + y[0] * y[0]) * b - np.log1p(y[2] * y[2]),
+                     y[0] / (1.0 + y[1] ** 2) + np.arctan(y[2]) * a])
+
+
+def ring(t, y, p):
+    a, b = p[0], p[1]
+    lap = np.roll(y, -1) - 2.0 * y + np.roll(y, 1)
+    return a * lap - y * y * y + b
+
+
+def robertson(t, y, p):
+    k1, k2, k3 = p[0], p[1], p[2]
+    return np.stack([-k1 * y[0] + k3 * y[1] * y[2],
+                     k1 * y[0] - k3 * y[1] * y[2] - k2 * y[1] * y[1],
+                     k2 * y[1] * y[1]])
+
+
+class HostField:
+    """The emitted struct compiled for the host."""
+
+    def __init__(self, f, dim, n_params, stiff=False):
+        tr = FG.trace(f, dim, n_params)
+        self.src = FG.emit_field(tr, stiff=stiff)
+        self.D, self.P = dim, max(n_params, 1)
+        self.time_dependent = tr.time_dependent
+        self.dir = tempfile.mkdtemp(prefix="fg_")
+        cpp = os.path.join(self.dir, "h.cpp")
+        with open(cpp, "w") as fh:
+            fh.write('#include "gsx_core.h"\nnamespace gsx_user {\nusing namespace ::gsx;\n')
+            fh.write(self.src)
+            fh.write('}\nextern "C" {\n'
+                     'void hf_f(double t, const double* y, const double* p, double* dy) '
+                     '{ gsx_user::Field::f<double>(t, y, p, dy, nullptr); }\n'
+                     'void hf_vjp(double t, const double* y, const double* p, const double* v, '
+                     'double* yb, double* pb) '
+                     '{ gsx_user::Field::vjp<double>(t, y, p, v, yb, pb, nullptr); }\n')
+            if stiff:
+                fh.write('void hf_jac(const double* y, const double* p, double* J) '
+                         '{ gsx_user::Field::jac<double>(0.0, y, p, J, nullptr); }\n'
+                         'void hf_jac_vjp(const double* y, const double* p, const double* Jb, '
+                         'double* yb, double* pb) '
+                         '{ gsx_user::Field::jac_vjp<double>(y, p, Jb, yb, pb, nullptr); }\n')
+            fh.write("}\n")
+        so = os.path.join(self.dir, "h.so")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-fno-builtin-sin", "-fno-builtin-cos", f"-I{CSRC}", "-o", so, cpp],
+                       check=True)
+        self.L = ctypes.CDLL(so)
+
+    @staticmethod
+    def _p(a):
+        return np.ascontiguousarray(a, np.float64).ctypes.data_as(ctypes.c_void_p)
+
+    def f(self, t, y, p):
+        out = np.zeros(self.D)
+        self.L.hf_f(ctypes.c_double(t), self._p(y), self._p(p),
+                    out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def vjp(self, t, y, p, v):
+        yb, pb = np.zeros(self.D), np.zeros(self.P)
+        self.L.hf_vjp(ctypes.c_double(t), self._p(y), self._p(p), self._p(v),
+                      yb.ctypes.data_as(ctypes.c_void_p), pb.ctypes.data_as(ctypes.c_void_p))
+        return yb, pb
+
+    def jac(self, y, p):
+        J = np.zeros(self.D * self.D)
+        self.L.hf_jac(self._p(y), self._p(p), J.ctypes.data_as(ctypes.c_void_p))
+        return J.reshape(self.D, self.D)
+
+    def jac_vjp(self, y, p, Jb):
+        yb, pb = np.zeros(self.D), np.zeros(self.P)
+        self.L.hf_jac_vjp(self._p(y), self._p(p), self._p(Jb),
+                          yb.ctypes.data_as(ctypes.c_void_p), pb.ctypes.data_as(ctypes.c_void_p))
+        return yb, pb
+
+
+def cs_jac(f, t, y, p, wrt="y"):
+    """Complex-step Jacobian of the Python right-hand side."""
+    h = 1e-30
+    x = y if wrt == "y" else p
+    cols = []
+    for j in range(x.shape[0]):
+        xc = x.astype(complex)
+        xc[j] += 1j * h
+        out = f(t, xc, p.astype(complex)) if wrt == "y" else f(t, y.astype(complex), xc)
+        cols.append(np.imag(np.asarray(out, dtype=complex)) / h)
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("f,dim,npar,exact", [(lorenz, 3, 1, True), (vdp, 2, 1, True),
+                                               (forced_pendulum, 2, 2, False),
+                                               (mixed, 3, 2, False), (ring, 20, 2, True)],
+                         ids=["lorenz", "vdp", "pendulum", "mixed", "ring20"])
+def test_generated_field_and_vjp(f, dim, npar, exact):
+    hf = HostField(f, dim, npar)
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        t = rng.uniform(0.0, 2.0)
+        y = rng.uniform(-2.0, 2.0, dim)
+        p = rng.uniform(0.5, 2.0, npar)
+        want = np.asarray(f(t, y, p), np.float64)
+        got = hf.f(t, y, p)
+        if exact:
+            assert np.array_equal(got, want), (got, want)
+        else:
+            np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-15)
+        v = rng.standard_normal(dim)
+        yb, pb = hf.vjp(t, y, p, v)
+        Jy, Jp = cs_jac(f, t, y, p, "y"), cs_jac(f, t, y, p, "p")
+        np.testing.assert_allclose(yb, v @ Jy, rtol=1e-13, atol=1e-13 * np.abs(v @ Jy).max())
+        np.testing.assert_allclose(pb[:npar], v @ Jp, rtol=1e-13,
+                                   atol=1e-13 * np.abs(v @ Jp).max())
+    assert hf.time_dependent == (f is forced_pendulum)
+
+
+def test_generated_lorenz_matches_builtin_expression_order():
+    """The generated Lorenz field emits the built-in field's operations in the same order
+    (gradsolve tests/test_jax_field.py:82-94 requires 1e-15; here bitwise on the host)."""
+    hf = HostField(lorenz, 3, 1)
+    rng = np.random.default_rng(1)
+    y = rng.uniform(-20.0, 20.0, (100, 3))
+    rho = rng.uniform(0.0, 40.0, 100)
+    for i in range(100):
+        yy, r = y[i], rho[i]
+        hand = np.array([10.0 * (yy[1] - yy[0]), r * yy[0] - yy[1] - yy[0] * yy[2],
+                         yy[0] * yy[1] - (8.0 / 3.0) * yy[2]])
+        assert np.array_equal(hf.f(0.0, yy, np.array([r])), hand)
+
+
+def test_generated_stiff_jacobian_and_its_vjp():
+    hf = HostField(robertson, 3, 3, stiff=True)
+    rng = np.random.default_rng(2)
+    for _ in range(30):
+        y = rng.uniform(0.0, 1.0, 3)
+        p = np.array([rng.uniform(0.02, 0.06), rng.uniform(2e7, 4e7), rng.uniform(8e3, 1.2e4)])
+        J = hf.jac(y, p)
+        np.testing.assert_allclose(J, cs_jac(robertson, 0.0, y, p, "y"), rtol=1e-14,
+                                   atol=1e-14 * np.abs(J).max())
+        Jb = rng.standard_normal((3, 3))
+        yb, pb = hf.jac_vjp(y, p, Jb.reshape(-1))
+        # J is linear in y and in p, so central differences are exact up to rounding
+        fd_y = np.zeros(3)
+        fd_p = np.zeros(3)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-3
+            fd_y[k] = np.sum(Jb * (hf.jac(y + e, p) - hf.jac(y - e, p))) / 2e-3
+            ep = np.zeros(3)
+            ep[k] = 1e-3 * p[k]
+            fd_p[k] = np.sum(Jb * (hf.jac(y, p + ep) - hf.jac(y, p - ep))) / (2e-3 * p[k])
+        np.testing.assert_allclose(yb, fd_y, rtol=1e-8, atol=1e-8 * np.abs(fd_y).max())
+        np.testing.assert_allclose(pb, fd_p, rtol=1e-7, atol=1e-7 * np.abs(fd_p).max())
+
+
+@pytest.mark.parametrize("body,name", [
+    (lambda t, y, p: y if y[0] > 0.0 else -y, "gt"),
+    (lambda t, y, p: np.array([max(y[0], y[1]), y[1]]), "gt"),
+    (lambda t, y, p: np.array([__import__("math").sin(y[0]), y[1]]), "float"),
+])
+def test_untraceable_right_hand_sides_raise(body, name):
+    with pytest.raises(UnsupportedRHS) as ei:
+        FG.trace(body, 2, 1)
+    assert ei.value.primitive_name == name
+
+
+def test_wrong_output_length_raises():
+    with pytest.raises(UnsupportedRHS) as ei:
+        FG.trace(lambda t, y, p: np.array([y[0]]), 2, 1)
+    assert ei.value.primitive_name == "shape"
+
+
+def test_stiff_field_must_be_autonomous():
+    with pytest.raises(UnsupportedRHS):
+        FG.emit_field(FG.trace(forced_pendulum, 2, 2), stiff=True)
+
+
+def test_library_registers_generated_field_and_reports_compile_errors():
+    """gsx_field_register compiles through hiprtc on the CPU: ids are issued from
+    GSX_FIELD_RUNTIME_BASE, layout queries answer for them (one extra checkpoint row for a
+    time-dependent field), and a bad source returns GSX_E_INVALID with the compiler's log."""
+    src = FG.emit_field(FG.trace(lorenz, 3, 1))
+    fid = N.field_register(src, 3, 1, 0)
+    assert fid >= N.FIELD_RUNTIME_BASE
+    assert N.call("gsx_ckpt_rows", fid, 3) == 3
+    src_t = FG.emit_field(FG.trace(forced_pendulum, 2, 2))
+    fid_t = N.field_register(src_t, 2, 2, N.FIELD_TIME_DEPENDENT)
+    assert fid_t > fid and N.call("gsx_ckpt_rows", fid_t, 2) == 3
+    with pytest.raises(ValueError, match="differs from the registered D=4"):
+        N.field_register(src, 4, 1, 0)
+    with pytest.raises(ValueError, match="error"):
+        N.field_register("struct Field { this is not C++ };", 3, 1, 0)
+    with pytest.raises(ValueError):
+        N.call("gsx_ckpt_rows", fid, 2)  # wrong D for the registered field
+    assert N.call("gsx_field_release", fid) == 0
+    with pytest.raises(ValueError):
+        N.call("gsx_ckpt_rows", fid, 3)
