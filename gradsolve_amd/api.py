@@ -309,7 +309,8 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         from . import dense
         closure = dense.make_saveat_closure(name, b, problem, y0, params, saveat, wrt=wrt_t,
                                             rtol=rtol, atol=atol, device=dev, aux=aux,
-                                            precision=precision)
+                                            precision=precision, ckpt_interval=ckpt_interval,
+                                            strict=strict)
     elif name == D.HIP_KURAMOTO and precision == "float64":
         from .closure import KuramotoClosure
         closure = KuramotoClosure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,

@@ -209,6 +209,78 @@ def make_rb23_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
                          device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
 
 
+class _SaveatFn(torch.autograd.Function):
+    """The saveat series of a Tsit5 record-once closure: forward replays the frozen mesh from
+    the given inputs and re-steps to the save times; backward runs the re-steps' VJPs and the
+    adjoint sweep with the save cotangents injected at their bracketing steps."""
+
+    @staticmethod
+    def forward(ctx, z, p_user, state):
+        y0_soa = z.detach().t().contiguous()
+        p_soa = state.binding.native_params(p_user.detach()).t().contiguous()
+        rec = state.rec
+        if state.is_recording_point(y0_soa, p_soa):
+            y_final, ckpt = rec.y_final, rec.ckpt
+        else:
+            y_final, ckpt = state.lane.replay(rec, y0_soa, p_soa)
+        sv = H.saveat(rec, state.ts, y0_soa=y0_soa, p_soa=p_soa, y_final=y_final,
+                      strict=state.lane.strict)
+        ctx.state, ctx.ckpt, ctx.p_soa, ctx.sv = state, ckpt, p_soa, sv
+        ctx.p_cols = p_user.shape[1]
+        return sv.ys.permute(2, 0, 1).contiguous()  # (n, k, D)
+
+    @staticmethod
+    def backward(ctx, g):
+        st = ctx.state
+        ysbar = g.to(torch.float64).permute(1, 2, 0).contiguous()  # [k, D, n]
+        y0bar, pbar = H.adjoint_saveat(st.rec, st.ts, ctx.sv, ysbar, ckpt=ctx.ckpt,
+                                       p_soa=ctx.p_soa, strict=st.lane.strict)
+        g_p = None
+        if ctx.needs_input_grad[1]:
+            cols = st.binding.param_cols
+            if cols is None:
+                g_p = pbar.t()[:, :ctx.p_cols].contiguous()
+            else:
+                g_p = torch.zeros((st.rec.n, ctx.p_cols), dtype=torch.float64,
+                                  device=g.device)
+                g_p[:, list(cols)] = pbar.t()
+        g_z = y0bar.t().contiguous() if ctx.needs_input_grad[0] else None
+        return g_z, g_p, None
+
+
+class Tsit5SaveatClosure(ReplayClosure):
+    """Record-once closure returning the series at the save times ``(n, k, D)``
+    (gradsolve/api.py:750-753 with gradsolve/solvers/dense.py:318-335)."""
+
+    def __init__(self, lane, binding, problem, y0, params, *, wrt, rtol, atol, device, ts,
+                 ckpt_interval=None, max_steps=100000):
+        super().__init__(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                         device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
+        self.ts = torch.as_tensor(np.asarray(ts, np.float64), device=device)
+
+    def __call__(self, *args):
+        if self.wrt == ("params",):
+            (p,), z = args, None
+        elif self.wrt == ("y0",):
+            (z,), p = args, None
+        else:
+            z, p = args
+        numpy_in = not any(isinstance(a, torch.Tensor) for a in args)
+        zt = self._y0 if z is None else self._coerce(z)
+        pt = self._params if p is None else self._coerce(p)
+        out = _SaveatFn.apply(zt, pt, self)
+        return out.detach().cpu().numpy() if numpy_in else out
+
+
+def make_tsit5_saveat_closure(binding, problem, y0, params, *, wrt, rtol, atol, device, ts,
+                              ckpt_interval=None, strict=False, aux=None,
+                              max_steps=100000) -> Tsit5SaveatClosure:
+    lane = _Tsit5Lane(binding, aux, strict)
+    return Tsit5SaveatClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol,
+                              atol=atol, device=device, ts=ts, ckpt_interval=ckpt_interval,
+                              max_steps=max_steps)
+
+
 class _KuramotoFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, z, om_user, state):

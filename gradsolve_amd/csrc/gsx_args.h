@@ -54,6 +54,57 @@ struct AdjArgs {
   const double* ybar;
   double* y0bar;
   double* pbar;
+  // Save-time cotangents (saveat gradients; tsit5_saveat_vjp_lane prepares them). With
+  // nsave > 0, mu[q] (a cotangent on the start state of the step bracketing save q) is added
+  // to lam when the reverse sweep reaches that step, save_step[q] == -1 adds it at y0, and
+  // pbar starts from pbar_in. ybar then already holds the final-state saves.
+  int64_t nsave = 0;
+  const int32_t* save_step = nullptr;  // [nsave][n]
+  const double* mu = nullptr;          // [nsave][D][n]
+  const double* pbar_in = nullptr;     // [P][n] or nullptr (start from zero)
+};
+
+// Dense output on a recorded mesh (gradsolve/solvers/dense.py:106-163): for every save time
+// ts[q] (ascending, shared by the ensemble), the accepted step s that brackets it
+// (t_s < ts[q] <= t_s + h_s), its start time and state, and the saved state, one Tsit5 step of
+// size ts[q] - t_s from there. Saves at or before t0 are y0 (save_step -1); saves at or after
+// the final time are y_final (save_step n_acc).
+struct SaveArgs {
+  int64_t n;
+  const double* y0;
+  const double* p;
+  const double* aux;
+  double t0;
+  const double* tape;     // [cap][n]
+  const int32_t* n_acc;
+  const double* ts;       // [nsave]
+  int64_t nsave;
+  double* ys;             // [nsave][D][n]
+  int32_t* save_step;     // [nsave][n]
+  double* save_t;         // [nsave][n]
+  double* states;         // [nsave][D][n] bracket start states, or nullptr
+  // [D][n] final states to copy into the final-time saves (the forward's own y_final, so that
+  // those saves equal it bit for bit), or nullptr to take the replay's
+  const double* y_final = nullptr;
+};
+
+// The re-steps' VJPs for save cotangents ysbar: mu (see AdjArgs), the final-state cotangent
+// with the final-time saves folded in (ybar_eff), and the re-steps' parameter gradient.
+struct SaveVjpArgs {
+  int64_t n;
+  const double* p;
+  const double* aux;
+  const int32_t* n_acc;
+  const double* ts;
+  int64_t nsave;
+  const int32_t* save_step;
+  const double* save_t;
+  const double* states;
+  const double* ysbar;    // [nsave][D][n]
+  const double* ybar;     // [D][n] final-state cotangent or nullptr
+  double* mu;             // [nsave][D][n]
+  double* ybar_eff;       // [D][n]
+  double* pbar;           // [P][n]
 };
 
 // Persistent work-queue forms (gsx_persist.h describes the layout).

@@ -30,6 +30,8 @@ namespace GSX_NS {
 using ::gsx::AdjArgs;
 using ::gsx::FwdArgs;
 using ::gsx::ReplayArgs;
+using ::gsx::SaveArgs;
+using ::gsx::SaveVjpArgs;
 using ::gsx::kExhausted;
 using ::gsx::kReached;
 using ::gsx::kTapeFull;
@@ -799,6 +801,108 @@ GSX_HD void tsit5_replay_lane(const ReplayArgs& a, int64_t j) {
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];
 }
 
+// Dense output for one trajectory (gradsolve/solvers/dense.py:106-163, restated as the C
+// oracle's replay_saveat): replays the recorded mesh; a save time in (t_s, t_s + h_s] is the
+// state one step of size ts[q] - t_s from (t_s, y_s), except that at or after the final time
+// it is y_final itself, and at or before t0 it is y0. save_step / save_t / states record the
+// bracket for the gradient (tsit5_saveat_vjp_lane).
+template <class F>
+GSX_HD void tsit5_saveat_lane(const SaveArgs& a, int64_t j) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t n = a.n, k = a.nsave;
+  double y[D], p[P];
+  GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.y0[d * n + j];
+  GSX_UNROLL for (int q = 0; q < P; ++q) p[q] = a.p[q * n + j];
+  auto put = [&](int64_t q, const double* v, int32_t st, double tq, const double* ys0) {
+    GSX_UNROLL for (int d = 0; d < D; ++d) a.ys[(q * D + d) * n + j] = v[d];
+    a.save_step[q * n + j] = st;
+    a.save_t[q * n + j] = tq;
+    if (a.states) {
+      GSX_UNROLL for (int d = 0; d < D; ++d) a.states[(q * D + d) * n + j] = ys0[d];
+    }
+  };
+  double t = a.t0;
+  int64_t q = 0;
+  while (q < k && a.ts[q] <= a.t0) {
+    put(q, y, -1, t, y);
+    ++q;
+  }
+  const int32_t na = a.n_acc[j];
+  for (int32_t s = 0; s < na; ++s) {
+    const double h = a.tape[static_cast<int64_t>(s) * n + j];
+    const double tn = t + h;
+    while (q < k && a.ts[q] <= tn) {
+      if (s == na - 1 && a.ts[q] >= tn) break;  // at the final time: y_final, below
+      double yq[D];
+      GSX_UNROLL for (int d = 0; d < D; ++d) yq[d] = y[d];
+      tsit5_advance<F>(t, yq, a.ts[q] - t, p, a.aux);
+      put(q, yq, s, t, y);
+      ++q;
+    }
+    tsit5_advance<F>(t, y, h, p, a.aux);
+    t = tn;
+  }
+  if (a.y_final) {
+    GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.y_final[d * n + j];
+  }
+  for (; q < k; ++q) put(q, y, na, t, y);
+}
+
+// The save cotangents' part of the gradient for one trajectory: mu[q] = (d ys[q] / d y_s)^T
+// ysbar[q] for a bracketed save (the VJP of its re-step, which also adds to the parameter
+// gradient), ysbar[q] itself for a save at y0, and the final-time saves folded into the
+// final-state cotangent. The adjoint sweep (AdjArgs with nsave) then adds mu at each bracket.
+template <class F>
+GSX_HD void tsit5_saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t n = a.n, k = a.nsave;
+  double p[P], pb[P], lf[D];
+  GSX_UNROLL for (int q = 0; q < P; ++q) {
+    p[q] = a.p[q * n + j];
+    pb[q] = -0.0;
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) lf[d] = a.ybar ? a.ybar[d * n + j] : 0.0;
+  const int32_t na = a.n_acc[j];
+  for (int64_t q = 0; q < k; ++q) {
+    const int32_t st = a.save_step[q * n + j];
+    double l[D];
+    GSX_UNROLL for (int d = 0; d < D; ++d) l[d] = a.ysbar[(q * D + d) * n + j];
+    if (st >= na) {
+      GSX_UNROLL for (int d = 0; d < D; ++d) {
+        lf[d] += l[d];
+        l[d] = 0.0;
+      }
+    } else if (st >= 0) {
+      double y[D];
+      GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.states[(q * D + d) * n + j];
+      const double ts0 = a.save_t[q * n + j];
+      tsit5_step_vjp<F>(ts0, y, a.ts[q] - ts0, p, l, pb, a.aux);
+    }
+    GSX_UNROLL for (int d = 0; d < D; ++d) a.mu[(q * D + d) * n + j] = l[d];
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.ybar_eff[d * n + j] = lf[d];
+  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q] + 0.0;
+}
+
+// Adds the save cotangents mu[q] of the saves bracketed by step s (a cotangent on y_s) to lam,
+// once the reverse sweep has turned lam into dL/dy_s; s = -1 takes the saves at or before t0.
+// `q` walks the ascending saves downwards; final-time saves (save_step == n_acc) are skipped,
+// their cotangent is already in ybar.
+template <int D>
+GSX_HD void inject_saves(const AdjArgs& a, int64_t j, int32_t s, int32_t& q, double* lam) {
+  if (a.nsave == 0) return;
+  const int64_t n = a.n;
+  while (q >= 0) {
+    const int32_t st = a.save_step[static_cast<int64_t>(q) * n + j];
+    if (st < s) break;
+    if (st == s) {
+      GSX_UNROLL for (int d = 0; d < D; ++d)
+        lam[d] += a.mu[(static_cast<int64_t>(q) * D + d) * n + j];
+    }
+    --q;
+  }
+}
+
 template <class F>
 GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
   constexpr int D = F::D, P = F::P;
@@ -807,9 +911,10 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
   double p[P], pb[P], lam[D];
   GSX_UNROLL for (int q = 0; q < P; ++q) {
     p[q] = a.p[q * n + j];
-    pb[q] = 0.0;
+    pb[q] = a.pbar_in ? a.pbar_in[q * n + j] : 0.0;
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  int32_t qs = static_cast<int32_t>(a.nsave) - 1;  // save cursor, walked downwards
   const int32_t na = a.n_acc[j];
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
@@ -843,8 +948,10 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
       if constexpr (!F::autonomous) t = slot[D * sstride];
       const double h = a.tape[static_cast<int64_t>(s) * n + j];
       tsit5_step_vjp<F>(t, y, h, p, lam, pb, a.aux);
+      inject_saves<D>(a, j, s, qs, lam);
     }
   }
+  inject_saves<D>(a, j, -1, qs, lam);
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }
@@ -858,7 +965,7 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
 // the kernel: all active lanes read checkpoint row m and tape rows 2m, 2m+1 together.
 template <class F>
 GSX_HD void tsit5_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32_t na,
-                                   const double* p, double* lam, double* pb) {
+                                   const double* p, double* lam, double* pb, int32_t& qs) {
   constexpr int D = F::D;
   constexpr int DCK = ckpt_rows<F>();
   const int64_t n = a.n;
@@ -878,7 +985,31 @@ GSX_HD void tsit5_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32
     tsit5_stage_inputs<F>(t + h0, y1, h1, p, z1, nullptr, a.aux);
     tsit5_step_vjp_z<F>(t + h0, z1, h1, p, lam, pb, a.aux);
   }
+  if (s0 + 1 < na) inject_saves<D>(a, j, s0 + 1, qs, lam);
   tsit5_step_vjp_z<F>(t, z0, h0, p, lam, pb, a.aux);
+  inject_saves<D>(a, j, s0, qs, lam);
+}
+
+// The pair adjoint of one lane. `mmax` is at least the lane's segment count; the kernel passes
+// the wavefront's largest, so every lane runs the loop in step and idles above its own count.
+template <class F>
+GSX_HD void tsit5_adj_pair_lane(const AdjArgs& a, int64_t j, int32_t na, int32_t mmax) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t n = a.n;
+  double p[P], pb[P], lam[D];
+  GSX_UNROLL for (int q = 0; q < P; ++q) {
+    p[q] = a.p[q * n + j];
+    pb[q] = a.pbar_in ? a.pbar_in[q * n + j] : 0.0;
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  int32_t qs = static_cast<int32_t>(a.nsave) - 1;
+  const int32_t nseg = (na + 1) / 2;
+  for (int32_t m = mmax - 1; m >= 0; --m) {
+    if (m < nseg) tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb, qs);
+  }
+  inject_saves<D>(a, j, -1, qs, lam);
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
+  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }
 
 }  // namespace GSX_NS

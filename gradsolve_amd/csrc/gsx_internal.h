@@ -27,6 +27,8 @@ bool field_info(int field, int D, FieldInfo* out);
 int tsit5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int tsit5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int tsit5_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s);
+int tsit5_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s);
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
@@ -50,6 +52,8 @@ bool info(int field, FieldInfo* out);
 int tsit5_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
 int tsit5_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
+int tsit5_saveat(int field, Family fam, const SaveArgs& a, hipStream_t s);
+int tsit5_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s);
 int rb23_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
 int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
 int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
@@ -61,6 +65,8 @@ namespace gsx_strict {
 int tsit5_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int tsit5_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int tsit5_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+int tsit5_saveat_dispatch(int field, int D, const ::gsx::SaveArgs& a, hipStream_t s);
+int tsit5_saveat_vjp_dispatch(int field, int D, const ::gsx::SaveVjpArgs& a, hipStream_t s);
 int rb23_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);

@@ -55,13 +55,15 @@ const FamilySpec& spec(Family f) {
         {"replay", "tsit5_replay_kernel<%s>"}, {"adj1", "tsit5_adj_kernel<%s, 1>"},
         {"adj_pair", "tsit5_adj_pair_kernel<%s>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
         {"adj8", "tsit5_adj_kernel<%s, 8>"}, {"adj16", "tsit5_adj_kernel<%s, 16>"},
-        {"adj32", "tsit5_adj_kernel<%s, 32>"}}},
+        {"adj32", "tsit5_adj_kernel<%s, 32>"}, {"saveat", "tsit5_saveat_kernel<%s>"},
+        {"saveat_vjp", "tsit5_saveat_vjp_kernel<%s>"}}},
       {"gsx_strict", "gsx_tsit5_kernels.h", kStrict,
        {{"fwd_rec", "tsit5_fwd_kernel<%s, true>"}, {"fwd", "tsit5_fwd_kernel<%s, false>"},
         {"replay", "tsit5_replay_kernel<%s>"}, {"adj1", "tsit5_adj_kernel<%s, 1>"},
         {"adj_pair", "tsit5_adj_pair_kernel<%s>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
         {"adj8", "tsit5_adj_kernel<%s, 8>"}, {"adj16", "tsit5_adj_kernel<%s, 16>"},
-        {"adj32", "tsit5_adj_kernel<%s, 32>"}}},
+        {"adj32", "tsit5_adj_kernel<%s, 32>"}, {"saveat", "tsit5_saveat_kernel<%s>"},
+        {"saveat_vjp", "tsit5_saveat_vjp_kernel<%s>"}}},
       {"gsx", "gsx_tsit5_kernels.h", {},
        {{"fwd", "tsit5_fwd_f32_kernel<%s>"}, {"replay", "tsit5_replay_f32_kernel<%s>"},
         {"adj1", "tsit5_adj_f32_kernel<%s, 1>"}, {"adj4", "tsit5_adj_f32_kernel<%s, 4>"},
@@ -313,6 +315,14 @@ int tsit5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
                      a.ckpt_interval, kMaxCkptInterval);
   const bool is_pair = std::strcmp(key, "adj_pair") == 0;
   return launch(field, fam, key, a.n, is_pair ? kFwdBlock : kAdjBlock, a, s);
+}
+
+int tsit5_saveat(int field, Family fam, const SaveArgs& a, hipStream_t s) {
+  return launch(field, fam, "saveat", a.n, kFwdBlock, a, s);
+}
+
+int tsit5_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s) {
+  return launch(field, fam, "saveat_vjp", a.n, kFwdBlock, a, s);
 }
 
 int rb23_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s) {

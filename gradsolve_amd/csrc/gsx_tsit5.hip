@@ -96,6 +96,22 @@ int launch_adj_f32(const AdjArgs& a, hipStream_t s) {
                    kMaxCkptInterval);
 }
 
+template <class F>
+int launch_saveat(const SaveArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  tsit5_saveat_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("tsit5_saveat_kernel");
+}
+
+template <class F>
+int launch_saveat_vjp(const SaveVjpArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  tsit5_saveat_vjp_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("tsit5_saveat_vjp_kernel");
+}
+
 #if defined(GSX_STRICT_ARITH)
 constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kTsit5Strict;
 #else
@@ -192,6 +208,43 @@ int tsit5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
   }
   return set_error(GSX_E_UNSUPPORTED, "tsit5 adjoint: field %d with D=%d is not built", field,
                    D);
+}
+
+int tsit5_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: return launch_saveat<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_saveat<FLorenz3>(a, s);
+    case GSX_FIELD_VDP: return launch_saveat<FVdp>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_saveat<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_saveat<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_saveat<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field)) return ::gsx::rt::tsit5_saveat(field, kRtFamily, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "tsit5 saveat: field %d with D=%d is not built", field,
+                   D);
+}
+
+int tsit5_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_LORENZ: return launch_saveat_vjp<FLorenz>(a, s);
+    case GSX_FIELD_LORENZ3: return launch_saveat_vjp<FLorenz3>(a, s);
+    case GSX_FIELD_VDP: return launch_saveat_vjp<FVdp>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_saveat_vjp<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_saveat_vjp<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_saveat_vjp<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field))
+        return ::gsx::rt::tsit5_saveat_vjp(field, kRtFamily, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "tsit5 saveat adjoint: field %d with D=%d is not built",
+                   field, D);
 }
 
 }  // namespace GSX_NS

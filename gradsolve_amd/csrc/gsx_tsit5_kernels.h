@@ -68,29 +68,32 @@ constexpr int pair_waves_per_eu() { return (F::D <= 3 && F::P == 1) ? 4 : 1; }
 template <class F>
 __global__ __launch_bounds__(kFwdBlock) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<F>())))
 void tsit5_adj_pair_kernel(AdjArgs a) {
-  constexpr int D = F::D, P = F::P;
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
   const bool valid = j < a.n;
   const int32_t na = valid ? a.n_acc[j] : 0;
-  const int32_t nseg = (na + 1) / 2;
-  int32_t mmax = nseg;
+  int32_t mmax = (na + 1) / 2;
   GSX_UNROLL for (int off = 32; off >= 1; off >>= 1) {
     const int32_t o = __shfl_xor(mmax, off);
     mmax = o > mmax ? o : mmax;
   }
   if (!valid) return;
-  const int64_t n = a.n;
-  double p[P], pb[P], lam[D];
-  GSX_UNROLL for (int q = 0; q < P; ++q) {
-    p[q] = a.p[q * n + j];
-    pb[q] = 0.0;
-  }
-  GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
-  for (int32_t m = mmax - 1; m >= 0; --m) {
-    if (m < nseg) tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb);
-  }
-  GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
-  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
+  tsit5_adj_pair_lane<F>(a, j, na, mmax);
+}
+
+// Dense output on a recorded mesh (tsit5_saveat_lane) and the VJPs of its re-steps
+// (tsit5_saveat_vjp_lane), one trajectory per lane.
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void tsit5_saveat_kernel(SaveArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  tsit5_saveat_lane<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void tsit5_saveat_vjp_kernel(SaveVjpArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  tsit5_saveat_vjp_lane<F>(a, j);
 }
 
 }  // namespace GSX_NS

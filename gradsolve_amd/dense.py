@@ -9,7 +9,9 @@ step of size ``ts[q] - t_s`` from that step's start state. Times at or before ``
 as the mesh replay, with the step sizes as data.
 
 This build serves saveat on the Kuramoto engine (the problem the reference's dense lane is
-benchmarked on); a request on another field raises ``ValueError`` naming what is built.
+benchmarked on) and on the Tsit5 lanes of every other non-stiff field, built-in or compiled at
+run time (``gsx_tsit5_saveat`` / ``gsx_tsit5_adjoint_saveat``); the stiff lane and FP32 dense
+output outside the Kuramoto engine raise ``ValueError`` naming what is built.
 """
 from __future__ import annotations
 
@@ -22,20 +24,49 @@ from . import kuramoto as KU
 from .base import SolveResult
 
 
-def _check(name, b, precision):
-    if b.field_id != N.FIELD_KURAMOTO or name != D.HIP_KURAMOTO:
-        raise ValueError(
-            f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
-            "dense output on the Kuramoto engine (hip_kuramoto). Drop saveat for a final-"
-            "state solve or gradient.")
+def _check(name, b, precision) -> str:
+    if b.field_id == N.FIELD_KURAMOTO and name == D.HIP_KURAMOTO:
+        return "kuramoto"
+    if name in (D.HIP_TSIT5, D.HIP_REPLAY) and not b.stiff:
+        if precision == "float32":
+            raise ValueError(
+                f"saveat with precision='float32' is built for the Kuramoto engine only; field "
+                f"{b.key!r} serves dense output in float64")
+        return "tsit5"
+    raise ValueError(
+        f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
+        "dense output on the Tsit5 lanes (hip_tsit5 / hip_replay) and the Kuramoto engine "
+        "(hip_kuramoto). Drop saveat for a final-state solve or gradient.")
+
+
+def _solve_saveat_tsit5(b, problem, y0, params, ts, *, rtol, atol, dev, max_steps):
+    from . import hip_tsit5 as H
+    y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
+    p_t = torch.as_tensor(np.asarray(params, np.float64), device=dev)
+    p_soa = b.native_params(p_t).t().contiguous()
+    aux = None if b.aux_of is None else torch.as_tensor(b.aux_of(problem), device=dev)
+    rec = H.record(b.field_id, y0_soa, p_soa, t0=problem.t0, t1=problem.t1, rtol=rtol,
+                   atol=atol, max_steps=max_steps, aux=aux)
+    sv = H.saveat(rec, torch.as_tensor(ts, device=dev), keep_states=False)
+    status = rec.status.cpu().numpy()
+    y_final = rec.y_final.t().cpu().numpy().copy()
+    y_saved = sv.ys.permute(2, 0, 1).cpu().numpy()
+    y_final[status != 0] = np.nan
+    y_saved[status != 0] = np.nan
+    return SolveResult(y_final=y_final, accepted_steps=rec.n_acc.cpu().numpy().astype(np.int64),
+                       rejected_steps=rec.n_rej.cpu().numpy().astype(np.int64),
+                       solver=D.HIP_TSIT5, y_saved=y_saved, ts_saved=ts)
 
 
 def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
                  precision="float64", max_steps=100000) -> SolveResult:
     from .api import _device
-    _check(name, b, precision)
+    lane = _check(name, b, precision)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
     dev = _device(device)
+    if lane == "tsit5":
+        return _solve_saveat_tsit5(b, problem, y0, params, ts, rtol=rtol, atol=atol, dev=dev,
+                                   max_steps=max_steps)
     y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
     om_soa = torch.as_tensor(np.asarray(params, np.float64), device=dev).t().contiguous()
     K = torch.as_tensor(b.aux_of(problem), device=dev)
@@ -54,10 +85,15 @@ def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
 
 
 def make_saveat_closure(name, b, problem, y0, params, saveat, *, wrt, rtol, atol, device,
-                        aux=None, precision="float64", max_steps=100000):
-    from .closure import KuramotoClosure
-    _check(name, b, precision)
+                        aux=None, precision="float64", max_steps=100000, ckpt_interval=None,
+                        strict=False):
+    from .closure import KuramotoClosure, make_tsit5_saveat_closure
+    lane = _check(name, b, precision)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
+    if lane == "tsit5":
+        return make_tsit5_saveat_closure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                                         device=device, ts=ts, ckpt_interval=ckpt_interval,
+                                         strict=strict, aux=aux, max_steps=max_steps)
     if aux is None:
         aux = torch.as_tensor(b.aux_of(problem), device=device)
     return KuramotoClosure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,

@@ -147,6 +147,38 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux,
                       const double* ybar, double* y0bar, double* pbar, int flags,
                       void* stream);
 
+/* Dense output on a recorded mesh (saveat) for any Tsit5 field: replays the tape and, for each
+ * save time ts[q] (ascending, shared by the ensemble), writes ys[q][d][j]: y0 for ts[q] <= t0,
+ * y_final for ts[q] at or after the final time, and otherwise one Tsit5 step of size
+ * ts[q] - t_s from the start (t_s, y_s) of the accepted step with t_s < ts[q] <= t_s + h_s.
+ * The bracket is returned for the gradient: save_step[q][j] (-1 at y0, n_acc at y_final),
+ * save_t[q][j] = t_s, and (states non-NULL) states[q][d][j] = y_s. y_final (optional) is
+ * copied into the final-time saves, so that they equal the forward's final state bit for bit
+ * (NULL: the replay's). flags: 0 or GSX_STRICT. Replaces replay_solve_saveat_jax
+ * (gradsolve/warp/warp_replay.py:58-75 -> gradsolve/solvers/dense.py:106-163) for the Tsit5
+ * lanes. */
+int gsx_tsit5_saveat(int field, int64_t n, int D, int P, const double* aux,
+                     const double* y0, const double* p, double t0,
+                     const double* tape, const int32_t* n_acc, const double* y_final,
+                     const double* ts, int64_t nsave, double* ys, int32_t* save_step,
+                     double* save_t, double* states, int flags, void* stream);
+
+/* Gradient of a saveat series (and optionally of the final state): the VJPs of the re-steps
+ * (mu, a caller scratch buffer [nsave][D][n]) followed by the adjoint sweep of
+ * gsx_tsit5_adjoint with each save's cotangent added at its bracketing step. ysbar [nsave][D][n]
+ * is dL/dys, ybar (NULL: zero) dL/dy_final; save_step / save_t / states as gsx_tsit5_saveat
+ * wrote them; tape, ckpt, tape_cap and ckpt_interval as gsx_tsit5_fwd wrote them. Outputs
+ * dL/dy0 and dL/dp. flags: 0 or GSX_STRICT. Replaces jax.grad through replay_solve_saveat_jax
+ * (gradsolve/solvers/dense.py:318-335). */
+int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                             const double* y0, const double* p, double t0,
+                             const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                             const double* ckpt, int32_t ckpt_interval,
+                             const double* ts, int64_t nsave, const int32_t* save_step,
+                             const double* save_t, const double* states, const double* ysbar,
+                             const double* ybar, double* mu, double* y0bar, double* pbar,
+                             int flags, void* stream);
+
 /* Fused adaptive Rosenbrock23 (ode23s, autonomous) forward + accepted-h record for stiff
  * fields with an analytic Jacobian: W = I - d*h*J(y, p) factorised once per step (LU with
  * partial pivoting in registers) and reused by the three stage solves; I-controller with

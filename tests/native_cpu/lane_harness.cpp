@@ -30,21 +30,29 @@ int run_adj(const AdjArgs& a) {
 }
 template <class F>
 int run_adj_pair(const AdjArgs& a) {
-  constexpr int D = F::D, P = F::P;
   for (int64_t j = 0; j < a.n; ++j) {
-    double p[P], pb[P], lam[D];
-    for (int q = 0; q < P; ++q) {
-      p[q] = a.p[q * a.n + j];
-      pb[q] = 0.0;
-    }
-    for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * a.n + j];
     const int32_t na = a.n_acc[j];
-    for (int32_t m = (na + 1) / 2 - 1; m >= 0; --m)
-      tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb);
-    for (int d = 0; d < D; ++d) a.y0bar[d * a.n + j] = lam[d];
-    for (int q = 0; q < P; ++q) a.pbar[q * a.n + j] = pb[q];
+    tsit5_adj_pair_lane<F>(a, j, na, (na + 1) / 2);
   }
   return 0;
+}
+template <class F>
+int run_saveat(const SaveArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) tsit5_saveat_lane<F>(a, j);
+  return 0;
+}
+// The saveat gradient as gsx_tsit5_adjoint_saveat runs it: the re-steps' VJPs, then the
+// segment (C != 2) or pair (C == 2) adjoint with the saves injected.
+template <class F>
+int run_adj_saveat(const SaveVjpArgs& v, AdjArgs a) {
+  for (int64_t j = 0; j < v.n; ++j) tsit5_saveat_vjp_lane<F>(v, j);
+  a.ybar = v.ybar_eff;
+  a.nsave = v.nsave;
+  a.save_step = v.save_step;
+  a.mu = v.mu;
+  a.pbar_in = v.pbar;
+  if (a.ckpt_interval == 2) return run_adj_pair<F>(a);
+  return run_adj<F>(a);
 }
 template <class F>
 int run_rb_fwd(const FwdArgs& a) {
@@ -167,6 +175,28 @@ int lh_tsit5_adjoint_pair(int field, int64_t n, const double* p, double t0, cons
   if (field == 6) return run_adj_pair<FLorenz3>(a);
   if (field == 5) return run_adj_pair<FVdp>(a);
   if (field == 2) return run_adj_pair<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_tsit5_saveat(int field, int64_t n, const double* y0, const double* p, double t0,
+                    const double* tape, const int32_t* n_acc, const double* ts, int64_t k,
+                    double* ys, int32_t* save_step, double* save_t, double* states) {
+  SaveArgs a{n, y0, p, nullptr, t0, tape, n_acc, ts, k, ys, save_step, save_t, states,
+             nullptr};
+  if (field == 1) return run_saveat<FLorenz>(a);
+  if (field == 2) return run_saveat<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_tsit5_adjoint_saveat(int field, int64_t n, const double* p, double t0,
+                            const double* tape, int64_t cap, const int32_t* n_acc,
+                            const double* ckpt, int32_t C, const double* ts, int64_t k,
+                            const int32_t* save_step, const double* save_t,
+                            const double* states, const double* ysbar, const double* ybar,
+                            double* mu, double* y0bar, double* pbar) {
+  SaveVjpArgs v{n, p, nullptr, n_acc, ts, k, save_step, save_t, states, ysbar, ybar, mu,
+                y0bar, pbar};
+  AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  if (field == 1) return run_adj_saveat<FLorenz>(v, a);
+  if (field == 2) return run_adj_saveat<FLinearDecay<2>>(v, a);
   return -1;
 }
 int lh_tsit5_adjoint(int field, int64_t n, const double* p, double t0, const double* tape,

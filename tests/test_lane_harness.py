@@ -237,3 +237,55 @@ def test_pair_adjoint_equals_generic_adjoint(lh):
                              ybar.ctypes.data, q0.ptr, qp.ptr)
     assert q0.intact() and qp.intact()
     assert np.array_equal(q0.a, g0.a) and np.array_equal(qp.a, gp.a)
+
+
+@pytest.mark.parametrize("C", [2, 8])
+def test_saveat_and_its_gradient_match_oracle(lh, C):
+    """The dense-output lanes (gsx_tsit5_saveat / gsx_tsit5_adjoint_saveat's per-trajectory
+    bodies, strict build) against the oracle's replay_saveat / adjoint_saveat: the saved
+    states bit for bit (saves before t0, interior, exactly at t1 and past it), the gradient of
+    a random save-series cotangent plus a final-state cotangent to 1e-12, through the pair
+    (C = 2) and the segment (C = 8) adjoints."""
+    vp, i64, i32, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, \
+        ctypes.c_int
+    lh.lh_tsit5_saveat.argtypes = [i, i64, vp, vp, d, vp, vp, vp, i64, vp, vp, vp, vp]
+    lh.lh_tsit5_adjoint_saveat.argtypes = [i, i64, vp, d, vp, i64, vp, vp, i32, vp, i64, vp,
+                                           vp, vp, vp, vp, vp, vp, vp]
+    e = W.lorenz(64, seed=31, noise=0.1)
+    n, cap = 64, 256
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape = Guarded((cap, n))
+    ck = Guarded((cap // C + 1, 3, n))
+    lh.lh_tsit5_fwd(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, 1.0, None, 1e-8, 1e-8, 100000,
+                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C)
+    ts = np.array([-0.5, 0.0, 1e-4, 0.013, 0.25, 0.5, 0.5, 0.77, 0.999, 1.0, 1.25])
+    k = ts.shape[0]
+    ys, sstep, st_t, states = Guarded((k, 3, n)), Guarded((k, n), np.int32), \
+        Guarded((k, n)), Guarded((k, 3, n))
+    lh.lh_tsit5_saveat(1, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
+                       ts.ctypes.data, k, ys.ptr, sstep.ptr, st_t.ptr, states.ptr)
+    assert all(g.intact() for g in (ys, sstep, st_t, states))
+    padded = np.zeros((n, int(na.a.max())))
+    for j in range(n):
+        padded[j, :na.a[j]] = tape.a[:na.a[j], j]
+    oyf, oys = O.tsit5_replay_saveat(O.LORENZ, e.y0, e.params, 0.0, padded, ts)
+    assert np.array_equal(ys.a.transpose(2, 0, 1), oys)
+    assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, e.y0)
+    assert np.all(sstep.a[0] == -1) and np.all(sstep.a[-1] == na.a)
+    rng = np.random.default_rng(32)
+    ysbar = rng.standard_normal((n, k, 3))
+    ybar = rng.standard_normal((n, 3))
+    mu = Guarded((k, 3, n))
+    g0, gp = Guarded((3, n)), Guarded((1, n))
+    ysb_soa = np.ascontiguousarray(ysbar.transpose(1, 2, 0))
+    ybar_soa = np.ascontiguousarray(ybar.T)
+    lh.lh_tsit5_adjoint_saveat(1, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+                               ts.ctypes.data, k, sstep.ptr, st_t.ptr, states.ptr,
+                               ysb_soa.ctypes.data, ybar_soa.ctypes.data, mu.ptr, g0.ptr, gp.ptr)
+    assert all(g.intact() for g in (mu, g0, gp))
+    og0, ogp = O.tsit5_adjoint_saveat(O.LORENZ, e.y0, e.params, 0.0, padded, ts, ysbar, ybar)
+    np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=1e-12 * np.abs(ogp).max())
+    np.testing.assert_allclose(g0.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
