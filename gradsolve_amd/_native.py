@@ -77,6 +77,8 @@ SIGNATURES = {
                                      _vp, _vp]),
     "gsx_tape_to_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
     "gsx_tape_from_padded": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "gsx_tape_zero_tail": (_i, [_i64, _vp, _i64, _vp, _i64, _vp]),
+    "gsx_tape_count": (_i, [_i64, _vp, _i64, _vp, _vp]),
     "gsx_tsit5_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i64,
                               _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_tsit5_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,

@@ -69,6 +69,28 @@ __global__ void tape_from_padded_kernel(int64_t n, const double* __restrict__ in
   n_acc[j] = last;
 }
 
+// Rows n_acc[j] <= s < S of column j become 0.0, so that the leading S rows of the step-major
+// tape are the reference's trimmed [S, n] record (a zero step is the replay's identity).
+__global__ void tape_zero_tail_kernel(int64_t n, double* __restrict__ tape,
+                                      const int32_t* __restrict__ n_acc, int64_t S) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n * S) return;
+  const int64_t s = idx / n;
+  const int64_t j = idx - s * n;
+  if (s >= n_acc[j]) tape[idx] = 0.0;
+}
+
+// n_acc[j] = 1 + the last row s < S with tape[s][j] != 0 (accepted steps are positive).
+__global__ void tape_count_kernel(int64_t n, const double* __restrict__ tape, int64_t S,
+                                  int32_t* __restrict__ n_acc) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int32_t last = 0;
+  for (int64_t s = 0; s < S; ++s)
+    if (tape[s * n + j] != 0.0) last = static_cast<int32_t>(s + 1);
+  n_acc[j] = last;
+}
+
 int check_field(int field, int D, int P, FieldInfo* fi) {
   if (!field_info(field, D, fi))
     return set_error(GSX_E_INVALID, "field %d is not built for D=%d", field, D);
@@ -654,6 +676,29 @@ int gsx_tape_from_padded(int64_t n, const double* in, int64_t S, double* tape,
   tape_from_padded_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(n, in, S, tape,
                                                                               tape_cap, n_acc);
   return check_launch("tape_from_padded_kernel");
+}
+
+int gsx_tape_zero_tail(int64_t n, double* tape, int64_t tape_cap, const int32_t* n_acc,
+                       int64_t S, void* stream) {
+  if (n < 0 || S < 0 || S > tape_cap)
+    return set_error(GSX_E_INVALID, "tape_zero_tail: n %lld, S %lld, cap %lld",
+                     static_cast<long long>(n), static_cast<long long>(S),
+                     static_cast<long long>(tape_cap));
+  if (n == 0 || S == 0) return GSX_OK;
+  if (!tape || !n_acc) return set_error(GSX_E_INVALID, "tape_zero_tail: null pointer");
+  const int64_t total = n * S;
+  const unsigned grid = static_cast<unsigned>((total + 255) / 256);
+  tape_zero_tail_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(n, tape, n_acc, S);
+  return check_launch("tape_zero_tail_kernel");
+}
+
+int gsx_tape_count(int64_t n, const double* tape, int64_t S, int32_t* n_acc, void* stream) {
+  if (n < 0 || S < 0) return set_error(GSX_E_INVALID, "tape_count: negative size");
+  if (n == 0) return GSX_OK;
+  if ((S > 0 && !tape) || !n_acc) return set_error(GSX_E_INVALID, "tape_count: null pointer");
+  const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+  tape_count_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(n, tape, S, n_acc);
+  return check_launch("tape_count_kernel");
 }
 
 }  // extern "C"

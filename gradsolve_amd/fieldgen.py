@@ -13,11 +13,14 @@ are generated by reverse-mode differentiation of the graph, with the primal sube
 shared.
 
 Supported: + - * / unary -, ``**`` with a numeric exponent (integer exponents up to 8 in
-magnitude become products, as XLA's integer_pow does) or a symbolic one, abs, and the
-functions sin cos tan exp log sqrt tanh sinh cosh arctan log1p expm1 square (as NumPy ufuncs
-or the functions of this module). Anything that needs a symbol's truth value (``if``,
-comparisons, ``max``/``min``) raises ``UnsupportedRHS`` naming the construct, as the
-reference's translator does for primitives outside its subset.
+magnitude become products, as XLA's integer_pow does) or a symbolic one, abs, the functions
+sin cos tan exp log sqrt tanh sinh cosh arctan log1p expm1 square (as NumPy ufuncs or the
+functions of this module), comparisons < <= > >= (traced 0/1 values, as the reference's
+translator emits its lt/le/gt/ge primitives), and this module's ``where``, ``maximum``,
+``minimum`` and ``sign`` (the reference's select, max, min, sign). Anything that needs a
+symbol's truth value (``if``, ``max()``/``min()``, ``np.maximum`` on object arrays) raises
+``UnsupportedRHS`` naming the construct, as the reference's translator does for primitives
+outside its subset (``cond``).
 """
 from __future__ import annotations
 
@@ -138,14 +141,28 @@ class Sym:
 
     def _no(self, what):
         raise UnsupportedRHS(what, f"{what} on a traced value: the right-hand side must be "
-                                   "straight-line arithmetic (no branches, comparisons, "
-                                   "max/min or rounding on the state or parameters)")
+                                   "straight-line arithmetic; branch with "
+                                   "gradsolve_amd.fieldgen.where / maximum / minimum "
+                                   "instead of if, max(), min() or np.maximum")
 
     def __bool__(self): self._no("cond")
-    def __lt__(self, o): self._no("lt")
-    def __le__(self, o): self._no("le")
-    def __gt__(self, o): self._no("gt")
-    def __ge__(self, o): self._no("ge")
+
+    # Comparisons are traced values (1.0 where true, 0.0 where false), usable in arithmetic
+    # and as the condition of where(); their truth value cannot steer Python control flow.
+    def _cmp(self, op, other):
+        if isinstance(other, np.ndarray) and other.ndim > 0:
+            mine = np.empty(other.shape, dtype=object)
+            mine[...] = self
+            return np.frompyfunc(lambda a, b: a._cmp(op, b), 2, 1)(mine, other)
+        o = self._lift(other)
+        if o is NotImplemented:
+            return NotImplemented
+        return self.g.node(op, (self, o))
+
+    def __lt__(self, o): return self._cmp("lt", o)
+    def __le__(self, o): return self._cmp("le", o)
+    def __gt__(self, o): return self._cmp("gt", o)
+    def __ge__(self, o): return self._cmp("ge", o)
     def __float__(self): self._no("float")
     def __int__(self): self._no("int")
     def __index__(self): self._no("index")
@@ -170,6 +187,55 @@ def _fn(name):
         return getattr(np, name)(x)  # numbers and arrays (object arrays call the methods)
     f.__name__ = name
     return f
+
+
+def _lift_pair(a, b):
+    g = a.g if isinstance(a, Sym) else b.g
+    la = a if isinstance(a, Sym) else g.const(float(a))
+    lb = b if isinstance(b, Sym) else g.const(float(b))
+    return g, la, lb
+
+
+def where(cond, a, b):
+    """Elementwise ``a`` where ``cond`` is nonzero, else ``b`` (traced select; NumPy values
+    pass through to ``np.where``)."""
+    if isinstance(cond, np.ndarray) and cond.dtype == object:
+        return np.frompyfunc(where, 3, 1)(cond, a, b)
+    if not isinstance(cond, Sym):
+        return np.where(cond, a, b)
+    g = cond.g
+    la = a if isinstance(a, Sym) else g.const(float(a))
+    lb = b if isinstance(b, Sym) else g.const(float(b))
+    return g.node("select", (cond, la, lb))
+
+
+def maximum(a, b):
+    """Elementwise maximum (traced; NumPy values pass through to ``np.maximum``)."""
+    if any(isinstance(x, np.ndarray) and x.dtype == object for x in (a, b)):
+        return np.frompyfunc(maximum, 2, 1)(a, b)
+    if not (isinstance(a, Sym) or isinstance(b, Sym)):
+        return np.maximum(a, b)
+    g, la, lb = _lift_pair(a, b)
+    return g.node("max", (la, lb))
+
+
+def minimum(a, b):
+    """Elementwise minimum (traced; NumPy values pass through to ``np.minimum``)."""
+    if any(isinstance(x, np.ndarray) and x.dtype == object for x in (a, b)):
+        return np.frompyfunc(minimum, 2, 1)(a, b)
+    if not (isinstance(a, Sym) or isinstance(b, Sym)):
+        return np.minimum(a, b)
+    g, la, lb = _lift_pair(a, b)
+    return g.node("min", (la, lb))
+
+
+def sign(x):
+    """sign(x) in {-1, 0, 1} (traced; NumPy values pass through to ``np.sign``)."""
+    if isinstance(x, np.ndarray) and x.dtype == object:
+        return np.frompyfunc(sign, 1, 1)(x)
+    if isinstance(x, Sym):
+        return x.g.node("sign", (x,))
+    return np.sign(x)
 
 
 sin, cos, tan, exp, log, sqrt = (_fn(n) for n in ("sin", "cos", "tan", "exp", "log", "sqrt"))
@@ -324,8 +390,19 @@ def _local(g: Graph, s: Sym, gbar: Sym) -> List[Tuple[Sym, Sym]]:
         return [(x, g.node("div", (gbar, _add(g, c(1.0), g.node("mul", (x, x))))))]
     if op == "abs":
         return [(x, _mul(g, gbar, g.node("sign", (x,))))]
-    if op == "sign":
+    if op in ("sign", "lt", "le", "gt", "ge"):
         return []
+    if op == "select":  # cond carries no gradient
+        z = c(0.0)
+        return [(a[1], g.node("select", (a[0], gbar, z))),
+                (a[2], g.node("select", (a[0], z, gbar)))]
+    if op in ("max", "min"):  # the larger (smaller) argument takes the cotangent; ties split
+        u, w = a
+        u_wins = g.node("gt" if op == "max" else "lt", (u, w))
+        w_wins = g.node("lt" if op == "max" else "gt", (u, w))
+        share_u = g.node("select", (u_wins, c(1.0), g.node("select", (w_wins, c(0.0), c(0.5)))))
+        share_w = g.node("select", (w_wins, c(1.0), g.node("select", (u_wins, c(0.0), c(0.5)))))
+        return [(u, _mul(g, gbar, share_u)), (w, _mul(g, gbar, share_w))]
     if op == "ipow":
         n = s.value
         if n == 0:
@@ -440,6 +517,15 @@ class _Emitter:
             return prod if n > 0 else f"T(1.0) / ({prod})"
         if op == "sign":  # sign(0) = 0, the derivative JAX takes for |x| at 0
             return f"T(({a[0]} > T(0.0)) - ({a[0]} < T(0.0)))"
+        if op in ("lt", "le", "gt", "ge"):
+            sym = {"lt": "<", "le": "<=", "gt": ">", "ge": ">="}[op]
+            return f"({a[0]} {sym} {a[1]} ? T(1.0) : T(0.0))"
+        if op == "select":
+            return f"({a[0]} != T(0.0) ? {a[1]} : {a[2]})"
+        if op == "max":  # NaN-propagating, as jnp.maximum
+            return f"({a[0]} != {a[0]} || {a[0]} > {a[1]} ? {a[0]} : {a[1]})"
+        if op == "min":
+            return f"({a[0]} != {a[0]} || {a[0]} < {a[1]} ? {a[0]} : {a[1]})"
         if op in _CFUN:
             return f"{_CFUN[op]}({a[0]})"
         raise UnsupportedRHS(op, f"cannot emit {op}")

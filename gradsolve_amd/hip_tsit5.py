@@ -22,6 +22,7 @@ from dataclasses import dataclass
 from typing import Optional
 
 import torch
+import torch.utils.dlpack
 
 from . import _native as N
 
@@ -90,6 +91,16 @@ class Tsit5Record:
 
     def max_steps_taken(self) -> int:
         return int(self.n_acc.max().item()) if self.n else 0
+
+    def trimmed_tape(self) -> torch.Tensor:
+        """The reference's trimmed step-major record ``[S, n]`` (S = max n_acc, zero past each
+        trajectory's n_acc), as a zero-copy view of this record's tape: hand it to JAX with
+        ``jax.dlpack.from_dlpack(t)`` or to any DLPack consumer via ``t.__dlpack__()``
+        (gradsolve/warp/warp_ode.py:164-212)."""
+        S = self.max_steps_taken()
+        N.call("gsx_tape_zero_tail", self.n, _ptr(self.tape), self.cap, _ptr(self.n_acc), S,
+               _stream_handle(None))
+        return self.tape[:S]
 
     def padded_tape(self, S: Optional[int] = None) -> torch.Tensor:
         """The reference's zero-padded trajectory-major tape ``[n, S]``, S = max n_acc."""
@@ -185,9 +196,42 @@ def adjoint(rec: Tsit5Record, ybar_soa: torch.Tensor,
     return y0bar, pbar
 
 
+def record_from_trimmed(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor, tape_sn,
+                        *, t0: float, t1: float, ckpt_interval: int = 8,
+                        aux: Optional[torch.Tensor] = None, strict: bool = False,
+                        entry: str = "gsx_tsit5_replay") -> Tsit5Record:
+    """A record built from a trimmed ``[S, n]`` step-major tape made elsewhere (the
+    reference's fused recorder hands one over DLPack; any DLPack producer or array works):
+    n_acc from its nonzero rows, checkpoints and the final state from one replay, ready for
+    the adjoint."""
+    if not isinstance(tape_sn, torch.Tensor):
+        tape_sn = torch.utils.dlpack.from_dlpack(tape_sn) if hasattr(tape_sn, "__dlpack__") \
+            else torch.as_tensor(tape_sn)
+    D, n = y0_soa.shape
+    P = p_soa.shape[0]
+    dev = y0_soa.device
+    S = int(tape_sn.shape[0])
+    if tape_sn.dim() != 2 or int(tape_sn.shape[1]) != n:
+        raise ValueError(f"tape must be [S, n={n}]; got {tuple(tape_sn.shape)}")
+    cap = max(S, 1)
+    tape = torch.zeros((cap, n), dtype=torch.float64, device=dev)
+    tape[:S] = tape_sn.to(device=dev, dtype=torch.float64)
+    n_acc = torch.empty(n, dtype=torch.int32, device=dev)
+    N.call("gsx_tape_count", n, _ptr(tape), S, _ptr(n_acc), _stream_handle(None))
+    C = max(1, min(MAX_CKPT_INTERVAL, int(ckpt_interval)))
+    y_final = torch.empty((D, n), dtype=torch.float64, device=dev)
+    ckpt = _alloc_ckpt(field, D, n, cap, C, dev)
+    N.call(entry, field, n, D, P, _ptr(aux), _ptr(y0_soa), _ptr(p_soa), float(t0), _ptr(tape),
+           _ptr(n_acc), _ptr(y_final), _ptr(ckpt), C, flags_for(strict),
+           _stream_handle(None))
+    zeros = torch.zeros(n, dtype=torch.int32, device=dev)
+    return Tsit5Record(field, n, D, P, float(t0), float(t1), y0_soa, p_soa, None, aux,
+                       y_final, n_acc, zeros.clone(), zeros, tape, cap, ckpt, C)
+
+
 def record_from_padded(field: int, y0_soa: torch.Tensor, p_soa: torch.Tensor,
                        padded: torch.Tensor, *, t0: float, t1: float,
-                       aux: Optional[torch.Tensor] = None) -> torch.Tensor:
+                       aux: Optional[torch.Tensor] = None):
     """Import a reference-format tape ``[n, S]`` into the step-major layout; returns
     ``(tape [cap, n], n_acc [n])`` with cap = S."""
     n, S = padded.shape

@@ -330,6 +330,17 @@ int gsx_tape_to_padded(int64_t n, const double* tape, int64_t tape_cap,
 int gsx_tape_from_padded(int64_t n, const double* in, int64_t S, double* tape,
                          int64_t tape_cap, int32_t* n_acc, void* stream);
 
+/* The reference's trimmed step-major hand-off (gradsolve/warp/warp_ode.py:164-212 hands
+ * dt_rec[:S] to JAX over DLPack, relying on rows past n_acc being 0.0): zeroes rows
+ * n_acc[j] <= s < S of each column in place, so that the leading S rows of the tape (a
+ * contiguous [S][n] block, exportable without a copy) are that record. */
+int gsx_tape_zero_tail(int64_t n, double* tape, int64_t tape_cap, const int32_t* n_acc,
+                       int64_t S, void* stream);
+
+/* The inverse for an imported trimmed [S][n] tape: n_acc[j] = 1 + the last row with a nonzero
+ * step (accepted steps are positive; zero rows are padding). */
+int gsx_tape_count(int64_t n, const double* tape, int64_t S, int32_t* n_acc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -207,8 +207,8 @@ def test_generated_stiff_jacobian_and_its_vjp():
 
 
 @pytest.mark.parametrize("body,name", [
-    (lambda t, y, p: y if y[0] > 0.0 else -y, "gt"),
-    (lambda t, y, p: np.array([max(y[0], y[1]), y[1]]), "gt"),
+    (lambda t, y, p: y if y[0] > 0.0 else -y, "cond"),
+    (lambda t, y, p: np.array([max(y[0], y[1]), y[1]]), "cond"),
     (lambda t, y, p: np.array([__import__("math").sin(y[0]), y[1]]), "float"),
 ])
 def test_untraceable_right_hand_sides_raise(body, name):
@@ -248,3 +248,42 @@ def test_library_registers_generated_field_and_reports_compile_errors():
     assert N.call("gsx_field_release", fid) == 0
     with pytest.raises(ValueError):
         N.call("gsx_ckpt_rows", fid, 3)
+
+
+def piecewise(t, y, p):
+    return np.array([y[1], -y[0] - p[0] * FG.maximum(y[1], 0.0)
+                     + FG.where(y[0] > 0.5, 0.1, -0.1) * (y[0] > -1.0)
+                     + 0.01 * FG.minimum(y[0] * y[1], p[1]) * FG.sign(y[0])])
+
+
+def test_piecewise_subset_field_and_vjp():
+    """Comparisons, where, maximum, minimum and sign (the reference translator's lt/gt,
+    select, max, min, sign): f bitwise equal to the same Python function on floats, the VJP
+    against central differences at points away from the kinks (rel 1e-7)."""
+    hf = HostField(piecewise, 2, 2)
+    rng = np.random.default_rng(5)
+    checked = 0
+    while checked < 40:
+        y = rng.uniform(-2.0, 2.0, 2)
+        p = rng.uniform(0.5, 2.0, 2)
+        if min(abs(y[1]), abs(y[0] - 0.5), abs(y[0] + 1.0), abs(y[0] * y[1] - p[1]),
+               abs(y[0])) < 1e-3:
+            continue
+        checked += 1
+        want = np.asarray(piecewise(0.0, y, p), np.float64)
+        assert np.array_equal(hf.f(0.0, y, p), want)
+        v = rng.standard_normal(2)
+        yb, pb = hf.vjp(0.0, y, p, v)
+        fd_y, fd_p = np.zeros(2), np.zeros(2)
+        for k in range(2):
+            e = np.zeros(2)
+            e[k] = 1e-7
+            fd_y[k] = v @ (piecewise(0.0, y + e, p) - piecewise(0.0, y - e, p)) / 2e-7
+            fd_p[k] = v @ (piecewise(0.0, y, p + e) - piecewise(0.0, y, p - e)) / 2e-7
+        np.testing.assert_allclose(yb, fd_y, rtol=1e-7, atol=1e-7)
+        np.testing.assert_allclose(pb, fd_p, rtol=1e-7, atol=1e-7)
+
+
+def test_python_max_on_traced_values_names_the_alternative():
+    with pytest.raises(UnsupportedRHS, match="maximum"):
+        FG.trace(lambda t, y, p: np.array([max(y[0], y[1]), y[1]]), 2, 1)
