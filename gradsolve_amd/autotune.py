@@ -30,7 +30,7 @@ class Choice:
 
 def tune(field, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, t1_per=None, fp32=False,
          sample: int = 1 << 18, waves: Sequence[int] = (2, 3, 4),
-         intervals: Sequence[int] = (4, 8, 16), aux: Optional[torch.Tensor] = None,
+         intervals: Sequence[int] = (2, 4, 8, 16), aux: Optional[torch.Tensor] = None,
          seed: int = 0) -> Choice:
     D, n = y0_soa.shape
     P = p_soa.shape[0]
@@ -52,6 +52,7 @@ def tune(field, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, t1_per=None, fp
             eng = PS.PersistentTsit5(field, D, P, m, est=est, waves_per_simd=w,
                                      ckpt_interval=c, fp32=fp32, device=y0_soa.device,
                                      aux=aux)
+            eng.plan_longest_first(t1s, t0=t0)
             kw = dict(t0=t0, t1=t1, rtol=rtol, atol=atol, max_steps=max_steps, t1_per=t1s)
             eng.run_forward(ys, ps, **kw)
             torch.mul(eng.y_final, 2.0, out=eng.ybar)

@@ -121,6 +121,7 @@ struct PersistFwdArgs {
   int32_t* tape_start; // [n]
   int32_t* ckpt_start; // [n]
   unsigned long long* queue;  // one counter, zero at launch
+  const int32_t* queue_order = nullptr;  // [n] order the queue hands trajectories out, or null
 };
 
 struct PersistAdjArgs {

@@ -379,8 +379,8 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
                           int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
                           int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
                           int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
-                          int32_t* tape_start, int32_t* ckpt_start, void* queue, int flags,
-                          void* stream) {
+                          int32_t* tape_start, int32_t* ckpt_start, void* queue,
+                          const int32_t* queue_order, int flags, void* stream) {
   FieldInfo fi;
   int rc = check_field(field, D, P, &fi);
   if (rc) return rc;
@@ -418,6 +418,7 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
   pa.tape_start = tape_start;
   pa.ckpt_start = ckpt_start;
   pa.queue = static_cast<unsigned long long*>(queue);
+  pa.queue_order = queue_order;
   return persist_fwd_dispatch(field, D, (flags & GSX_FP32) != 0, pa,
                               static_cast<hipStream_t>(stream));
 }

@@ -18,8 +18,11 @@ constexpr int kPAdjBlock = 64;
 
 __global__ void queue_reset_kernel(unsigned long long* q) { *q = 0ull; }
 
+// Held to 128 VGPRs (four waves per SIMD): the Lorenz FP64 lane otherwise takes 129 and would
+// cap the persistent grid at three waves.
 template <class F, class State>
-__global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa) {
+__global__ __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void persist_fwd_kernel(PersistFwdArgs pa) {
   constexpr int DCK = ckpt_rows<F>();
   const FwdArgs& a = pa.f;
   const int64_t L = pa.lanes;
@@ -74,6 +77,12 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
           jj = nb + (r - avail);
         } else {
           got = false;
+        }
+        if (got && pa.queue_order) {
+          // A planned order (e.g. longest first); an entry outside [0, n) hands out no work.
+          const int32_t jo = pa.queue_order[jj];
+          got = jo >= 0 && static_cast<unsigned long long>(jo) < n;
+          jj = static_cast<unsigned long long>(jo);
         }
         if (got) {
           j = static_cast<int64_t>(jj);
@@ -158,7 +167,8 @@ __global__ __launch_bounds__(kPBlock) void persist_fwd_kernel(PersistFwdArgs pa)
 
 // One lane walks its trajectories backwards, one checkpoint segment per iteration.
 template <class F, int CMAX, bool FP32>
-__global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs pa) {
+__global__ __launch_bounds__(kPAdjBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void persist_adj_kernel(PersistAdjArgs pa) {
   constexpr int D = F::D, P = F::P, DCK = ckpt_rows<F>();
   __shared__ double seg_all[CMAX * DCK * kPAdjBlock];
   const AdjArgs& a = pa.a;
@@ -230,6 +240,26 @@ __global__ __launch_bounds__(kPAdjBlock) void persist_adj_kernel(PersistAdjArgs 
               static_cast<float>(a.tape[persist_tape_index(tstart + s, lane, L)]);
           tsit5_step_vjp_f32<F>(y, hf, p_f, lam_f, lam_lo, pb_f, pb_lo, a.aux);
         }
+      } else if constexpr (CMAX == 2) {
+        // Checkpoint interval 2: one advance from the checkpoint forms step s0's stage inputs
+        // and y_{s0+1}; step s0+1's own stage inputs serve its VJP, step s0's reuse the kept
+        // ones, so every step's stages are formed once and nothing goes through LDS (the
+        // pair adjoint of tsit5_adj_pair_segment, on the persistent layout; a missing second
+        // step runs with h = 0, which leaves lam and pb unchanged).
+        double y[D];
+        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d];
+        const double h0 = a.tape[persist_tape_index(tstart + s0, lane, L)];
+        const double h1 =
+            (s0 + 1 < na) ? a.tape[persist_tape_index(tstart + s0 + 1, lane, L)] : 0.0;
+        double z0[6][D], y1[D];
+        tsit5_stage_inputs<F>(0.0, y, h0, p_d, z0, y1, a.aux);
+        {
+          double z1[6][D];
+          tsit5_stage_inputs<F>(0.0, y1, h1, p_d, z1, nullptr, a.aux);
+          tsit5_step_vjp_z<F>(0.0, z1, h1, p_d, lam_d, pb_d, a.aux);
+        }
+        tsit5_step_vjp_z<F>(0.0, z0, h0, p_d, lam_d, pb_d, a.aux);
+        (void)s1;
       } else {
         double y[D];
         GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d];
@@ -283,6 +313,7 @@ template <class F, bool FP32>
 int launch_padj(const PersistAdjArgs& pa, hipStream_t s) {
   const int C = pa.a.ckpt_interval;
   if (C <= 1) return launch_padj_c<F, 1, FP32>(pa, s);
+  if (C == 2 && !FP32) return launch_padj_c<F, 2, FP32>(pa, s);
   if (C <= 4) return launch_padj_c<F, 4, FP32>(pa, s);
   if (C <= 8) return launch_padj_c<F, 8, FP32>(pa, s);
   if (C <= 16) return launch_padj_c<F, 16, FP32>(pa, s);

@@ -226,7 +226,11 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux,
  * stream positions are lane_of[j], tape_start[j], ckpt_start[j]. `queue` is 8 bytes of device
  * memory the call resets (with a kernel, so the sequence can be captured in a graph).
  * A trajectory that does not fit its lane's streams gets status 3; the caller grows the
- * capacities and calls again. flags: 0 or GSX_FP32. t1_per may be NULL. */
+ * capacities and calls again. flags: 0 or GSX_FP32. t1_per may be NULL. queue_order (NULL:
+ * index order) is the order in which the queue hands trajectories out, a permutation of
+ * 0..n-1; handing out the longest first (e.g. by t1_per descending) keeps the lanes' last
+ * trajectories short, so fewer lanes idle while the longest ones finish. An entry outside
+ * [0, n) retires the lane that draws it. */
 int64_t gsx_persist_lanes(int waves_per_simd);
 int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
                           const double* y0, const double* p, double t0, double t1,
@@ -235,8 +239,8 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
                           int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
                           int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
                           int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
-                          int32_t* tape_start, int32_t* ckpt_start, void* queue, int flags,
-                          void* stream);
+                          int32_t* tape_start, int32_t* ckpt_start, void* queue,
+                          const int32_t* queue_order, int flags, void* stream);
 
 /* Adjoint of a persistent record: lane l walks its trajectories (order) backwards, one
  * checkpoint segment per iteration. Same lanes, streams and ckpt_interval as the forward;
