@@ -84,6 +84,11 @@ SIGNATURES = {
     "gsx_tsit5_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                       _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _i, _vp]),
+    "gsx_rb23_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i64,
+                             _vp, _vp, _vp, _vp, _i, _vp]),
+    "gsx_rb23_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
+                                     _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _i, _vp]),
     "gsx_field_register": (_i, [ctypes.c_char_p, _i, _i, _i, ctypes.POINTER(_i)]),
     "gsx_field_release": (_i, [_i]),
 }

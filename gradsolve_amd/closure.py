@@ -31,6 +31,7 @@ class _Lane:
     """Native calls of one record-and-replay lane (Tsit5 or Rosenbrock23)."""
 
     FWD, REPLAY, ADJ = "gsx_tsit5_fwd", "gsx_tsit5_replay", "gsx_tsit5_adjoint"
+    METHOD = "tsit5"  # the dense-output entry points' method (hip_tsit5.saveat)
 
     def __init__(self, binding, aux, strict, fp32=False):
         self.b, self.aux, self.strict, self.fp32 = binding, aux, strict, fp32
@@ -66,6 +67,7 @@ class _Tsit5Lane(_Lane):
 
 class _Rb23Lane(_Lane):
     FWD, REPLAY, ADJ = "gsx_rb23_fwd", "gsx_rb23_replay", "gsx_rb23_adjoint"
+    METHOD = "rb23"
 
 
 class _ReplayFn(torch.autograd.Function):
@@ -210,9 +212,10 @@ def make_rb23_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
 
 
 class _SaveatFn(torch.autograd.Function):
-    """The saveat series of a Tsit5 record-once closure: forward replays the frozen mesh from
-    the given inputs and re-steps to the save times; backward runs the re-steps' VJPs and the
-    adjoint sweep with the save cotangents injected at their bracketing steps."""
+    """The saveat series of a record-once closure (Tsit5 or Rosenbrock23 lane): forward replays
+    the frozen mesh from the given inputs and re-steps to the save times; backward runs the
+    re-steps' VJPs and the adjoint sweep with the save cotangents injected at their bracketing
+    steps."""
 
     @staticmethod
     def forward(ctx, z, p_user, state):
@@ -224,7 +227,7 @@ class _SaveatFn(torch.autograd.Function):
         else:
             y_final, ckpt = state.lane.replay(rec, y0_soa, p_soa)
         sv = H.saveat(rec, state.ts, y0_soa=y0_soa, p_soa=p_soa, y_final=y_final,
-                      strict=state.lane.strict)
+                      strict=state.lane.strict, method=state.lane.METHOD)
         ctx.state, ctx.ckpt, ctx.p_soa, ctx.sv = state, ckpt, p_soa, sv
         ctx.p_cols = p_user.shape[1]
         return sv.ys.permute(2, 0, 1).contiguous()  # (n, k, D)
@@ -234,7 +237,8 @@ class _SaveatFn(torch.autograd.Function):
         st = ctx.state
         ysbar = g.to(torch.float64).permute(1, 2, 0).contiguous()  # [k, D, n]
         y0bar, pbar = H.adjoint_saveat(st.rec, st.ts, ctx.sv, ysbar, ckpt=ctx.ckpt,
-                                       p_soa=ctx.p_soa, strict=st.lane.strict)
+                                       p_soa=ctx.p_soa, strict=st.lane.strict,
+                                       method=st.lane.METHOD)
         g_p = None
         if ctx.needs_input_grad[1]:
             cols = st.binding.param_cols
@@ -250,7 +254,9 @@ class _SaveatFn(torch.autograd.Function):
 
 class Tsit5SaveatClosure(ReplayClosure):
     """Record-once closure returning the series at the save times ``(n, k, D)``
-    (gradsolve/api.py:750-753 with gradsolve/solvers/dense.py:318-335)."""
+    (gradsolve/api.py:750-753 with gradsolve/solvers/dense.py:318-335); with a Rosenbrock23
+    lane it is the stiff series of make_rosenbrock_replay_kernel(saveat=...)
+    (gradsolve/warp/warp_replay.py:364-382)."""
 
     def __init__(self, lane, binding, problem, y0, params, *, wrt, rtol, atol, device, ts,
                  ckpt_interval=None, max_steps=100000):
@@ -276,6 +282,15 @@ def make_tsit5_saveat_closure(binding, problem, y0, params, *, wrt, rtol, atol, 
                               ckpt_interval=None, strict=False, aux=None,
                               max_steps=100000) -> Tsit5SaveatClosure:
     lane = _Tsit5Lane(binding, aux, strict)
+    return Tsit5SaveatClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol,
+                              atol=atol, device=device, ts=ts, ckpt_interval=ckpt_interval,
+                              max_steps=max_steps)
+
+
+def make_rb23_saveat_closure(binding, problem, y0, params, *, wrt, rtol, atol, device, ts,
+                             ckpt_interval=None, strict=False, aux=None,
+                             max_steps=50000) -> Tsit5SaveatClosure:
+    lane = _Rb23Lane(binding, aux, strict)
     return Tsit5SaveatClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol,
                               atol=atol, device=device, ts=ts, ckpt_interval=ckpt_interval,
                               max_steps=max_steps)

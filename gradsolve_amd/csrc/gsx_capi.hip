@@ -223,26 +223,108 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux, con
   return tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
 }
 
+namespace {
+
+// Dense output and its gradient for one of the two one-step methods; the Tsit5 and
+// Rosenbrock23 entry points below differ only in the method and the fields they accept.
+enum class DenseMethod { kTsit5, kRb23 };
+
+int saveat_common(DenseMethod m, const char* what, int field, int64_t n, int D, int P,
+                  const double* aux, const double* y0, const double* p, double t0,
+                  const double* tape, const int32_t* n_acc, const double* y_final,
+                  const double* ts, int64_t nsave, double* ys, int32_t* save_step,
+                  double* save_t, double* states, int flags, void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (m == DenseMethod::kTsit5 && fi.stiff)
+    return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
+  if (flags & ~GSX_STRICT)
+    return set_error(flags & GSX_FP32 ? GSX_E_UNSUPPORTED : GSX_E_INVALID,
+                     "%s: flags 0x%x (0 or GSX_STRICT; no FP32 dense output)", what, flags);
+  if (n == 0 || nsave == 0) return GSX_OK;
+  if (!y0 || !p || !tape || !n_acc || !ts || !ys || !save_step || !save_t)
+    return set_error(GSX_E_INVALID, "%s: null pointer", what);
+  SaveArgs a{n, y0, p, aux, t0, tape, n_acc, ts, nsave, ys, save_step, save_t, states, y_final};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool strict = (flags & GSX_STRICT) != 0;
+  if (m == DenseMethod::kRb23)
+    return strict ? gsx_strict::rb23_saveat_dispatch(field, D, a, st)
+                  : rb23_saveat_dispatch(field, D, a, st);
+  return strict ? gsx_strict::tsit5_saveat_dispatch(field, D, a, st)
+                : tsit5_saveat_dispatch(field, D, a, st);
+}
+
+int adjoint_saveat_common(DenseMethod m, const char* what, int field, int64_t n, int D, int P,
+                          const double* aux, const double* y0, const double* p, double t0,
+                          const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                          const double* ckpt, int32_t ckpt_interval, const double* ts,
+                          int64_t nsave, const int32_t* save_step, const double* save_t,
+                          const double* states, const double* ysbar, const double* ybar,
+                          double* mu, double* y0bar, double* pbar, int flags, void* stream) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (m == DenseMethod::kTsit5 && fi.stiff)
+    return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
+  if (flags & ~GSX_STRICT)
+    return set_error(flags & GSX_FP32 ? GSX_E_UNSUPPORTED : GSX_E_INVALID,
+                     "%s: flags 0x%x (0 or GSX_STRICT)", what, flags);
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (n == 0) return GSX_OK;
+  if (!p || !tape || !n_acc || !ckpt || !y0bar || !pbar ||
+      (nsave > 0 && (!ts || !save_step || !save_t || !states || !ysbar || !mu)))
+    return set_error(GSX_E_INVALID, "%s: null pointer", what);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool strict = (flags & GSX_STRICT) != 0;
+  const bool rb = m == DenseMethod::kRb23;
+  // Phase 1: the re-steps' VJPs -> mu, the final-state cotangent (into y0bar) and the
+  // re-steps' parameter gradient (into pbar). Phase 2 reads both back in the same lane before
+  // writing them, so the buffers are shared.
+  const double* ybar_in = ybar;
+  if (nsave > 0) {
+    SaveVjpArgs v{n, p, aux, n_acc, ts, nsave, save_step, save_t, states, ysbar, ybar,
+                  mu, y0bar, pbar};
+    if (rb)
+      rc = strict ? gsx_strict::rb23_saveat_vjp_dispatch(field, D, v, st)
+                  : rb23_saveat_vjp_dispatch(field, D, v, st);
+    else
+      rc = strict ? gsx_strict::tsit5_saveat_vjp_dispatch(field, D, v, st)
+                  : tsit5_saveat_vjp_dispatch(field, D, v, st);
+    if (rc) return rc;
+    ybar_in = y0bar;
+  } else if (!ybar) {
+    return set_error(GSX_E_INVALID, "%s: no cotangent (ybar and nsave 0)", what);
+  }
+  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar_in, y0bar,
+            pbar};
+  if (nsave > 0) {
+    a.nsave = nsave;
+    a.save_step = save_step;
+    a.mu = mu;
+    a.pbar_in = pbar;
+  }
+  if (rb)
+    return strict ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
+                  : rb23_adj_dispatch(field, D, a, st);
+  return strict ? gsx_strict::tsit5_adj_dispatch(field, D, a, st)
+                : tsit5_adj_dispatch(field, D, a, st);
+}
+
+}  // namespace
+
 int gsx_tsit5_saveat(int field, int64_t n, int D, int P, const double* aux, const double* y0,
                      const double* p, double t0, const double* tape, const int32_t* n_acc,
                      const double* y_final, const double* ts, int64_t nsave, double* ys,
                      int32_t* save_step, double* save_t, double* states, int flags,
                      void* stream) {
-  FieldInfo fi;
-  int rc = check_field(field, D, P, &fi);
-  if (rc) return rc;
-  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
-  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
-  if (flags & ~GSX_STRICT)
-    return set_error(flags & GSX_FP32 ? GSX_E_UNSUPPORTED : GSX_E_INVALID,
-                     "tsit5_saveat: flags 0x%x (0 or GSX_STRICT; no FP32 dense output)", flags);
-  if (n == 0 || nsave == 0) return GSX_OK;
-  if (!y0 || !p || !tape || !n_acc || !ts || !ys || !save_step || !save_t)
-    return set_error(GSX_E_INVALID, "tsit5_saveat: null pointer");
-  SaveArgs a{n, y0, p, aux, t0, tape, n_acc, ts, nsave, ys, save_step, save_t, states, y_final};
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  return (flags & GSX_STRICT) ? gsx_strict::tsit5_saveat_dispatch(field, D, a, st)
-                              : tsit5_saveat_dispatch(field, D, a, st);
+  return saveat_common(DenseMethod::kTsit5, "tsit5_saveat", field, n, D, P, aux, y0, p, t0,
+                       tape, n_acc, y_final, ts, nsave, ys, save_step, save_t, states, flags,
+                       stream);
 }
 
 int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
@@ -253,47 +335,33 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                              const double* states, const double* ysbar, const double* ybar,
                              double* mu, double* y0bar, double* pbar, int flags,
                              void* stream) {
-  FieldInfo fi;
-  int rc = check_field(field, D, P, &fi);
-  if (rc) return rc;
-  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
-  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
-  if (flags & ~GSX_STRICT)
-    return set_error(flags & GSX_FP32 ? GSX_E_UNSUPPORTED : GSX_E_INVALID,
-                     "tsit5_adjoint_saveat: flags 0x%x (0 or GSX_STRICT)", flags);
-  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
-    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
-                     kMaxCkptInterval);
-  if (n == 0) return GSX_OK;
-  if (!p || !tape || !n_acc || !ckpt || !y0bar || !pbar ||
-      (nsave > 0 && (!ts || !save_step || !save_t || !states || !ysbar || !mu)))
-    return set_error(GSX_E_INVALID, "tsit5_adjoint_saveat: null pointer");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool strict = (flags & GSX_STRICT) != 0;
-  // Phase 1: the re-steps' VJPs -> mu, the final-state cotangent (into y0bar) and the
-  // re-steps' parameter gradient (into pbar). Phase 2 reads both back in the same lane before
-  // writing them, so the buffers are shared.
-  const double* ybar_in = ybar;
-  if (nsave > 0) {
-    SaveVjpArgs v{n, p, aux, n_acc, ts, nsave, save_step, save_t, states, ysbar, ybar,
-                  mu, y0bar, pbar};
-    rc = strict ? gsx_strict::tsit5_saveat_vjp_dispatch(field, D, v, st)
-                : tsit5_saveat_vjp_dispatch(field, D, v, st);
-    if (rc) return rc;
-    ybar_in = y0bar;
-  } else if (!ybar) {
-    return set_error(GSX_E_INVALID, "tsit5_adjoint_saveat: no cotangent (ybar and nsave 0)");
-  }
-  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar_in, y0bar,
-            pbar};
-  if (nsave > 0) {
-    a.nsave = nsave;
-    a.save_step = save_step;
-    a.mu = mu;
-    a.pbar_in = pbar;
-  }
-  return strict ? gsx_strict::tsit5_adj_dispatch(field, D, a, st)
-                : tsit5_adj_dispatch(field, D, a, st);
+  return adjoint_saveat_common(DenseMethod::kTsit5, "tsit5_adjoint_saveat", field, n, D, P,
+                               aux, y0, p, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ts,
+                               nsave, save_step, save_t, states, ysbar, ybar, mu, y0bar, pbar,
+                               flags, stream);
+}
+
+int gsx_rb23_saveat(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                    const double* p, double t0, const double* tape, const int32_t* n_acc,
+                    const double* y_final, const double* ts, int64_t nsave, double* ys,
+                    int32_t* save_step, double* save_t, double* states, int flags,
+                    void* stream) {
+  return saveat_common(DenseMethod::kRb23, "rb23_saveat", field, n, D, P, aux, y0, p, t0,
+                       tape, n_acc, y_final, ts, nsave, ys, save_step, save_t, states, flags,
+                       stream);
+}
+
+int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                            const double* y0, const double* p, double t0, const double* tape,
+                            int64_t tape_cap, const int32_t* n_acc, const double* ckpt,
+                            int32_t ckpt_interval, const double* ts, int64_t nsave,
+                            const int32_t* save_step, const double* save_t,
+                            const double* states, const double* ysbar, const double* ybar,
+                            double* mu, double* y0bar, double* pbar, int flags, void* stream) {
+  return adjoint_saveat_common(DenseMethod::kRb23, "rb23_adjoint_saveat", field, n, D, P, aux,
+                               y0, p, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ts, nsave,
+                               save_step, save_t, states, ysbar, ybar, mu, y0bar, pbar, flags,
+                               stream);
 }
 
 int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,

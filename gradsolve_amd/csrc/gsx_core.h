@@ -801,13 +801,28 @@ GSX_HD void tsit5_replay_lane(const ReplayArgs& a, int64_t j) {
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];
 }
 
+// The one-step method a dense-output lane replays and re-steps with: M::advance<F> is the
+// value-only step (y <- y + step), M::step_vjp<F> its VJP. Tsit5 here; Rosenbrock23's policy
+// sits beside its step in gsx_rb23.h.
+struct MethodTsit5 {
+  template <class F>
+  GSX_HD static void advance(double t, double* y, double h, const double* p, const double* aux) {
+    tsit5_advance<F>(t, y, h, p, aux);
+  }
+  template <class F>
+  GSX_HD static void step_vjp(double t, const double* y, double h, const double* p,
+                              double* lam, double* pb, const double* aux) {
+    tsit5_step_vjp<F>(t, y, h, p, lam, pb, aux);
+  }
+};
+
 // Dense output for one trajectory (gradsolve/solvers/dense.py:106-163, restated as the C
 // oracle's replay_saveat): replays the recorded mesh; a save time in (t_s, t_s + h_s] is the
 // state one step of size ts[q] - t_s from (t_s, y_s), except that at or after the final time
 // it is y_final itself, and at or before t0 it is y0. save_step / save_t / states record the
-// bracket for the gradient (tsit5_saveat_vjp_lane).
-template <class F>
-GSX_HD void tsit5_saveat_lane(const SaveArgs& a, int64_t j) {
+// bracket for the gradient (saveat_vjp_lane).
+template <class F, class M>
+GSX_HD void saveat_lane(const SaveArgs& a, int64_t j) {
   constexpr int D = F::D, P = F::P;
   const int64_t n = a.n, k = a.nsave;
   double y[D], p[P];
@@ -835,11 +850,11 @@ GSX_HD void tsit5_saveat_lane(const SaveArgs& a, int64_t j) {
       if (s == na - 1 && a.ts[q] >= tn) break;  // at the final time: y_final, below
       double yq[D];
       GSX_UNROLL for (int d = 0; d < D; ++d) yq[d] = y[d];
-      tsit5_advance<F>(t, yq, a.ts[q] - t, p, a.aux);
+      M::template advance<F>(t, yq, a.ts[q] - t, p, a.aux);
       put(q, yq, s, t, y);
       ++q;
     }
-    tsit5_advance<F>(t, y, h, p, a.aux);
+    M::template advance<F>(t, y, h, p, a.aux);
     t = tn;
   }
   if (a.y_final) {
@@ -852,8 +867,8 @@ GSX_HD void tsit5_saveat_lane(const SaveArgs& a, int64_t j) {
 // ysbar[q] for a bracketed save (the VJP of its re-step, which also adds to the parameter
 // gradient), ysbar[q] itself for a save at y0, and the final-time saves folded into the
 // final-state cotangent. The adjoint sweep (AdjArgs with nsave) then adds mu at each bracket.
-template <class F>
-GSX_HD void tsit5_saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
+template <class F, class M>
+GSX_HD void saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
   constexpr int D = F::D, P = F::P;
   const int64_t n = a.n, k = a.nsave;
   double p[P], pb[P], lf[D];
@@ -876,12 +891,22 @@ GSX_HD void tsit5_saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
       double y[D];
       GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = a.states[(q * D + d) * n + j];
       const double ts0 = a.save_t[q * n + j];
-      tsit5_step_vjp<F>(ts0, y, a.ts[q] - ts0, p, l, pb, a.aux);
+      M::template step_vjp<F>(ts0, y, a.ts[q] - ts0, p, l, pb, a.aux);
     }
     GSX_UNROLL for (int d = 0; d < D; ++d) a.mu[(q * D + d) * n + j] = l[d];
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) a.ybar_eff[d * n + j] = lf[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q] + 0.0;
+}
+
+template <class F>
+GSX_HD void tsit5_saveat_lane(const SaveArgs& a, int64_t j) {
+  saveat_lane<F, MethodTsit5>(a, j);
+}
+
+template <class F>
+GSX_HD void tsit5_saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
+  saveat_vjp_lane<F, MethodTsit5>(a, j);
 }
 
 // Adds the save cotangents mu[q] of the saves bracketed by step s (a cotangent on y_s) to lam,

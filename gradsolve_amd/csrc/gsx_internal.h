@@ -32,6 +32,8 @@ int tsit5_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int rb23_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s);
+int rb23_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s);
 int tsit5_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
@@ -57,6 +59,8 @@ int tsit5_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s)
 int rb23_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
 int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
 int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
+int rb23_saveat(int field, Family fam, const SaveArgs& a, hipStream_t s);
+int rb23_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s);
 }  // namespace rt
 
 }  // namespace gsx
@@ -70,6 +74,8 @@ int tsit5_saveat_vjp_dispatch(int field, int D, const ::gsx::SaveVjpArgs& a, hip
 int rb23_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int rb23_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+int rb23_saveat_dispatch(int field, int D, const ::gsx::SaveArgs& a, hipStream_t s);
+int rb23_saveat_vjp_dispatch(int field, int D, const ::gsx::SaveVjpArgs& a, hipStream_t s);
 int tsit5_fwd_f32_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int tsit5_replay_f32_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int tsit5_adj_f32_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);

@@ -202,6 +202,31 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
   GSX_UNROLL for (int i = 0; i < D; ++i) lam[i] = yb[i];
 }
 
+// Rosenbrock23 as the step policy of the dense-output lanes (saveat_lane / saveat_vjp_lane):
+// the re-step to a save time is rosenbrock23_step of size ts - t_s, as the reference's
+// replay_solve_rosenbrock_saveat_jax takes it (gradsolve/warp/warp_replay.py:78-84).
+struct MethodRb23 {
+  template <class F>
+  GSX_HD static void advance(double t, double* y, double h, const double* p, const double* aux) {
+    rb23_advance<F>(t, y, h, p, aux);
+  }
+  template <class F>
+  GSX_HD static void step_vjp(double t, const double* y, double h, const double* p,
+                              double* lam, double* pb, const double* aux) {
+    rb23_step_vjp<F>(t, y, h, p, lam, pb, aux);
+  }
+};
+
+template <class F>
+GSX_HD void rb23_saveat_lane(const SaveArgs& a, int64_t j) {
+  saveat_lane<F, MethodRb23>(a, j);
+}
+
+template <class F>
+GSX_HD void rb23_saveat_vjp_lane(const SaveVjpArgs& a, int64_t j) {
+  saveat_vjp_lane<F, MethodRb23>(a, j);
+}
+
 // Forward + record (same tape/checkpoint contract as the Tsit5 lane).
 template <class F>
 GSX_HD void rb23_fwd_lane(const FwdArgs& a, int64_t j) {
@@ -291,9 +316,10 @@ GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstr
   double p[P], pb[P], lam[D];
   GSX_UNROLL for (int q = 0; q < P; ++q) {
     p[q] = a.p[q * n + j];
-    pb[q] = 0.0;
+    pb[q] = a.pbar_in ? a.pbar_in[q * n + j] : 0.0;
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  int32_t qs = static_cast<int32_t>(a.nsave) - 1;  // save cursor, walked downwards
   const int32_t na = a.n_acc[j];
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
@@ -321,8 +347,10 @@ GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstr
       if constexpr (!F::autonomous) t = slot[D * sstride];
       const double h = a.tape[static_cast<int64_t>(s) * n + j];
       rb23_step_vjp<F>(t, y, h, p, lam, pb, a.aux);
+      inject_saves<D>(a, j, s, qs, lam);
     }
   }
+  inject_saves<D>(a, j, -1, qs, lam);
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }

@@ -50,6 +50,20 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
                    kMaxCkptInterval);
 }
 
+template <class F>
+int launch_saveat(const SaveArgs& a, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_saveat_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_saveat_kernel");
+}
+
+template <class F>
+int launch_saveat_vjp(const SaveVjpArgs& a, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_saveat_vjp_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_saveat_vjp_kernel");
+}
+
 #if defined(GSX_STRICT_ARITH)
 constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23Strict;
 #else
@@ -103,6 +117,41 @@ int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
       break;
   }
   return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 adjoint: field %d with D=%d has no "
+                   "analytic Jacobian in this build", field, D);
+}
+
+int rb23_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_saveat<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_saveat<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_saveat<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_saveat<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field)) return ::gsx::rt::rb23_saveat(field, kRtFamily, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 saveat: field %d with D=%d has no "
+                   "analytic Jacobian in this build", field, D);
+}
+
+int rb23_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_saveat_vjp<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_saveat_vjp<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_saveat_vjp<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_saveat_vjp<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field))
+        return ::gsx::rt::rb23_saveat_vjp(field, kRtFamily, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 saveat adjoint: field %d with D=%d has no "
                    "analytic Jacobian in this build", field, D);
 }
 

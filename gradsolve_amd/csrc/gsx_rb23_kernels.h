@@ -29,4 +29,19 @@ __global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
   rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
 }
 
+// Dense output on a recorded mesh and the VJPs of its re-steps, one trajectory per lane.
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_saveat_kernel(SaveArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_saveat_lane<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_saveat_vjp_kernel(SaveVjpArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_saveat_vjp_lane<F>(a, j);
+}
+
 }  // namespace GSX_NS

@@ -73,12 +73,14 @@ const FamilySpec& spec(Family f) {
        {{"fwd", "rb23_fwd_kernel<%s>"}, {"replay", "rb23_replay_kernel<%s>"},
         {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
         {"adj8", "rb23_adj_kernel<%s, 8>"}, {"adj16", "rb23_adj_kernel<%s, 16>"},
-        {"adj32", "rb23_adj_kernel<%s, 32>"}}},
+        {"adj32", "rb23_adj_kernel<%s, 32>"}, {"saveat", "rb23_saveat_kernel<%s>"},
+        {"saveat_vjp", "rb23_saveat_vjp_kernel<%s>"}}},
       {"gsx_strict", "gsx_rb23_kernels.h", kStrict,
        {{"fwd", "rb23_fwd_kernel<%s>"}, {"replay", "rb23_replay_kernel<%s>"},
         {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
         {"adj8", "rb23_adj_kernel<%s, 8>"}, {"adj16", "rb23_adj_kernel<%s, 16>"},
-        {"adj32", "rb23_adj_kernel<%s, 32>"}}},
+        {"adj32", "rb23_adj_kernel<%s, 32>"}, {"saveat", "rb23_saveat_kernel<%s>"},
+        {"saveat_vjp", "rb23_saveat_vjp_kernel<%s>"}}},
   };
   return specs[f];
 }
@@ -339,6 +341,14 @@ int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
     return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d",
                      a.ckpt_interval, kMaxCkptInterval);
   return launch(field, fam, key, a.n, kAdjBlock, a, s);
+}
+
+int rb23_saveat(int field, Family fam, const SaveArgs& a, hipStream_t s) {
+  return launch(field, fam, "saveat", a.n, kFwdBlock, a, s);
+}
+
+int rb23_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s) {
+  return launch(field, fam, "saveat_vjp", a.n, kFwdBlock, a, s);
 }
 
 }  // namespace rt

@@ -9,9 +9,12 @@ step of size ``ts[q] - t_s`` from that step's start state. Times at or before ``
 as the mesh replay, with the step sizes as data.
 
 This build serves saveat on the Kuramoto engine (the problem the reference's dense lane is
-benchmarked on) and on the Tsit5 lanes of every other non-stiff field, built-in or compiled at
-run time (``gsx_tsit5_saveat`` / ``gsx_tsit5_adjoint_saveat``); the stiff lane and FP32 dense
-output outside the Kuramoto engine raise ``ValueError`` naming what is built.
+benchmarked on), on the Tsit5 lanes of every other non-stiff field, built-in or compiled at
+run time (``gsx_tsit5_saveat`` / ``gsx_tsit5_adjoint_saveat``), and on the stiff Rosenbrock23
+lane, whose re-step is one ``rosenbrock23_step`` (``gsx_rb23_saveat`` /
+``gsx_rb23_adjoint_saveat``; the reference's ``replay_solve_rosenbrock_saveat_jax``,
+gradsolve/warp/warp_replay.py:78-84). FP32 dense output outside the Kuramoto engine raises
+``ValueError`` naming what is built.
 """
 from __future__ import annotations
 
@@ -33,21 +36,31 @@ def _check(name, b, precision) -> str:
                 f"saveat with precision='float32' is built for the Kuramoto engine only; field "
                 f"{b.key!r} serves dense output in float64")
         return "tsit5"
+    if name == D.HIP_ROSENBROCK23:
+        if precision == "float32":
+            raise ValueError(
+                f"saveat with precision='float32' is built for the Kuramoto engine only; the "
+                f"stiff lane serves field {b.key!r} in float64")
+        return "rb23"
     raise ValueError(
         f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
-        "dense output on the Tsit5 lanes (hip_tsit5 / hip_replay) and the Kuramoto engine "
-        "(hip_kuramoto). Drop saveat for a final-state solve or gradient.")
+        "dense output on the Tsit5 lanes (hip_tsit5 / hip_replay), the Rosenbrock23 lane "
+        "(hip_rosenbrock23) and the Kuramoto engine (hip_kuramoto). Drop saveat for a "
+        "final-state solve or gradient.")
 
 
-def _solve_saveat_tsit5(b, problem, y0, params, ts, *, rtol, atol, dev, max_steps):
+_FWD_ENTRY = {"tsit5": "gsx_tsit5_fwd", "rb23": "gsx_rb23_fwd"}
+
+
+def _solve_saveat_lane(lane, b, problem, y0, params, ts, *, rtol, atol, dev, max_steps):
     from . import hip_tsit5 as H
     y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
     p_t = torch.as_tensor(np.asarray(params, np.float64), device=dev)
     p_soa = b.native_params(p_t).t().contiguous()
     aux = None if b.aux_of is None else torch.as_tensor(b.aux_of(problem), device=dev)
     rec = H.record(b.field_id, y0_soa, p_soa, t0=problem.t0, t1=problem.t1, rtol=rtol,
-                   atol=atol, max_steps=max_steps, aux=aux)
-    sv = H.saveat(rec, torch.as_tensor(ts, device=dev), keep_states=False)
+                   atol=atol, max_steps=max_steps, aux=aux, entry=_FWD_ENTRY[lane])
+    sv = H.saveat(rec, torch.as_tensor(ts, device=dev), keep_states=False, method=lane)
     status = rec.status.cpu().numpy()
     y_final = rec.y_final.t().cpu().numpy().copy()
     y_saved = sv.ys.permute(2, 0, 1).cpu().numpy()
@@ -55,7 +68,8 @@ def _solve_saveat_tsit5(b, problem, y0, params, ts, *, rtol, atol, dev, max_step
     y_saved[status != 0] = np.nan
     return SolveResult(y_final=y_final, accepted_steps=rec.n_acc.cpu().numpy().astype(np.int64),
                        rejected_steps=rec.n_rej.cpu().numpy().astype(np.int64),
-                       solver=D.HIP_TSIT5, y_saved=y_saved, ts_saved=ts)
+                       solver=D.HIP_ROSENBROCK23 if lane == "rb23" else D.HIP_TSIT5,
+                       y_saved=y_saved, ts_saved=ts)
 
 
 def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
@@ -64,9 +78,11 @@ def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
     lane = _check(name, b, precision)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
     dev = _device(device)
-    if lane == "tsit5":
-        return _solve_saveat_tsit5(b, problem, y0, params, ts, rtol=rtol, atol=atol, dev=dev,
-                                   max_steps=max_steps)
+    if lane in ("tsit5", "rb23"):
+        if lane == "rb23":
+            from .hip_rosenbrock23 import MAX_STEPS as max_steps  # record_rosenbrock's 50000
+        return _solve_saveat_lane(lane, b, problem, y0, params, ts, rtol=rtol, atol=atol,
+                                  dev=dev, max_steps=max_steps)
     y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
     om_soa = torch.as_tensor(np.asarray(params, np.float64), device=dev).t().contiguous()
     K = torch.as_tensor(b.aux_of(problem), device=dev)
@@ -87,9 +103,14 @@ def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
 def make_saveat_closure(name, b, problem, y0, params, saveat, *, wrt, rtol, atol, device,
                         aux=None, precision="float64", max_steps=100000, ckpt_interval=None,
                         strict=False):
-    from .closure import KuramotoClosure, make_tsit5_saveat_closure
+    from .closure import KuramotoClosure, make_rb23_saveat_closure, make_tsit5_saveat_closure
     lane = _check(name, b, precision)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
+    if lane == "rb23":
+        from .hip_rosenbrock23 import MAX_STEPS
+        return make_rb23_saveat_closure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                                        device=device, ts=ts, ckpt_interval=ckpt_interval,
+                                        strict=strict, aux=aux, max_steps=MAX_STEPS)
     if lane == "tsit5":
         return make_tsit5_saveat_closure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                                          device=device, ts=ts, ckpt_interval=ckpt_interval,

@@ -291,13 +291,25 @@ class SaveatResult:
     states: Optional[torch.Tensor]  # [nsave, D, n]
 
 
+_DENSE_ENTRIES = {"tsit5": ("gsx_tsit5_saveat", "gsx_tsit5_adjoint_saveat"),
+                  "rb23": ("gsx_rb23_saveat", "gsx_rb23_adjoint_saveat")}
+
+
+def _dense_entry(method: str, which: int) -> str:
+    try:
+        return _DENSE_ENTRIES[method][which]
+    except KeyError:
+        raise ValueError(f"unknown method {method!r}; known: {sorted(_DENSE_ENTRIES)}") from None
+
+
 def saveat(rec: Tsit5Record, ts: torch.Tensor, *, y0_soa: Optional[torch.Tensor] = None,
            p_soa: Optional[torch.Tensor] = None, y_final: Optional[torch.Tensor] = None,
-           keep_states: bool = True, strict: bool = False,
+           keep_states: bool = True, strict: bool = False, method: str = "tsit5",
            stream: Optional[torch.cuda.Stream] = None) -> SaveatResult:
-    """Dense output on the record's mesh (gsx_tsit5_saveat): the states at the save times
-    ``ts`` (ascending, float64 device tensor), replayed from ``y0_soa`` / ``p_soa`` (default:
-    the recording inputs, whose final-time saves then copy the record's final state)."""
+    """Dense output on the record's mesh (gsx_tsit5_saveat, or gsx_rb23_saveat for a
+    Rosenbrock23 record with ``method="rb23"``): the states at the save times ``ts``
+    (ascending, float64 device tensor), replayed from ``y0_soa`` / ``p_soa`` (default: the
+    recording inputs, whose final-time saves then copy the record's final state)."""
     if y0_soa is None and p_soa is None and y_final is None:
         y_final = rec.y_final
     y0_soa = rec.y0 if y0_soa is None else y0_soa
@@ -309,7 +321,7 @@ def saveat(rec: Tsit5Record, ts: torch.Tensor, *, y0_soa: Optional[torch.Tensor]
     st_t = torch.empty((k, rec.n), dtype=torch.float64, device=dev)
     states = torch.empty((k, rec.D, rec.n), dtype=torch.float64, device=dev) \
         if keep_states else None
-    N.call("gsx_tsit5_saveat", rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux), _ptr(y0_soa),
+    N.call(_dense_entry(method, 0), rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux), _ptr(y0_soa),
            _ptr(p_soa), rec.t0, _ptr(rec.tape), _ptr(rec.n_acc), _ptr(y_final), _ptr(ts), k,
            _ptr(ys),
            _ptr(step), _ptr(st_t), _ptr(states), flags_for(strict), _stream_handle(stream))
@@ -319,9 +331,11 @@ def saveat(rec: Tsit5Record, ts: torch.Tensor, *, y0_soa: Optional[torch.Tensor]
 def adjoint_saveat(rec: Tsit5Record, ts: torch.Tensor, sv: SaveatResult,
                    ysbar: torch.Tensor, ybar: Optional[torch.Tensor] = None, *,
                    ckpt: Optional[torch.Tensor] = None, p_soa: Optional[torch.Tensor] = None,
-                   strict: bool = False, stream: Optional[torch.cuda.Stream] = None):
+                   strict: bool = False, method: str = "tsit5",
+                   stream: Optional[torch.cuda.Stream] = None):
     """dL/dy0 [D, n] and dL/dp [P, n] for save cotangents ``ysbar [nsave, D, n]`` and an
-    optional final-state cotangent ``ybar [D, n]`` (gsx_tsit5_adjoint_saveat)."""
+    optional final-state cotangent ``ybar [D, n]`` (gsx_tsit5_adjoint_saveat, or
+    gsx_rb23_adjoint_saveat with ``method="rb23"``)."""
     dev = rec.y0.device
     k = int(ts.shape[0])
     p_soa = rec.p if p_soa is None else p_soa
@@ -332,7 +346,7 @@ def adjoint_saveat(rec: Tsit5Record, ts: torch.Tensor, sv: SaveatResult,
     mu = torch.empty((k, rec.D, rec.n), dtype=torch.float64, device=dev)
     y0bar = torch.empty((rec.D, rec.n), dtype=torch.float64, device=dev)
     pbar = torch.empty((rec.P, rec.n), dtype=torch.float64, device=dev)
-    N.call("gsx_tsit5_adjoint_saveat", rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux),
+    N.call(_dense_entry(method, 1), rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux),
            _ptr(rec.y0), _ptr(p_soa), rec.t0, _ptr(rec.tape), rec.cap, _ptr(rec.n_acc),
            _ptr(ckpt), rec.ckpt_interval, _ptr(ts), k, _ptr(sv.save_step), _ptr(sv.save_t),
            _ptr(sv.states), _ptr(ysbar), _ptr(ybar), _ptr(mu), _ptr(y0bar), _ptr(pbar),

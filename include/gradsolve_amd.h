@@ -212,6 +212,32 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux,
                      const double* ybar, double* y0bar, double* pbar, int flags,
                      void* stream);
 
+/* Dense output on a recorded Rosenbrock23 mesh: the bracket-and-re-step of gsx_tsit5_saveat
+ * with rosenbrock23_step as the re-step (W built at the re-step's size). Same arguments and
+ * layouts as gsx_tsit5_saveat; tape and n_acc as gsx_rb23_fwd wrote them. Replaces
+ * replay_solve_rosenbrock_saveat_jax (gradsolve/warp/warp_replay.py:78-84 ->
+ * gradsolve/solvers/dense.py:106-163), the warp_rosenbrock lane of solve(saveat=...)
+ * (gradsolve/api.py:484-489). */
+int gsx_rb23_saveat(int field, int64_t n, int D, int P, const double* aux,
+                    const double* y0, const double* p, double t0,
+                    const double* tape, const int32_t* n_acc, const double* y_final,
+                    const double* ts, int64_t nsave, double* ys, int32_t* save_step,
+                    double* save_t, double* states, int flags, void* stream);
+
+/* Gradient of a Rosenbrock23 saveat series: the re-steps' VJPs (through their solves and
+ * Jacobians) into mu, then the gsx_rb23_adjoint sweep with each save's cotangent added at its
+ * bracketing step. Same arguments as gsx_tsit5_adjoint_saveat; tape, ckpt, tape_cap and
+ * ckpt_interval as gsx_rb23_fwd wrote them. Replaces jax.grad through
+ * replay_solve_rosenbrock_saveat_jax (gradsolve/warp/warp_replay.py:78-84, 364-382). */
+int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                            const double* y0, const double* p, double t0,
+                            const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                            const double* ckpt, int32_t ckpt_interval,
+                            const double* ts, int64_t nsave, const int32_t* save_step,
+                            const double* save_t, const double* states, const double* ysbar,
+                            const double* ybar, double* mu, double* y0bar, double* pbar,
+                            int flags, void* stream);
+
 /* ---- Persistent work-queue form (ensembles with very uneven step counts) ----------------
  * No reference counterpart as a kernel: the reference integrates one trajectory per thread
  * with one t1 per ensemble (gradsolve/warp/warp_ode.py:292) and leaves a wave idle until its

@@ -830,6 +830,63 @@ int adjoint_ensemble(int field, int64_t n, int D, int P, const double* aux, cons
   return 0;
 }
 
+// Bracket-and-re-step dense output (gradsolve/solvers/dense.py:106-163): while the mesh is
+// replayed, each save time remembers the start state and time of the last step with
+// dt > 0 and t < ts <= t + dt; afterwards a save at or past the final time is y_final, a
+// bracketed save is one step of size ts - t_left from there, and any other save is y0.
+template <int M>
+int replay_saveat_ensemble(int field, int64_t n, int D, int P, const double* aux,
+                           const double* y0, const double* p, double t0, const double* tape,
+                           int64_t S, const double* ts, int64_t k, double* y_final, double* ys,
+                           int nthreads) {
+  FieldSpec fs{field, D, P, aux};
+  if (!field_ok(fs) || n < 0 || S < 0 || k < 0) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j) {
+    const double* tp = tape + j * S;
+    const double* pj = p + j * P;
+    std::vector<double> yl(k * D), tl(k, t0);
+    std::vector<char> written(k, 0);
+    double y[kMaxD], yn[kMaxD];
+    for (int i = 0; i < D; ++i) y[i] = y0[j * D + i];
+    for (int64_t q = 0; q < k; ++q)
+      for (int i = 0; i < D; ++i) yl[q * D + i] = y[i];
+    double t = t0;
+    for (int64_t s = 0; s < S; ++s) {
+      const double dt = tp[s];
+      const double tn = t + dt;
+      for (int64_t q = 0; q < k; ++q) {
+        if (dt > 0.0 && ts[q] > t && ts[q] <= tn) {
+          for (int i = 0; i < D; ++i) yl[q * D + i] = y[i];
+          tl[q] = t;
+          written[q] = 1;
+        }
+      }
+      if (M == kTsit5)
+        tsit5_step(fs, t, y, dt, pj, yn);
+      else
+        rb23_step(fs, t, y, dt, pj, yn);
+      for (int i = 0; i < D; ++i) y[i] = yn[i];
+      t = tn;
+    }
+    for (int i = 0; i < D; ++i) y_final[j * D + i] = y[i];
+    for (int64_t q = 0; q < k; ++q) {
+      double* out = ys + (j * k + q) * D;
+      if (ts[q] >= t) {
+        for (int i = 0; i < D; ++i) out[i] = y[i];
+      } else if (written[q]) {
+        if (M == kTsit5)
+          tsit5_step(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
+        else
+          rb23_step(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
+      } else {
+        for (int i = 0; i < D; ++i) out[i] = y0[j * D + i];
+      }
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -921,46 +978,8 @@ int gso_tsit5_replay_saveat(int field, int64_t n, int D, int P, const double* au
                             const double* y0, const double* p, double t0, const double* tape,
                             int64_t S, const double* ts, int64_t k, double* y_final,
                             double* ys, int nthreads) {
-  FieldSpec fs{field, D, P, aux};
-  if (!field_ok(fs) || n < 0 || S < 0 || k < 0) return -1;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
-  for (int64_t j = 0; j < n; ++j) {
-    const double* tp = tape + j * S;
-    const double* pj = p + j * P;
-    std::vector<double> yl(k * D), tl(k, t0);
-    std::vector<char> written(k, 0);
-    double y[kMaxD], yn[kMaxD];
-    for (int i = 0; i < D; ++i) y[i] = y0[j * D + i];
-    for (int64_t q = 0; q < k; ++q)
-      for (int i = 0; i < D; ++i) yl[q * D + i] = y[i];
-    double t = t0;
-    for (int64_t s = 0; s < S; ++s) {
-      const double dt = tp[s];
-      const double tn = t + dt;
-      for (int64_t q = 0; q < k; ++q) {
-        if (dt > 0.0 && ts[q] > t && ts[q] <= tn) {
-          for (int i = 0; i < D; ++i) yl[q * D + i] = y[i];
-          tl[q] = t;
-          written[q] = 1;
-        }
-      }
-      tsit5_step(fs, t, y, dt, pj, yn);
-      for (int i = 0; i < D; ++i) y[i] = yn[i];
-      t = tn;
-    }
-    for (int i = 0; i < D; ++i) y_final[j * D + i] = y[i];
-    for (int64_t q = 0; q < k; ++q) {
-      double* out = ys + (j * k + q) * D;
-      if (ts[q] >= t) {
-        for (int i = 0; i < D; ++i) out[i] = y[i];
-      } else if (written[q]) {
-        tsit5_step(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
-      } else {
-        for (int i = 0; i < D; ++i) out[i] = y0[j * D + i];
-      }
-    }
-  }
-  return 0;
+  return replay_saveat_ensemble<kTsit5>(field, n, D, P, aux, y0, p, t0, tape, S, ts, k,
+                                        y_final, ys, nthreads);
 }
 
 int gso_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
@@ -996,6 +1015,27 @@ int gso_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, cons
                      const double* ybar, double* y0bar, double* pbar, int nthreads) {
   return adjoint_ensemble<kRb23>(field, n, D, P, aux, y0, p, t0, tape, S, ybar, nullptr, 0,
                                  nullptr, y0bar, pbar, nthreads);
+}
+
+int gso_rb23_replay_saveat(int field, int64_t n, int D, int P, const double* aux,
+                           const double* y0, const double* p, double t0, const double* tape,
+                           int64_t S, const double* ts, int64_t k, double* y_final, double* ys,
+                           int nthreads) {
+  return replay_saveat_ensemble<kRb23>(field, n, D, P, aux, y0, p, t0, tape, S, ts, k, y_final,
+                                       ys, nthreads);
+}
+
+int gso_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                            const double* y0, const double* p, double t0, const double* tape,
+                            int64_t S, const double* ts, int64_t k, const double* ysbar,
+                            const double* yfbar, double* y0bar, double* pbar, int nthreads) {
+  std::vector<double> zeros;
+  if (!ysbar) {
+    zeros.assign(static_cast<size_t>(n * k * D), 0.0);
+    ysbar = zeros.data();
+  }
+  return adjoint_ensemble<kRb23>(field, n, D, P, aux, y0, p, t0, tape, S, yfbar, ts, k, ysbar,
+                                 y0bar, pbar, nthreads);
 }
 
 int gso_field(int field, int D, int P, const double* aux, double t, const double* y,

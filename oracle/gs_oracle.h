@@ -93,6 +93,18 @@ int gso_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                              const double* ysbar, const double* yfbar,
                              double* y0bar, double* pbar, int nthreads);
 
+/* Rosenbrock23 dense output: the same bracket-and-re-step with rosenbrock23_step as the
+ * re-step (gradsolve/warp/warp_replay.py:78-84), and its adjoint. */
+int gso_rb23_replay_saveat(int field, int64_t n, int D, int P, const double* aux,
+                           const double* y0, const double* p, double t0,
+                           const double* tape, int64_t S, const double* ts, int64_t k,
+                           double* y_final, double* ys, int nthreads);
+int gso_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                            const double* y0, const double* p, double t0,
+                            const double* tape, int64_t S, const double* ts, int64_t k,
+                            const double* ysbar, const double* yfbar,
+                            double* y0bar, double* pbar, int nthreads);
+
 /* Adaptive Rosenbrock23 forward + record (gradsolve_g2_spike_oracle.py:138-184;
  * _warp_rosenbrock.py:383-452), W = I - d*h*J factored once per step. */
 int gso_rb23_record(int field, int64_t n, int D, int P, const double* aux,

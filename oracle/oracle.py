@@ -64,10 +64,12 @@ def _declare(L):
     adj = [i, i64, i, i, _dp, _dp, _dp, d, _dp, i64, _dp, _dp, _dp, i]
     L.gso_tsit5_adjoint.argtypes = adj
     L.gso_rb23_adjoint.argtypes = adj
-    L.gso_tsit5_replay_saveat.argtypes = [i, i64, i, i, _dp, _dp, _dp, d, _dp, i64, _dp, i64,
-                                          _dp, _dp, i]
-    L.gso_tsit5_adjoint_saveat.argtypes = [i, i64, i, i, _dp, _dp, _dp, d, _dp, i64, _dp, i64,
-                                           _dp, _dp, _dp, _dp, i]
+    rsv = [i, i64, i, i, _dp, _dp, _dp, d, _dp, i64, _dp, i64, _dp, _dp, i]
+    asv = [i, i64, i, i, _dp, _dp, _dp, d, _dp, i64, _dp, i64, _dp, _dp, _dp, _dp, i]
+    L.gso_tsit5_replay_saveat.argtypes = rsv
+    L.gso_rb23_replay_saveat.argtypes = rsv
+    L.gso_tsit5_adjoint_saveat.argtypes = asv
+    L.gso_rb23_adjoint_saveat.argtypes = asv
     L.gso_tsit5_teacher_forced.argtypes = [i, i64, i, i, _dp, _dp, _dp, d, _dp, d, d, _dp, i64,
                                            _i64p, _dp, _i64p, _i64p, i]
     L.gso_tsit5_teacher_forced.restype = ctypes.c_int
@@ -76,7 +78,8 @@ def _declare(L):
     L.gso_count_ops.argtypes = [i, i, i, i, _dp, _dp, _dp]
     for name in ("gso_tsit5_tableau", "gso_tsit5_record", "gso_rb23_record", "gso_tsit5_replay",
                  "gso_rb23_replay", "gso_tsit5_adjoint", "gso_rb23_adjoint",
-                 "gso_tsit5_replay_saveat", "gso_tsit5_adjoint_saveat", "gso_field",
+                 "gso_tsit5_replay_saveat", "gso_tsit5_adjoint_saveat", "gso_rb23_replay_saveat",
+                 "gso_rb23_adjoint_saveat", "gso_field",
                  "gso_field_vjp", "gso_count_ops"):
         getattr(L, name).restype = ctypes.c_int
 
@@ -215,6 +218,17 @@ def rb23_adjoint(field, y0, params, t0, tape, ybar, *, aux=None, nthreads=None):
 
 
 def tsit5_replay_saveat(field, y0, params, t0, tape, ts, *, aux=None, nthreads=None):
+    return _replay_saveat(lib().gso_tsit5_replay_saveat, field, y0, params, t0, tape, ts, aux,
+                          nthreads)
+
+
+def rb23_replay_saveat(field, y0, params, t0, tape, ts, *, aux=None, nthreads=None):
+    """Rosenbrock23 bracket-and-re-step (gradsolve/warp/warp_replay.py:78-84)."""
+    return _replay_saveat(lib().gso_rb23_replay_saveat, field, y0, params, t0, tape, ts, aux,
+                          nthreads)
+
+
+def _replay_saveat(fn, field, y0, params, t0, tape, ts, aux, nthreads):
     y0 = _f64(np.atleast_2d(y0))
     params = _f64(np.atleast_2d(params))
     tape = _f64(np.atleast_2d(tape))
@@ -224,15 +238,25 @@ def tsit5_replay_saveat(field, y0, params, t0, tape, ts, *, aux=None, nthreads=N
     yf = np.zeros((n, D))
     ys = np.zeros((n, k, D))
     auxa = None if aux is None else _f64(aux)
-    _check(lib().gso_tsit5_replay_saveat(field, n, D, params.shape[1], _ptr(auxa), _ptr(y0),
-                                         _ptr(params), float(t0), _ptr(tape), tape.shape[1],
-                                         _ptr(ts), k, _ptr(yf), _ptr(ys), _threads(nthreads)),
+    _check(fn(field, n, D, params.shape[1], _ptr(auxa), _ptr(y0), _ptr(params), float(t0),
+              _ptr(tape), tape.shape[1], _ptr(ts), k, _ptr(yf), _ptr(ys), _threads(nthreads)),
            "replay_saveat")
     return yf, ys
 
 
 def tsit5_adjoint_saveat(field, y0, params, t0, tape, ts, ysbar, yfbar=None, *, aux=None,
                          nthreads=None):
+    return _adjoint_saveat(lib().gso_tsit5_adjoint_saveat, field, y0, params, t0, tape, ts,
+                           ysbar, yfbar, aux, nthreads)
+
+
+def rb23_adjoint_saveat(field, y0, params, t0, tape, ts, ysbar, yfbar=None, *, aux=None,
+                        nthreads=None):
+    return _adjoint_saveat(lib().gso_rb23_adjoint_saveat, field, y0, params, t0, tape, ts,
+                           ysbar, yfbar, aux, nthreads)
+
+
+def _adjoint_saveat(fn, field, y0, params, t0, tape, ts, ysbar, yfbar, aux, nthreads):
     y0 = _f64(np.atleast_2d(y0))
     params = _f64(np.atleast_2d(params))
     tape = _f64(np.atleast_2d(tape))
@@ -244,10 +268,9 @@ def tsit5_adjoint_saveat(field, y0, params, t0, tape, ts, ysbar, yfbar=None, *, 
     y0b = np.zeros((n, D))
     pb = np.zeros((n, P))
     auxa = None if aux is None else _f64(aux)
-    _check(lib().gso_tsit5_adjoint_saveat(field, n, D, P, _ptr(auxa), _ptr(y0), _ptr(params),
-                                          float(t0), _ptr(tape), tape.shape[1], _ptr(ts),
-                                          ts.shape[0], _ptr(ysbar), _ptr(yfb), _ptr(y0b),
-                                          _ptr(pb), _threads(nthreads)), "adjoint_saveat")
+    _check(fn(field, n, D, P, _ptr(auxa), _ptr(y0), _ptr(params), float(t0), _ptr(tape),
+              tape.shape[1], _ptr(ts), ts.shape[0], _ptr(ysbar), _ptr(yfb), _ptr(y0b), _ptr(pb),
+              _threads(nthreads)), "adjoint_saveat")
     return y0b, pb
 
 

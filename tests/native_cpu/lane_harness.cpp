@@ -70,6 +70,23 @@ int run_rb_adj(const AdjArgs& a) {
   for (int64_t j = 0; j < a.n; ++j) rb23_adj_lane<F>(a, j, seg.data(), 1);
   return 0;
 }
+template <class F>
+int run_rb_saveat(const SaveArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) rb23_saveat_lane<F>(a, j);
+  return 0;
+}
+// As gsx_rb23_adjoint_saveat runs it: the re-steps' VJPs, then the segment adjoint with the
+// saves injected.
+template <class F>
+int run_rb_adj_saveat(const SaveVjpArgs& v, AdjArgs a) {
+  for (int64_t j = 0; j < v.n; ++j) rb23_saveat_vjp_lane<F>(v, j);
+  a.ybar = v.ybar_eff;
+  a.nsave = v.nsave;
+  a.save_step = v.save_step;
+  a.mu = v.mu;
+  a.pbar_in = v.pbar;
+  return run_rb_adj<F>(a);
+}
 }  // namespace
 
 extern "C" {
@@ -140,6 +157,28 @@ int lh_rb23_adjoint(int field, int64_t n, const double* p, double t0, const doub
   AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
   if (field == 3) return run_rb_adj<FRobertson>(a);
   if (field == 2) return run_rb_adj<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_rb23_saveat(int field, int64_t n, const double* y0, const double* p, double t0,
+                   const double* tape, const int32_t* n_acc, const double* ts, int64_t k,
+                   double* ys, int32_t* save_step, double* save_t, double* states) {
+  SaveArgs a{n, y0, p, nullptr, t0, tape, n_acc, ts, k, ys, save_step, save_t, states,
+             nullptr};
+  if (field == 3) return run_rb_saveat<FRobertson>(a);
+  if (field == 2) return run_rb_saveat<FLinearDecay<2>>(a);
+  return -1;
+}
+int lh_rb23_adjoint_saveat(int field, int64_t n, const double* p, double t0,
+                           const double* tape, int64_t cap, const int32_t* n_acc,
+                           const double* ckpt, int32_t C, const double* ts, int64_t k,
+                           const int32_t* save_step, const double* save_t,
+                           const double* states, const double* ysbar, const double* ybar,
+                           double* mu, double* y0bar, double* pbar) {
+  SaveVjpArgs v{n, p, nullptr, n_acc, ts, k, save_step, save_t, states, ysbar, ybar, mu,
+                y0bar, pbar};
+  AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  if (field == 3) return run_rb_adj_saveat<FRobertson>(v, a);
+  if (field == 2) return run_rb_adj_saveat<FLinearDecay<2>>(v, a);
   return -1;
 }
 int lh_tsit5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,

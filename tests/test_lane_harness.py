@@ -289,3 +289,75 @@ def test_saveat_and_its_gradient_match_oracle(lh, C):
     og0, ogp = O.tsit5_adjoint_saveat(O.LORENZ, e.y0, e.params, 0.0, padded, ts, ysbar, ybar)
     np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=1e-12 * np.abs(ogp).max())
     np.testing.assert_allclose(g0.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
+
+
+@pytest.mark.parametrize("C", [1, 4])
+def test_rb23_lanes_and_saveat_match_oracle(lh, C):
+    """The Rosenbrock23 lane bodies (strict build) against the oracle on Robertson: the record
+    (counts, tape and final state bit for bit), the replay, the adjoint (1e-12), the dense
+    output of gsx_rb23_saveat bit for bit, and the gradient of gsx_rb23_adjoint_saveat (save
+    cotangents plus a final-state cotangent) to 1e-12, with every buffer bounds-guarded."""
+    vp, i64, i32, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, \
+        ctypes.c_int
+    lh.lh_rb23_fwd.argtypes = [i, i64, vp, vp, d, d, vp, d, d, i64, vp, vp, vp, vp, vp, i64, vp,
+                               i32]
+    lh.lh_rb23_replay.argtypes = [i, i64, vp, vp, d, vp, vp, vp, vp, i32]
+    lh.lh_rb23_adjoint.argtypes = [i, i64, vp, d, vp, i64, vp, vp, i32, vp, vp, vp]
+    lh.lh_rb23_saveat.argtypes = [i, i64, vp, vp, d, vp, vp, vp, i64, vp, vp, vp, vp]
+    lh.lh_rb23_adjoint_saveat.argtypes = [i, i64, vp, d, vp, i64, vp, vp, i32, vp, i64, vp,
+                                          vp, vp, vp, vp, vp, vp, vp]
+    e = W.robertson(12, seed=33)
+    n, cap, t1 = 12, 1024, 20.0
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape = Guarded((cap, n))
+    ck = Guarded((cap // C + 1, 3, n))
+    lh.lh_rb23_fwd(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, None, 1e-6, 1e-10, 50000,
+                   yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C)
+    assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
+    oyf, otape, ona, onr, ost = O.rb23_record(O.ROBERTSON, e.y0, e.params, 0.0, t1, rtol=1e-6,
+                                              atol=1e-10)
+    assert np.all(st.a == 0) and np.array_equal(na.a, ona) and np.array_equal(nr.a, onr)
+    assert np.array_equal(yf.a.T, oyf)
+    padded = np.zeros((n, int(na.a.max())))
+    for j in range(n):
+        padded[j, :na.a[j]] = tape.a[:na.a[j], j]
+    assert np.array_equal(padded, otape[:, :padded.shape[1]])
+    ry, rck = Guarded((3, n)), Guarded((cap // C + 1, 3, n))
+    lh.lh_rb23_replay(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr,
+                      rck.ptr, C)
+    assert ry.intact() and rck.intact() and np.array_equal(ry.a, yf.a)
+    w = np.ascontiguousarray(np.tile([[1.0], [10.0], [1.0]], (1, n)))
+    gy, gp = Guarded((3, n)), Guarded((3, n))
+    lh.lh_rb23_adjoint(3, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+                       w.ctypes.data, gy.ptr, gp.ptr)
+    assert gy.intact() and gp.intact()
+    og0, ogp = O.rb23_adjoint(O.ROBERTSON, e.y0, e.params, 0.0, padded, w.T.copy())
+    np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(gy.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
+    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0])
+    k = ts.shape[0]
+    ys, sstep, st_t, states = Guarded((k, 3, n)), Guarded((k, n), np.int32), \
+        Guarded((k, n)), Guarded((k, 3, n))
+    lh.lh_rb23_saveat(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
+                      ts.ctypes.data, k, ys.ptr, sstep.ptr, st_t.ptr, states.ptr)
+    assert all(g.intact() for g in (ys, sstep, st_t, states))
+    _, oys = O.rb23_replay_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts)
+    assert np.array_equal(ys.a.transpose(2, 0, 1), oys)
+    assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, e.y0)
+    rng = np.random.default_rng(34)
+    ysbar = rng.standard_normal((n, k, 3))
+    ybar = rng.standard_normal((n, 3))
+    mu = Guarded((k, 3, n))
+    g0, gq = Guarded((3, n)), Guarded((3, n))
+    ysb_soa = np.ascontiguousarray(ysbar.transpose(1, 2, 0))  # kept alive across the call
+    ybar_soa = np.ascontiguousarray(ybar.T)
+    lh.lh_rb23_adjoint_saveat(3, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+                              ts.ctypes.data, k, sstep.ptr, st_t.ptr, states.ptr,
+                              ysb_soa.ctypes.data, ybar_soa.ctypes.data, mu.ptr, g0.ptr, gq.ptr)
+    assert all(g.intact() for g in (mu, g0, gq))
+    sg0, sgp = O.rb23_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts, ysbar, ybar)
+    np.testing.assert_allclose(gq.a.T, sgp, rtol=1e-12, atol=1e-12 * np.abs(sgp).max())
+    np.testing.assert_allclose(g0.a.T, sg0, rtol=1e-12, atol=1e-12 * np.abs(sg0).max())
