@@ -59,11 +59,12 @@ SIGNATURES = {
     "gsx_rb23_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                               _i32, _vp, _vp, _vp, _i, _vp]),
     "gsx_persist_lanes": (_i64, [_i]),
+    "gsx_persist_record_len": (_i32, [_i, _i, _i32]),
     "gsx_tsit5_persist_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _d, _d, _i64,
-                                   _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp,
-                                   _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
-    "gsx_tsit5_persist_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _d, _vp, _i64, _vp, _vp,
-                                       _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+                                   _vp, _vp, _vp, _vp, _i64, _vp, _i64, _i32, _vp, _i64, _vp,
+                                   _vp, _vp, _vp, _vp, _i, _vp]),
+    "gsx_tsit5_persist_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _d, _vp, _i64, _vp, _i32,
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_kuramoto_fwd": (_i, [_i64, _i, _vp, _vp, _vp, _d, _d, _d, _d, _i64, _vp, _i64, _vp,
                               _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i, _vp]),
     "gsx_kuramoto_replay": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),

@@ -110,27 +110,28 @@ struct SaveVjpArgs {
 // Persistent work-queue forms (gsx_persist.h describes the layout).
 struct PersistFwdArgs {
   FwdArgs f;           // n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final,
-                       // n_acc, n_rej, status; f.tape / f.ckpt / f.ckpt_interval as below
+                       // n_acc, n_rej, status, ckpt_interval (C); f.tape / f.ckpt unused
   int64_t lanes;       // L = grid * block
-  int64_t lane_cap;    // tape rows per lane
-  int64_t ckpt_cap;    // checkpoint slots per lane
+  double* rec;         // [rec_cap][L][rec_len] segment records
+  int64_t rec_cap;     // records per lane
+  int32_t rec_len;     // doubles per record
   int64_t traj_cap;    // trajectories per lane
   int32_t* order;      // [traj_cap][L]
   int32_t* lane_count; // [L] trajectories processed by each lane
   int32_t* lane_of;    // [n]
-  int32_t* tape_start; // [n]
-  int32_t* ckpt_start; // [n]
+  int32_t* rec_start;  // [n] record of the trajectory's first segment, in its lane's stream
   unsigned long long* queue;  // one counter, zero at launch
   const int32_t* queue_order = nullptr;  // [n] order the queue hands trajectories out, or null
 };
 
 struct PersistAdjArgs {
-  AdjArgs a;           // n, p, aux, t0, tape, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar
+  AdjArgs a;           // n, p, aux, n_acc, ckpt_interval, ybar, y0bar, pbar
   int64_t lanes;
+  const double* rec;   // the forward's segment records
+  int32_t rec_len;
   const int32_t* order;
   const int32_t* lane_count;
-  const int32_t* tape_start;
-  const int32_t* ckpt_start;
+  const int32_t* rec_start;
 };
 
 

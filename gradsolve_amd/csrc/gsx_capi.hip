@@ -440,14 +440,23 @@ int64_t gsx_persist_lanes(int waves_per_simd) {
   return static_cast<int64_t>(cus) * 4 * waves_per_simd * 64;
 }
 
+int32_t gsx_persist_record_len(int field, int D, int32_t ckpt_interval) {
+  FieldInfo fi;
+  if (!field_info(field, D, &fi))
+    return set_error(GSX_E_UNSUPPORTED, "field %d with D=%d is not built", field, D);
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  return gsx::persist_record_len(fi.D + (fi.autonomous ? 0 : 1), ckpt_interval);
+}
+
 int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
                           const double* y0, const double* p, double t0, double t1,
                           const double* t1_per, double rtol, double atol, int64_t max_steps,
                           double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
-                          int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
-                          int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
-                          int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
-                          int32_t* tape_start, int32_t* ckpt_start, void* queue,
+                          int64_t lanes, double* rec, int64_t rec_cap, int32_t ckpt_interval,
+                          int32_t* order, int64_t traj_cap, int32_t* lane_count,
+                          int32_t* lane_of, int32_t* rec_start, void* queue,
                           const int32_t* queue_order, int flags, void* stream) {
   FieldInfo fi;
   int rc = check_field(field, D, P, &fi);
@@ -458,33 +467,29 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
   if (lanes <= 0 || lanes % 256 != 0)
     return set_error(GSX_E_INVALID, "lanes %lld must be a positive multiple of 256",
                      static_cast<long long>(lanes));
-  if (lane_cap < 1 || ckpt_cap < 1 || traj_cap < 1 || lane_cap > INT32_MAX ||
-      ckpt_cap > INT32_MAX)
+  if (rec_cap < 1 || traj_cap < 1 || rec_cap > INT32_MAX || traj_cap > INT32_MAX)
     return set_error(GSX_E_INVALID, "capacities must be in [1, 2^31)");
-  if (lane_cap % gsx::kTapeBlock != 0)
-    return set_error(GSX_E_INVALID, "lane_cap %lld must be a multiple of %d",
-                     static_cast<long long>(lane_cap), gsx::kTapeBlock);
   if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
   if (flags & ~GSX_FP32) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
-  if (!queue || !tape || !ckpt || !order || !lane_count ||
+  if (!queue || !rec || !order || !lane_count ||
       (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status || !lane_of ||
-                 !tape_start || !ckpt_start)))
+                 !rec_start)))
     return set_error(GSX_E_INVALID, "tsit5_persist_fwd: null pointer");
   if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
   PersistFwdArgs pa{};
   pa.f = FwdArgs{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
-                 status, tape, lane_cap, ckpt, ckpt_interval};
+                 status, nullptr, 0, nullptr, ckpt_interval};
   pa.lanes = lanes;
-  pa.lane_cap = lane_cap;
-  pa.ckpt_cap = ckpt_cap;
+  pa.rec = rec;
+  pa.rec_cap = rec_cap;
+  pa.rec_len = gsx::persist_record_len(fi.D + (fi.autonomous ? 0 : 1), ckpt_interval);
   pa.traj_cap = traj_cap;
   pa.order = order;
   pa.lane_count = lane_count;
   pa.lane_of = lane_of;
-  pa.tape_start = tape_start;
-  pa.ckpt_start = ckpt_start;
+  pa.rec_start = rec_start;
   pa.queue = static_cast<unsigned long long*>(queue);
   pa.queue_order = queue_order;
   return persist_fwd_dispatch(field, D, (flags & GSX_FP32) != 0, pa,
@@ -493,9 +498,8 @@ int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
 
 int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* aux,
                               const double* p, double t0, const int32_t* n_acc, int64_t lanes,
-                              const double* tape, const double* ckpt, int32_t ckpt_interval,
-                              const int32_t* order, const int32_t* lane_count,
-                              const int32_t* tape_start, const int32_t* ckpt_start,
+                              const double* rec, int32_t ckpt_interval, const int32_t* order,
+                              const int32_t* lane_count, const int32_t* rec_start,
                               const double* ybar, double* y0bar, double* pbar, int flags,
                               void* stream) {
   FieldInfo fi;
@@ -507,17 +511,18 @@ int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* 
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
   if (flags & ~GSX_FP32) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
-  if (!tape || !ckpt || !order || !lane_count ||
-      (n > 0 && (!p || !n_acc || !tape_start || !ckpt_start || !ybar || !y0bar || !pbar)))
+  if (!rec || !order || !lane_count ||
+      (n > 0 && (!p || !n_acc || !rec_start || !ybar || !y0bar || !pbar)))
     return set_error(GSX_E_INVALID, "tsit5_persist_adjoint: null pointer");
   PersistAdjArgs pa{};
-  pa.a = AdjArgs{n, nullptr, p, aux, t0, tape, 0, n_acc, ckpt, ckpt_interval, ybar, y0bar,
-                 pbar};
+  pa.a = AdjArgs{n, nullptr, p, aux, t0, nullptr, 0, n_acc, nullptr, ckpt_interval, ybar,
+                 y0bar, pbar};
   pa.lanes = lanes;
+  pa.rec = rec;
+  pa.rec_len = gsx::persist_record_len(fi.D + (fi.autonomous ? 0 : 1), ckpt_interval);
   pa.order = order;
   pa.lane_count = lane_count;
-  pa.tape_start = tape_start;
-  pa.ckpt_start = ckpt_start;
+  pa.rec_start = rec_start;
   return persist_adj_dispatch(field, D, (flags & GSX_FP32) != 0, pa,
                               static_cast<hipStream_t>(stream));
 }

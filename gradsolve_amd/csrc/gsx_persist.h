@@ -11,18 +11,36 @@
 // even when their trajectories' lengths differ by 20x.
 //
 // Record layout. Each lane appends to its own streams:
-//   tape  [lane_cap/16][L][16]     accepted step sizes; a lane's steps 16k..16k+15 are one
-//                                  128-byte line, so lanes whose positions have drifted apart
-//                                  still write and read whole lines
-//   ckpt  [ckpt_cap][L][DCK]       trajectory start states and every C-th accepted state;
-//                                  one checkpoint's rows are contiguous
+//   rec   [rec_cap][L][R]          segment records: record r of lane l holds, at
+//                                  ((r * L) + l) * R, the state at the start of one
+//                                  checkpoint segment (DCK doubles; in FP32 the pairs
+//                                  (s, lo), each in the bytes of one double) followed by the
+//                                  step sizes of the segment's C accepted steps. R is DCK + C
+//                                  rounded up to 8 doubles, so a record is whole 64-byte
+//                                  sectors and the adjoint reads each segment's checkpoint
+//                                  and steps with one contiguous access.
 //   order [traj_cap][L]            the trajectories a lane processed, in order
-// and per trajectory j: lane[j], tape_start[j], ckpt_start[j] (positions in that lane's
-// streams), n_acc[j], n_rej[j], status[j].
+// and per trajectory j: lane_of[j], rec_start[j] (its first record in that lane's stream),
+// n_acc[j], n_rej[j], status[j]. Trajectory j's accepted step s is slot DCK + s % C of record
+// rec_start[j] + s / C.
 //
-// Adjoint. Lane l walks its own trajectory list backwards, one checkpoint segment (C accepted
-// steps: re-integrate into LDS, then reverse) per iteration, so per-iteration work is uniform
-// across lanes.
+// Why records. Lanes advance at their own pace, so the lanes of a wavefront write and read
+// different positions of their streams: every per-lane access is a separate memory
+// transaction. A 24-byte checkpoint or an 8-byte step size each touching its own line cost
+// 3-6x their size in HBM traffic (2^22 uneven Lorenz, C = 2: 46 GB written by the forward
+// and 49 GB fetched by the adjoint for 15 GB of data, the adjoint at 48% VALU busy). A record
+// is written within C accepted steps, while its line is still in L2, and read whole.
+//
+// Writing records. For records of up to kStageMaxLen doubles the forward assembles each
+// lane's open record in LDS and stores it whole (16-byte stores) once its C steps are in or its
+// trajectory ends; a record written piecemeal over C steps is evicted from L2 half-written, and
+// partial-sector writes to ECC-protected HBM cost a read-modify-write each (2^22 uneven Lorenz,
+// C = 2: forward at 37% VALU busy). Longer records (C = 16, 32) are written in place: their
+// LDS would not fit four 256-lane blocks per CU.
+//
+// Adjoint. Lane l walks its own trajectory list backwards, one record (checkpoint segment:
+// re-integrate into LDS, then reverse; at C = 2 the pair adjoint) per iteration, so
+// per-iteration work is uniform across lanes.
 #pragma once
 
 #include "gsx_core.h"
@@ -30,12 +48,15 @@
 
 namespace gsx {
 
-constexpr int kTapeBlock = 16;                 // tape entries of one lane per 128-byte line
 constexpr unsigned long long kQueueChunk = 64;  // queue indices a wavefront claims per atomic
+constexpr int kStageMaxLen = 16;                // longest record assembled in LDS (doubles)
 
-// Position pos of lane `lane`'s tape stream in the [lane_cap/16][L][16] array.
-GSX_HD int64_t persist_tape_index(int64_t pos, int64_t lane, int64_t L) {
-  return ((pos >> 4) * L + lane) * kTapeBlock + (pos & (kTapeBlock - 1));
+// Doubles per segment record: the checkpoint rows and C step sizes, in whole 64-byte sectors.
+GSX_HD constexpr int persist_record_len(int dck, int C) { return (dck + C + 7) / 8 * 8; }
+
+// First double of record r of lane `lane`.
+GSX_HD int64_t persist_record(int64_t r, int64_t lane, int64_t L, int R) {
+  return (r * L + lane) * R;
 }
 
 }  // namespace gsx
@@ -43,7 +64,7 @@ GSX_HD int64_t persist_tape_index(int64_t pos, int64_t lane, int64_t L) {
 namespace GSX_NS {
 
 using ::gsx::kQueueChunk;
-using ::gsx::persist_tape_index;
+using ::gsx::persist_record;
 using ::gsx::PersistAdjArgs;
 using ::gsx::PersistFwdArgs;
 
@@ -74,8 +95,9 @@ struct StateF64 {
       k[0][d] = k[6][d];
     }
   }
-  GSX_HD void store_ckpt(double* ck, int64_t row0, int64_t L) const {
-    GSX_UNROLL for (int d = 0; d < D; ++d) ck[(row0 + d) * L] = y[d];
+  // The checkpoint rows at ck[0], ck[stride], ...
+  GSX_HD void store_ckpt(double* ck, int64_t stride) const {
+    GSX_UNROLL for (int d = 0; d < D; ++d) ck[d * stride] = y[d];
   }
   GSX_HD void write_final(const FwdArgs& a, int64_t j) const {
     GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * a.n + j] = y[d];
@@ -119,9 +141,9 @@ struct StateF32 {
     }
   }
   // An FP32 checkpoint row pair (s, lo) occupies the bytes of one FP64 row.
-  GSX_HD void store_ckpt(double* ck, int64_t row0, int64_t L) const {
+  GSX_HD void store_ckpt(double* ck, int64_t stride) const {
     GSX_UNROLL for (int d = 0; d < D; ++d) {
-      float* cell = reinterpret_cast<float*>(ck + (row0 + d) * L);
+      float* cell = reinterpret_cast<float*>(ck + d * stride);
       cell[0] = y[d];
       cell[1] = lo[d];
     }

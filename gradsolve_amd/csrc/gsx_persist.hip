@@ -20,12 +20,18 @@ __global__ void queue_reset_kernel(unsigned long long* q) { *q = 0ull; }
 
 // Held to 128 VGPRs (four waves per SIMD): the Lorenz FP64 lane otherwise takes 129 and would
 // cap the persistent grid at three waves.
-template <class F, class State>
+// kStage: the open record is assembled in LDS (dynamic shared memory of rec_len * kPBlock
+// doubles, element i of this lane's record at [i * kPBlock + threadIdx.x]) and stored whole.
+template <class F, class State, bool kStage>
 __global__ __launch_bounds__(kPBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void persist_fwd_kernel(PersistFwdArgs pa) {
+  static_assert(F::autonomous, "the persistent engine is built for autonomous fields");
   constexpr int DCK = ckpt_rows<F>();
+  extern __shared__ double stage_all[];
+  double* const stg = stage_all + threadIdx.x;
   const FwdArgs& a = pa.f;
   const int64_t L = pa.lanes;
+  const int R = pa.rec_len;
   const int64_t lane = static_cast<int64_t>(blockIdx.x) * kPBlock + threadIdx.x;
   const int wl = static_cast<int>(__lane_id());
   const int32_t C = a.ckpt_interval;
@@ -34,20 +40,50 @@ void persist_fwd_kernel(PersistFwdArgs pa) {
   bool active = false, done = false;
   int64_t j = 0;
   double t = 0.0, t1 = 0.0, dt = 0.0;
-  int32_t acc = 0, rej = 0, ck_left = C;
+  int32_t acc = 0, rej = 0;
   int64_t it = 0;
-  int32_t cursor = 0, ccursor = 0, tcount = 0;
+  int32_t rcursor = 0, tcount = 0;
+  int32_t seg_left = 0;   // free step slots of the open record
+  bool rec_open = false;
+  double* rp = nullptr;   // the open record in HBM
+  double* hp = nullptr;   // its next step slot (in LDS when staged)
   // The wavefront's claimed queue range [cbase, cend) and whether the queue has run dry
   // (wavefront-uniform).
   unsigned long long cbase = 0ull, cend = 0ull;
   bool drained = false;
 
+  // Stores the staged record whole (a partial one when its trajectory ended early).
+  auto close_record = [&]() {
+    if constexpr (kStage) {
+      for (int i = 0; i < R; i += 2)
+        *reinterpret_cast<double2*>(rp + i) = make_double2(stg[i * kPBlock],
+                                                           stg[(i + 1) * kPBlock]);
+    }
+    rec_open = false;
+  };
   auto finish = [&](int32_t status) {
+    if (rec_open) close_record();
     st.write_final(a, j);
     a.n_acc[j] = acc;
     a.n_rej[j] = rej;
     a.status[j] = status;
     active = false;
+  };
+  // Opens the lane's next record at the current state (false: the stream is full).
+  auto open_record = [&]() -> bool {
+    if (rcursor >= pa.rec_cap) return false;
+    rp = pa.rec + persist_record(rcursor, lane, L, R);
+    if constexpr (kStage) {
+      st.store_ckpt(stg, kPBlock);
+      hp = stg + DCK * kPBlock;
+    } else {
+      st.store_ckpt(rp, 1);
+      hp = rp + DCK;
+    }
+    ++rcursor;
+    seg_left = C;
+    rec_open = true;
+    return true;
   };
 
   while (true) {
@@ -93,19 +129,19 @@ void persist_fwd_kernel(PersistFwdArgs pa) {
           acc = 0;
           rej = 0;
           it = 0;
-          ck_left = C;
           active = true;
-          if (tcount >= pa.traj_cap || ccursor >= pa.ckpt_cap) {
+          if (tcount >= pa.traj_cap || rcursor >= pa.rec_cap) {
             finish(kTapeFull);
           } else {
             pa.order[static_cast<int64_t>(tcount) * L + lane] = static_cast<int32_t>(j);
             ++tcount;
             pa.lane_of[j] = static_cast<int32_t>(lane);
-            pa.tape_start[j] = cursor;
-            pa.ckpt_start[j] = ccursor;
-            st.store_ckpt(a.ckpt + (static_cast<int64_t>(ccursor) * L + lane) * DCK, 0, 1);
-            ++ccursor;
-            if (!(t < t1)) finish(kReached);
+            pa.rec_start[j] = rcursor;
+            if (t < t1) {
+              open_record();  // room checked above
+            } else {
+              finish(kReached);
+            }
           }
         } else {
           done = true;
@@ -134,23 +170,17 @@ void persist_fwd_kernel(PersistFwdArgs pa) {
       continue;
     }
     if (err <= 1.0) {
-      if (cursor >= pa.lane_cap) {
-        finish(kTapeFull);
-        continue;
-      }
       t = t + h;
       st.accept();
-      a.tape[persist_tape_index(cursor, lane, L)] = h;
-      ++cursor;
+      *hp = h;
+      hp += kStage ? kPBlock : 1;
       ++acc;
-      if (--ck_left == 0) {
-        if (ccursor >= pa.ckpt_cap) {
+      if (--seg_left == 0) {
+        close_record();
+        if (t < t1 && !open_record()) {
           finish(kTapeFull);
           continue;
         }
-        st.store_ckpt(a.ckpt + (static_cast<int64_t>(ccursor) * L + lane) * DCK, 0, 1);
-        ++ccursor;
-        ck_left = C;
       }
     } else {
       ++rej;
@@ -165,14 +195,18 @@ void persist_fwd_kernel(PersistFwdArgs pa) {
   pa.lane_count[lane] = tcount;
 }
 
-// One lane walks its trajectories backwards, one checkpoint segment per iteration.
+// One lane walks its trajectories backwards, one segment record per iteration.
 template <class F, int CMAX, bool FP32>
 __global__ __launch_bounds__(kPAdjBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void persist_adj_kernel(PersistAdjArgs pa) {
+  static_assert(F::autonomous, "the persistent engine is built for autonomous fields");
   constexpr int D = F::D, P = F::P, DCK = ckpt_rows<F>();
-  __shared__ double seg_all[CMAX * DCK * kPAdjBlock];
+  // segment states for re-integration (the pair path at C = 2 keeps none)
+  constexpr int kSegRows = (CMAX == 2 && !FP32) ? 1 : CMAX;
+  __shared__ double seg_all[kSegRows * DCK * kPAdjBlock];
   const AdjArgs& a = pa.a;
   const int64_t L = pa.lanes;
+  const int R = pa.rec_len;
   const int64_t lane = static_cast<int64_t>(blockIdx.x) * kPAdjBlock + threadIdx.x;
   if (lane >= L) return;
   double* seg = seg_all + threadIdx.x;
@@ -181,7 +215,7 @@ void persist_adj_kernel(PersistAdjArgs pa) {
   int32_t k = pa.lane_count[lane] - 1;
   bool have = false;
   int64_t j = 0;
-  int32_t na = 0, m = 0, tstart = 0, cstart = 0;
+  int32_t na = 0, m = 0, rstart = 0;
   double lam_d[D], pb_d[P], p_d[P];
   float lam_f[D], lam_lo[D], pb_f[P], pb_lo[P], p_f[P];
   while (true) {
@@ -190,8 +224,7 @@ void persist_adj_kernel(PersistAdjArgs pa) {
       j = pa.order[static_cast<int64_t>(k) * L + lane];
       --k;
       na = a.n_acc[j];
-      tstart = pa.tape_start[j];
-      cstart = pa.ckpt_start[j];
+      rstart = pa.rec_start[j];
       m = (na + C - 1) / C - 1;
       GSX_UNROLL for (int d = 0; d < D; ++d) {
         const double v = a.ybar[d * n + j];
@@ -211,11 +244,12 @@ void persist_adj_kernel(PersistAdjArgs pa) {
     if (m >= 0) {
       const int32_t s0 = m * C;
       const int32_t s1 = (s0 + C < na) ? s0 + C : na;
-      const double* ck = a.ckpt + (static_cast<int64_t>(cstart + m) * L + lane) * DCK;
+      const double* rp = pa.rec + persist_record(rstart + m, lane, L, R);
+      const double* hp = rp + DCK;  // the segment's step sizes
       if constexpr (FP32) {
         float y[D], lo[D];
         GSX_UNROLL for (int d = 0; d < D; ++d) {
-          const float* cell = reinterpret_cast<const float*>(ck + d);
+          const float* cell = reinterpret_cast<const float*>(rp + d);
           y[d] = cell[0];
           lo[d] = cell[1];
         }
@@ -226,31 +260,25 @@ void persist_adj_kernel(PersistAdjArgs pa) {
             cell[0] = y[d];
             cell[1] = lo[d];
           }
-          if (s + 1 < s1) {
-            const float hf =
-                static_cast<float>(a.tape[persist_tape_index(tstart + s, lane, L)]);
-            tsit5_advance_f32<F>(y, lo, hf, p_f, a.aux);
-          }
+          if (s + 1 < s1) tsit5_advance_f32<F>(y, lo, static_cast<float>(hp[s - s0]), p_f, a.aux);
         }
         for (int32_t s = s1 - 1; s >= s0; --s) {
           const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d)
             y[d] = reinterpret_cast<const float*>(slot + d * kPAdjBlock)[0];
-          const float hf =
-              static_cast<float>(a.tape[persist_tape_index(tstart + s, lane, L)]);
-          tsit5_step_vjp_f32<F>(y, hf, p_f, lam_f, lam_lo, pb_f, pb_lo, a.aux);
+          tsit5_step_vjp_f32<F>(y, static_cast<float>(hp[s - s0]), p_f, lam_f, lam_lo, pb_f,
+                                pb_lo, a.aux);
         }
       } else if constexpr (CMAX == 2) {
         // Checkpoint interval 2: one advance from the checkpoint forms step s0's stage inputs
         // and y_{s0+1}; step s0+1's own stage inputs serve its VJP, step s0's reuse the kept
         // ones, so every step's stages are formed once and nothing goes through LDS (the
-        // pair adjoint of tsit5_adj_pair_segment, on the persistent layout; a missing second
-        // step runs with h = 0, which leaves lam and pb unchanged).
+        // pair adjoint of tsit5_adj_pair_segment; a missing second step runs with h = 0,
+        // which leaves lam and pb unchanged).
         double y[D];
-        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d];
-        const double h0 = a.tape[persist_tape_index(tstart + s0, lane, L)];
-        const double h1 =
-            (s0 + 1 < na) ? a.tape[persist_tape_index(tstart + s0 + 1, lane, L)] : 0.0;
+        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = rp[d];
+        const double h0 = hp[0];
+        const double h1 = (s0 + 1 < na) ? hp[1] : 0.0;
         double z0[6][D], y1[D];
         tsit5_stage_inputs<F>(0.0, y, h0, p_d, z0, y1, a.aux);
         {
@@ -260,22 +288,19 @@ void persist_adj_kernel(PersistAdjArgs pa) {
         }
         tsit5_step_vjp_z<F>(0.0, z0, h0, p_d, lam_d, pb_d, a.aux);
         (void)s1;
+        (void)seg;
       } else {
         double y[D];
-        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = ck[d];
+        GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = rp[d];
         for (int32_t s = s0; s < s1; ++s) {
           double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d) slot[d * kPAdjBlock] = y[d];
-          if (s + 1 < s1) {
-            const double h = a.tape[persist_tape_index(tstart + s, lane, L)];
-            tsit5_advance<F>(0.0, y, h, p_d, a.aux);
-          }
+          if (s + 1 < s1) tsit5_advance<F>(0.0, y, hp[s - s0], p_d, a.aux);
         }
         for (int32_t s = s1 - 1; s >= s0; --s) {
           const double* slot = seg + static_cast<int64_t>(s - s0) * D * kPAdjBlock;
           GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = slot[d * kPAdjBlock];
-          const double h = a.tape[persist_tape_index(tstart + s, lane, L)];
-          tsit5_step_vjp<F>(0.0, y, h, p_d, lam_d, pb_d, a.aux);
+          tsit5_step_vjp<F>(0.0, y, hp[s - s0], p_d, lam_d, pb_d, a.aux);
         }
       }
       --m;
@@ -298,7 +323,12 @@ int launch_pfwd(const PersistFwdArgs& pa, hipStream_t s) {
   int rc = check_launch("queue_reset_kernel");
   if (rc) return rc;
   const unsigned grid = static_cast<unsigned>(pa.lanes / kPBlock);
-  persist_fwd_kernel<F, State><<<grid, kPBlock, 0, s>>>(pa);
+  if (pa.rec_len <= ::gsx::kStageMaxLen) {
+    const size_t lds = static_cast<size_t>(pa.rec_len) * kPBlock * sizeof(double);
+    persist_fwd_kernel<F, State, true><<<grid, kPBlock, lds, s>>>(pa);
+  } else {
+    persist_fwd_kernel<F, State, false><<<grid, kPBlock, 0, s>>>(pa);
+  }
   return check_launch("persist_fwd_kernel");
 }
 

@@ -2,17 +2,22 @@
 ``gsx_tsit5_persist_adjoint``): buffer sizing, the forward+adjoint pair, and capture of the
 pair in a HIP graph.
 
-Layout. tape[lane_cap/16][lanes][16] (a lane's steps 16k..16k+15 contiguous),
-ckpt[ckpt_cap][lanes][rows], order[traj_cap][lanes].
+Layout. rec[rec_cap][lanes][R]: each lane's stream of segment records, record = the state at
+the start of a checkpoint segment followed by the step sizes of its C accepted steps
+(R = gsx_persist_record_len: rows + C rounded up to 8 doubles); order[traj_cap][lanes].
 
-Capacities. Each lane appends to its own tape, checkpoint and trajectory-list streams, so
-the per-lane capacities follow from the expected number of accepted steps per lane:
-``lane_cap = margin * total_steps / lanes + longest_trajectory`` (the longest trajectory
-bounds what one lane can add after crossing the average). ``estimate_steps`` measures the
-step-count distribution on a random subsample with the per-trajectory kernel. A trajectory
-that still does not fit gets status 3, and ``run_forward`` doubles the capacities and runs
-again, as the reference's device recorder doubles its tape (gradsolve/solvers/record_jax.py:
-192-199).
+Capacities. Each lane appends to its own record and trajectory-list streams, so the per-lane
+capacities follow from the expected accepted steps per lane: a trajectory of S steps takes
+ceil(S / C) records, so ``rec_cap = margin * (mean_steps / C + 1) * n / lanes`` plus the
+longest trajectory's records (what one lane can add after crossing the average). Lanes do not
+share the work evenly: a lane takes trajectories as fast as its wavefront steps, and on
+2^22 uneven Lorenz the busiest lane used 1.56-1.6x the mean records, hence margin 2.
+``estimate_steps`` measures the step-count distribution on a random subsample with the
+per-trajectory kernel. The trajectory list is 4 bytes per entry and sized generously: with a
+longest-first queue the last trajectories every lane draws are the shortest, so the count per
+lane spreads more than the steps per lane. A trajectory that still does not fit gets status 3,
+and ``run_forward`` doubles the capacities and runs again, as the reference's device recorder
+doubles its tape (gradsolve/solvers/record_jax.py:192-199).
 """
 from __future__ import annotations
 
@@ -25,9 +30,6 @@ import torch
 
 from . import _native as N
 from . import hip_tsit5 as H
-
-
-TAPE_BLOCK = 16  # accepted steps of one lane stored contiguously in the tape
 
 
 def _ptr(t):
@@ -60,7 +62,7 @@ class PersistentTsit5:
 
     def __init__(self, field: int, D: int, P: int, n: int, *, est: StepEstimate,
                  waves_per_simd: int = 4, ckpt_interval: int = 8, fp32: bool = False,
-                 margin: float = 1.3, device=None, aux: Optional[torch.Tensor] = None):
+                 margin: float = 2.0, device=None, aux: Optional[torch.Tensor] = None):
         self.field, self.D, self.P, self.n = field, D, P, n
         self.C = int(ckpt_interval)
         self.fp32 = bool(fp32)
@@ -69,26 +71,24 @@ class PersistentTsit5:
         self.waves_per_simd = int(waves_per_simd)
         self.lanes = int(N.call("gsx_persist_lanes", self.waves_per_simd))
         self.rows = int(N.call("gsx_ckpt_rows", field, D))
-        L = self.lanes
-        self.lane_cap = int(math.ceil(margin * est.mean_steps * n / L)) + 2 * est.max_steps + 8
-        self.traj_cap = int(math.ceil(margin * n / L)) + 8
+        self.R = int(N.call("gsx_persist_record_len", field, D, self.C))
+        L, C = self.lanes, self.C
+        per_lane = n / L
+        self.rec_cap = int(math.ceil(margin * (est.mean_steps / C + 1.0) * per_lane)) + \
+            2 * (est.max_steps // C + 1) + 8
+        self.traj_cap = int(math.ceil(4.0 * per_lane)) + 64
         self.queue_order: Optional[torch.Tensor] = None  # see plan_longest_first
-        self._alloc(margin)
+        self._alloc()
 
-    def _alloc(self, margin):
+    def _alloc(self):
         L, n, dev = self.lanes, self.n, self.device
-        # the tape stores a lane's steps in blocks of TAPE_BLOCK (one 128-byte line)
-        self.lane_cap = -(-self.lane_cap // TAPE_BLOCK) * TAPE_BLOCK
-        self.ckpt_cap = self.lane_cap // self.C + self.traj_cap + 8
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
-        self.tape = torch.empty((self.lane_cap // TAPE_BLOCK, L, TAPE_BLOCK), **f64)
-        self.ckpt = torch.empty((self.ckpt_cap, L, self.rows), **f64)
+        self.rec = torch.empty((self.rec_cap, L, self.R), **f64)
         self.order = torch.empty((self.traj_cap, L), **i32)
         self.lane_count = torch.empty(L, **i32)
         self.lane_of = torch.empty(n, **i32)
-        self.tape_start = torch.empty(n, **i32)
-        self.ckpt_start = torch.empty(n, **i32)
+        self.rec_start = torch.empty(n, **i32)
         self.queue = torch.zeros(1, dtype=torch.int64, device=dev)
         self.y_final = torch.empty((self.D, n), **f64)
         self.n_acc = torch.empty(n, **i32)
@@ -120,27 +120,24 @@ class PersistentTsit5:
 
     def bytes(self) -> int:
         return sum(t.numel() * t.element_size() for t in
-                   (self.tape, self.ckpt, self.order, self.lane_of, self.tape_start,
-                    self.ckpt_start))
+                   (self.rec, self.order, self.lane_of, self.rec_start))
 
     def forward(self, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, t1_per=None,
                 stream=None):
         N.call("gsx_tsit5_persist_fwd", self.field, self.n, self.D, self.P, _ptr(self.aux),
                _ptr(y0_soa), _ptr(p_soa), float(t0), float(t1), _ptr(t1_per), float(rtol),
                float(atol), int(max_steps), _ptr(self.y_final), _ptr(self.n_acc),
-               _ptr(self.n_rej), _ptr(self.status), self.lanes, _ptr(self.tape), self.lane_cap,
-               _ptr(self.ckpt), self.ckpt_cap, self.C, _ptr(self.order), self.traj_cap,
-               _ptr(self.lane_count), _ptr(self.lane_of), _ptr(self.tape_start),
-               _ptr(self.ckpt_start), _ptr(self.queue), _ptr(self.queue_order), self.flags,
-               H._stream_handle(stream))
+               _ptr(self.n_rej), _ptr(self.status), self.lanes, _ptr(self.rec), self.rec_cap,
+               self.C, _ptr(self.order), self.traj_cap, _ptr(self.lane_count),
+               _ptr(self.lane_of), _ptr(self.rec_start), _ptr(self.queue),
+               _ptr(self.queue_order), self.flags, H._stream_handle(stream))
 
     def adjoint(self, p_soa, ybar_soa, *, t0, stream=None):
         N.call("gsx_tsit5_persist_adjoint", self.field, self.n, self.D, self.P,
                _ptr(self.aux), _ptr(p_soa), float(t0), _ptr(self.n_acc), self.lanes,
-               _ptr(self.tape), _ptr(self.ckpt), self.C, _ptr(self.order),
-               _ptr(self.lane_count), _ptr(self.tape_start), _ptr(self.ckpt_start),
-               _ptr(ybar_soa), _ptr(self.y0bar), _ptr(self.pbar), self.flags,
-               H._stream_handle(stream))
+               _ptr(self.rec), self.C, _ptr(self.order), _ptr(self.lane_count),
+               _ptr(self.rec_start), _ptr(ybar_soa), _ptr(self.y0bar), _ptr(self.pbar),
+               self.flags, H._stream_handle(stream))
         return self.y0bar, self.pbar
 
     def run_forward(self, y0_soa, p_soa, *, t0, t1, rtol, atol, max_steps, t1_per=None,
@@ -151,21 +148,20 @@ class PersistentTsit5:
                          max_steps=max_steps, t1_per=t1_per)
             if not bool((self.status == 3).any().item()):
                 return
-            self.lane_cap *= 2
+            self.rec_cap *= 2
             self.traj_cap *= 2
-            self._alloc(None)
-        raise RuntimeError("persistent record: tape capacity still exceeded after growth")
+            self._alloc()
+        raise RuntimeError("persistent record: capacity still exceeded after growth")
 
     def padded_tape(self) -> torch.Tensor:
-        """The reference's zero-padded [n, S] tape, gathered from the lane streams."""
+        """The reference's zero-padded [n, S] tape, gathered from the lane records."""
         S = int(self.n_acc.max().item()) if self.n else 0
-        s = torch.arange(S, device=self.tape.device, dtype=torch.int64)[None, :]
-        pos = self.tape_start.to(torch.int64)[:, None] + s
-        lane = self.lane_of.to(torch.int64)[:, None]
+        s = torch.arange(S, device=self.rec.device, dtype=torch.int64)[None, :]
         valid = s < self.n_acc.to(torch.int64)[:, None]
-        pos = pos.clamp(max=self.lane_cap - 1)
-        flat = ((pos // TAPE_BLOCK) * self.lanes + lane) * TAPE_BLOCK + pos % TAPE_BLOCK
-        out = self.tape.reshape(-1)[flat.reshape(-1)].reshape(self.n, S)
+        r = (self.rec_start.to(torch.int64)[:, None] + s // self.C).clamp(max=self.rec_cap - 1)
+        lane = self.lane_of.to(torch.int64)[:, None]
+        flat = (r * self.lanes + lane) * self.R + self.rows + s % self.C
+        out = self.rec.reshape(-1)[flat.reshape(-1)].reshape(self.n, S)
         return torch.where(valid, out, torch.zeros((), dtype=out.dtype, device=out.device))
 
 

@@ -243,39 +243,43 @@ int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
  * with one t1 per ensemble (gradsolve/warp/warp_ode.py:292) and leaves a wave idle until its
  * slowest lane finishes. Here a fixed grid of `lanes` lanes (a multiple of 256; see
  * gsx_persist_lanes) takes one trial step per lane per iteration and refills idle lanes from
- * an atomic queue (each wavefront claims 64 indices per atomic). Each lane appends its
- * accepted step sizes to its own tape stream, stored as tape[lane_cap/16][lanes][16] (lane l's
- * position s at ((s/16)*lanes + l)*16 + s%16; lane_cap a multiple of 16), its checkpoints to
- * ckpt[ckpt_cap][lanes][ckpt_rows] (one slot at each trajectory's start and one every
- * ckpt_interval accepted steps), and the trajectories it ran to order[traj_cap][lanes];
- * lane_count[lanes] counts them; per trajectory j the lane and the
- * stream positions are lane_of[j], tape_start[j], ckpt_start[j]. `queue` is 8 bytes of device
- * memory the call resets (with a kernel, so the sequence can be captured in a graph).
- * A trajectory that does not fit its lane's streams gets status 3; the caller grows the
- * capacities and calls again. flags: 0 or GSX_FP32. t1_per may be NULL. queue_order (NULL:
- * index order) is the order in which the queue hands trajectories out, a permutation of
- * 0..n-1; handing out the longest first (e.g. by t1_per descending) keeps the lanes' last
- * trajectories short, so fewer lanes idle while the longest ones finish. An entry outside
- * [0, n) retires the lane that draws it. */
+ * an atomic queue (each wavefront claims 64 indices per atomic). Each lane appends to its own
+ * stream of segment records rec[rec_cap][lanes][R], R = gsx_persist_record_len(field, D,
+ * ckpt_interval) doubles: record r of lane l, at ((r*lanes) + l)*R, holds the state at the
+ * start of one checkpoint segment (ckpt_rows doubles; FP32 stores each (value, compensation)
+ * pair in the bytes of one double) and then the step sizes of the segment's ckpt_interval
+ * accepted steps. R is ckpt_rows + ckpt_interval rounded up to 8, so a record is whole
+ * 64-byte sectors: written while its line is in L2 and read whole, where separately strided
+ * checkpoints and step sizes would each cost a memory transaction per lane. The
+ * trajectories a lane ran go to order[traj_cap][lanes]; lane_count[lanes] counts them; per
+ * trajectory j the lane and its first record are lane_of[j] and rec_start[j], so its accepted
+ * step s is slot ckpt_rows + s % ckpt_interval of record rec_start[j] + s / ckpt_interval.
+ * `queue` is 8 bytes of device memory the call resets (with a kernel, so the sequence can be
+ * captured in a graph). A trajectory that does not fit its lane's streams gets status 3; the
+ * caller grows the capacities and calls again. flags: 0 or GSX_FP32. t1_per may be NULL.
+ * queue_order (NULL: index order) is the order in which the queue hands trajectories out, a
+ * permutation of 0..n-1; handing out the longest first (e.g. by t1_per descending) keeps the
+ * lanes' last trajectories short, so fewer lanes idle while the longest ones finish. An entry
+ * outside [0, n) retires the lane that draws it. */
 int64_t gsx_persist_lanes(int waves_per_simd);
+int32_t gsx_persist_record_len(int field, int D, int32_t ckpt_interval);
 int gsx_tsit5_persist_fwd(int field, int64_t n, int D, int P, const double* aux,
                           const double* y0, const double* p, double t0, double t1,
                           const double* t1_per, double rtol, double atol, int64_t max_steps,
                           double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
-                          int64_t lanes, double* tape, int64_t lane_cap, double* ckpt,
-                          int64_t ckpt_cap, int32_t ckpt_interval, int32_t* order,
-                          int64_t traj_cap, int32_t* lane_count, int32_t* lane_of,
-                          int32_t* tape_start, int32_t* ckpt_start, void* queue,
+                          int64_t lanes, double* rec, int64_t rec_cap, int32_t ckpt_interval,
+                          int32_t* order, int64_t traj_cap, int32_t* lane_count,
+                          int32_t* lane_of, int32_t* rec_start, void* queue,
                           const int32_t* queue_order, int flags, void* stream);
 
 /* Adjoint of a persistent record: lane l walks its trajectories (order) backwards, one
- * checkpoint segment per iteration. Same lanes, streams and ckpt_interval as the forward;
- * call only when every status is 0. */
+ * segment record per iteration (at ckpt_interval 2 the pair adjoint, which re-integrates
+ * nothing). Same lanes, records and ckpt_interval as the forward; call only when every
+ * status is 0. */
 int gsx_tsit5_persist_adjoint(int field, int64_t n, int D, int P, const double* aux,
                               const double* p, double t0, const int32_t* n_acc, int64_t lanes,
-                              const double* tape, const double* ckpt, int32_t ckpt_interval,
-                              const int32_t* order, const int32_t* lane_count,
-                              const int32_t* tape_start, const int32_t* ckpt_start,
+                              const double* rec, int32_t ckpt_interval, const int32_t* order,
+                              const int32_t* lane_count, const int32_t* rec_start,
                               const double* ybar, double* y0bar, double* pbar, int flags,
                               void* stream);
 

@@ -26,7 +26,7 @@ def _inputs(n, seed, dev):
     return e, y0, p, t1
 
 
-def _engine(n, y0, p, t1, dev, fp32=False, C=8, waves=4, margin=1.3):
+def _engine(n, y0, p, t1, dev, fp32=False, C=8, waves=4, margin=2.0):
     est = PS.estimate_steps(N.FIELD_LORENZ, y0, p, t0=0.0, t1=10.0, t1_per=t1, rtol=1e-6,
                             atol=1e-6)
     eng = PS.PersistentTsit5(N.FIELD_LORENZ, 3, 1, n, est=est, waves_per_simd=waves,
@@ -212,9 +212,9 @@ def test_capacity_growth(gpu):
     n = 3000
     e, y0, p, t1 = _inputs(n, 3, gpu)
     eng = _engine(n, y0, p, t1, gpu, margin=0.2)
-    eng.lane_cap = 64
+    eng.rec_cap = 8
     eng.traj_cap = 2
-    eng._alloc(None)
+    eng._alloc()
     eng.run_forward(y0, p, t0=0.0, t1=10.0, rtol=1e-6, atol=1e-6, max_steps=100000,
                     t1_per=t1, max_grow=12)
     torch.cuda.synchronize()
