@@ -218,7 +218,9 @@ def lorenz1m(args, dev):
 KURA_RHS_FLOPS = 4 * 32 * 32
 KURA_TRIAL_FLOPS = 6 * KURA_RHS_FLOPS              # k2..k7 (k1 carried over)
 KURA_RESTEP_FLOPS = 6 * KURA_RHS_FLOPS             # k1..k6 of one save re-step
-KURA_VJP_STEP_FLOPS = 5 * KURA_RHS_FLOPS + 6 * 2 * KURA_RHS_FLOPS  # recompute + 6 stage VJPs
+# the adjoint of one step: the couplings of its 6 stage inputs (each once; the stage VJPs reuse
+# their diagonal terms) and the 6 transposed products
+KURA_VJP_STEP_FLOPS = 6 * KURA_RHS_FLOPS + 6 * KURA_RHS_FLOPS
 
 
 def kuramoto(args, dev):
