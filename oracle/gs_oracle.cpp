@@ -12,11 +12,13 @@
 // The core is templated on the scalar type so that the same source, instantiated on an
 // op-counting scalar, measures the algorithmic flop count of each step (gso_count_ops).
 #include "gs_oracle.h"
+#include "../gradsolve_amd/csrc/gsx_rodas5p_tab.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -660,9 +662,222 @@ void rb23_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T
 }
 
 // ---------------------------------------------------------------------------------------
+// Rodas5P (gradsolve/solvers/rodas5p_step.py:119-168), autonomous fields. The alpha/Gamma
+// tableau is the library's generated header (gsx_rodas5p_tab.h: NumPy's conversion of the
+// published a/C form, rodas5p_step.py:63-75); it is pinned separately by its generator test,
+// the published row sums and the observed orders.
+//   W = I - (h gamma) J;  for i = 0..7:  u_i = y + sum_{j<i} alpha_ij k_j,
+//   a_i = 0 + sum_{j<i} Gamma_ij k_j,  k_i = W \ (h f(u_i) + h (J a_i));
+//   y_next = y + sum_i m_i k_i,  err vector = 0 + sum_i (m_i - mhat_i) k_i.
+// ---------------------------------------------------------------------------------------
+namespace r5 = gsx_rodas5p_tab;
+
+template <class T>
+struct R5Stages {
+  T k[8][kMaxD];
+  T J[kMaxD * kMaxD];
+  LU<T> lu;
+};
+
+template <class T>
+void r5_stage_input(int D, const T* y, const T (&k)[8][kMaxD], int i, T* u, T* acc) {
+  for (int d = 0; d < D; ++d) { u[d] = y[d]; acc[d] = T(0.0); }
+  for (int j = 0; j < i; ++j)
+    for (int d = 0; d < D; ++d) {
+      u[d] = u[d] + T(r5::kAlpha[i * 8 + j]) * k[j][d];
+      acc[d] = acc[d] + T(r5::kGammaOff[i * 8 + j]) * k[j][d];
+    }
+}
+
+template <class T>
+void r5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, R5Stages<T>& w) {
+  const int D = fs.D;
+  field_jac(fs, t, y, p, w.J);
+  w.lu.D = D;
+  const T hg = h * T(r5::kGamma);
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      w.lu.a[i * D + j] = (i == j ? T(1.0) : T(0.0)) - hg * w.J[i * D + j];
+  lu_factor(w.lu);
+  for (int i = 0; i < 8; ++i) {
+    T u[kMaxD], acc[kMaxD], F[kMaxD], rhs[kMaxD];
+    r5_stage_input(D, y, w.k, i, u, acc);
+    field_eval(fs, t + T(r5::kC[i]) * h, u, p, F);
+    for (int r = 0; r < D; ++r) {
+      T ja = T(0.0);
+      for (int c = 0; c < D; ++c) ja = ja + w.J[r * D + c] * acc[c];
+      rhs[r] = h * F[r] + h * ja;
+    }
+    lu_solve(w.lu, rhs, w.k[i]);
+  }
+}
+
+// One trial step: y_next and the WRMS error (rodas5p_step.py:158-168).
+template <class T>
+T r5_trial(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* ynew, T rtol,
+           T atol, bool* finite) {
+  const int D = fs.D;
+  R5Stages<T> w;
+  r5_stages(fs, t, y, h, p, w);
+  T e[kMaxD];
+  for (int d = 0; d < D; ++d) { ynew[d] = y[d]; e[d] = T(0.0); }
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < D; ++d) {
+      ynew[d] = ynew[d] + T(r5::kM[i]) * w.k[i][d];
+      e[d] = e[d] + T(r5::kE[i]) * w.k[i][d];
+    }
+  bool ok = true;
+  for (int d = 0; d < D; ++d) ok = ok && std::isfinite(double(ynew[d])) && std::isfinite(double(e[d]));
+  *finite = ok;
+  return wrms(D, e, y, ynew, rtol, atol);
+}
+
+// Value-only step rodas5p_advance (rodas5p_step.py:149-155).
+template <class T>
+void r5_step(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* ynew) {
+  const int D = fs.D;
+  R5Stages<T> w;
+  r5_stages(fs, t, y, h, p, w);
+  for (int d = 0; d < D; ++d) ynew[d] = y[d];
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < D; ++d) ynew[d] = ynew[d] + T(r5::kM[i]) * w.k[i][d];
+}
+
+// The dense-output bundle of a step (rodas5p_step.py:198-226): D_r = 0 + sum_i hd[r][i] k_i.
+template <class T>
+void r5_dense(int D, const R5Stages<T>& w, T (&dv)[3][kMaxD]) {
+  for (int r = 0; r < 3; ++r) {
+    for (int d = 0; d < D; ++d) dv[r][d] = T(0.0);
+    for (int i = 0; i < 8; ++i)
+      for (int d = 0; d < D; ++d) dv[r][d] = dv[r][d] + T(r5::kHD[r * 8 + i]) * w.k[i][d];
+  }
+}
+
+// y(theta) = (1 - th) y_l + th (y_r + (1 - th) (D1 + th (D2 + th D3))) (rodas5p_step.py:171-195).
+template <class T>
+void r5_dense_eval(int D, const T* yl, const T* yr, const T (&dv)[3][kMaxD], T th, T* out) {
+  const T om = T(1.0) - th;
+  for (int d = 0; d < D; ++d)
+    out[d] = om * yl[d] + th * (yr[d] + om * (dv[0][d] + th * (dv[1][d] + th * dv[2][d])));
+}
+
+// Adjoint of r5_step with extra cotangents on the dense vectors: lam = dL/dynew in, dL/dy out;
+// dbar (may be null) = dL/dD_r of this step's bundle; pbar += dL/dp.
+template <class T>
+void r5_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* lam,
+                 const T (*dbar)[kMaxD], T* pbar) {
+  const int D = fs.D;
+  R5Stages<T> w;
+  r5_stages(fs, t, y, h, p, w);
+  T kb[8][kMaxD], yb[kMaxD], Wb[kMaxD * kMaxD], Jb[kMaxD * kMaxD];
+  for (int d = 0; d < D; ++d) yb[d] = lam[d];
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < D; ++d) {
+      kb[i][d] = T(r5::kM[i]) * lam[d];
+      if (dbar)
+        for (int r = 0; r < 3; ++r) kb[i][d] = kb[i][d] + T(r5::kHD[r * 8 + i]) * dbar[r][d];
+    }
+  for (int i = 0; i < D * D; ++i) { Wb[i] = T(0.0); Jb[i] = T(0.0); }
+  for (int i = 7; i >= 0; --i) {
+    T u[kMaxD], acc[kMaxD], rb[kMaxD], Fb[kMaxD], ub[kMaxD], accb[kMaxD];
+    r5_stage_input(D, y, w.k, i, u, acc);
+    lu_solve_t(w.lu, kb[i], rb);  // k_i = W^-1 rhs_i
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) Wb[a * D + b] = Wb[a * D + b] - rb[a] * w.k[i][b];
+    for (int d = 0; d < D; ++d) { Fb[d] = h * rb[d]; ub[d] = T(0.0); }
+    field_vjp(fs, t + T(r5::kC[i]) * h, u, p, Fb, ub, pbar);
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b) Jb[a * D + b] = Jb[a * D + b] + h * rb[a] * acc[b];
+    for (int c = 0; c < D; ++c) {
+      T s = T(0.0);
+      for (int r = 0; r < D; ++r) s = s + w.J[r * D + c] * rb[r];
+      accb[c] = h * s;
+    }
+    for (int d = 0; d < D; ++d) yb[d] = yb[d] + ub[d];
+    for (int j = 0; j < i; ++j)
+      for (int d = 0; d < D; ++d)
+        kb[j][d] = kb[j][d] + T(r5::kAlpha[i * 8 + j]) * ub[d] +
+                   T(r5::kGammaOff[i * 8 + j]) * accb[d];
+  }
+  const T hg = h * T(r5::kGamma);
+  for (int i = 0; i < D * D; ++i) Jb[i] = Jb[i] - hg * Wb[i];
+  field_jac_vjp(fs, y, p, Jb, yb, pbar);
+  for (int d = 0; d < D; ++d) lam[d] = yb[d];
+}
+
+// Record with the device recorder's Rodas5P semantics (gradsolve/solvers/record_jax.py:83-132
+// with RODAS5P_METHOD): a step below the underflow floor (t + h == t or h <= floor) ends the
+// trajectory with status 2; a non-finite trial is rejected and shrinks h by FAC_MIN without a
+// controller update; the I-controller uses exponent -1/5; max_steps counts attempts.
+void r5_record_one(const FieldSpec& fs, const double* y0, const double* p, double t0, double t1,
+                   double dt0, double rtol, double atol, int64_t max_steps, double floor,
+                   double* yf, int64_t* nacc, int64_t* nrej, int32_t* st, double* tape,
+                   int64_t cap) {
+  const int D = fs.D;
+  double y[kMaxD], ynew[kMaxD];
+  for (int i = 0; i < D; ++i) y[i] = y0[i];
+  double t = t0, dt = dt0;
+  int64_t acc = 0, rej = 0, attempts = 0;
+  int32_t status = 0;
+  if (tape)
+    for (int64_t s = 0; s < cap; ++s) tape[s] = 0.0;
+  while (t < t1 && status == 0) {
+    const double h = min_(dt, t1 - t);
+    if (t + h == t || h <= floor) {
+      status = 2;
+      break;
+    }
+    bool finite = true;
+    const double err = r5_trial(fs, t, y, h, p, ynew, rtol, atol, &finite);
+    const bool ok = finite && std::isfinite(err);
+    ++attempts;
+    if (ok && err <= 1.0) {
+      if (tape && acc >= cap) { status = 3; break; }
+      t = t + h;
+      for (int i = 0; i < D; ++i) y[i] = ynew[i];
+      if (tape) tape[acc] = h;
+      ++acc;
+      dt = h * min_(max_(SAFETY * pow_(max_(err, ERR_FLOOR), TSIT5_EXP), FAC_MIN), FAC_MAX);
+    } else {
+      ++rej;
+      dt = ok ? h * min_(max_(SAFETY * pow_(max_(err, ERR_FLOOR), TSIT5_EXP), FAC_MIN), FAC_MAX)
+              : h * FAC_MIN;
+    }
+    if (attempts >= max_steps && t < t1) status = 1;
+  }
+  for (int i = 0; i < D; ++i) yf[i] = y[i];
+  *nacc = acc;
+  *nrej = rej;
+  *st = status;
+}
+
+// ---------------------------------------------------------------------------------------
 // Ensemble drivers.
 // ---------------------------------------------------------------------------------------
-enum Method { kTsit5 = 0, kRb23 = 1 };
+enum Method { kTsit5 = 0, kRb23 = 1, kRodas5p = 2 };
+
+// The value-only step and its VJP of method M.
+template <int M>
+void m_step(const FieldSpec& fs, double t, const double* y, double h, const double* p,
+            double* yn) {
+  if (M == kTsit5)
+    tsit5_step(fs, t, y, h, p, yn);
+  else if (M == kRb23)
+    rb23_step(fs, t, y, h, p, yn);
+  else
+    r5_step(fs, t, y, h, p, yn);
+}
+
+template <int M>
+void m_step_vjp(const FieldSpec& fs, double t, const double* y, double h, const double* p,
+                double* lam, double* pb) {
+  if (M == kTsit5)
+    tsit5_step_vjp(fs, t, y, h, p, lam, pb);
+  else if (M == kRb23)
+    rb23_step_vjp(fs, t, y, h, p, lam, pb);
+  else
+    r5_step_vjp<double>(fs, t, y, h, p, lam, nullptr, pb);
+}
 
 template <int M>
 void record_one(const FieldSpec& fs, const double* y0, const double* p, double t0, double t1,
@@ -742,10 +957,7 @@ int replay_ensemble(int field, int64_t n, int D, int P, const double* aux, const
     double t = t0;
     for (int64_t s = 0; s < S; ++s) {
       const double h = tape[j * S + s];
-      if (M == kTsit5)
-        tsit5_step(fs, t, y, h, p + j * P, yn);
-      else
-        rb23_step(fs, t, y, h, p + j * P, yn);
+      m_step<M>(fs, t, y, h, p + j * P, yn);
       for (int i = 0; i < D; ++i) y[i] = yn[i];
       t = t + h;
     }
@@ -765,10 +977,7 @@ void adjoint_one(const FieldSpec& fs, const double* y0, const double* p, double 
   for (int i = 0; i < D; ++i) ys[i] = y0[i];
   tt[0] = t0;
   for (int64_t s = 0; s < S; ++s) {
-    if (M == kTsit5)
-      tsit5_step(fs, tt[s], &ys[s * D], tape[s], p, &ys[(s + 1) * D]);
-    else
-      rb23_step(fs, tt[s], &ys[s * D], tape[s], p, &ys[(s + 1) * D]);
+    m_step<M>(fs, tt[s], &ys[s * D], tape[s], p, &ys[(s + 1) * D]);
     tt[s + 1] = tt[s] + tape[s];
   }
   double lam[kMaxD], pb[kMaxD];
@@ -794,20 +1003,14 @@ void adjoint_one(const FieldSpec& fs, const double* y0, const double* p, double 
     }
   }
   for (int64_t s = S - 1; s >= 0; --s) {
-    if (M == kTsit5)
-      tsit5_step_vjp(fs, tt[s], &ys[s * D], tape[s], p, lam, pb);
-    else
-      rb23_step_vjp(fs, tt[s], &ys[s * D], tape[s], p, lam, pb);
+    m_step_vjp<M>(fs, tt[s], &ys[s * D], tape[s], p, lam, pb);
     if (ts) {
       for (int64_t q = 0; q < k; ++q) {
         if (br[q] != s) continue;
         double mu[kMaxD];
         for (int i = 0; i < D; ++i) mu[i] = ysbar[q * D + i];
         const double hq = ts[q] - tt[s];
-        if (M == kTsit5)
-          tsit5_step_vjp(fs, tt[s], &ys[s * D], hq, p, mu, pb);
-        else
-          rb23_step_vjp(fs, tt[s], &ys[s * D], hq, p, mu, pb);
+        m_step_vjp<M>(fs, tt[s], &ys[s * D], hq, p, mu, pb);
         for (int i = 0; i < D; ++i) lam[i] += mu[i];
       }
     }
@@ -862,10 +1065,7 @@ int replay_saveat_ensemble(int field, int64_t n, int D, int P, const double* aux
           written[q] = 1;
         }
       }
-      if (M == kTsit5)
-        tsit5_step(fs, t, y, dt, pj, yn);
-      else
-        rb23_step(fs, t, y, dt, pj, yn);
+      m_step<M>(fs, t, y, dt, pj, yn);
       for (int i = 0; i < D; ++i) y[i] = yn[i];
       t = tn;
     }
@@ -875,10 +1075,7 @@ int replay_saveat_ensemble(int field, int64_t n, int D, int P, const double* aux
       if (ts[q] >= t) {
         for (int i = 0; i < D; ++i) out[i] = y[i];
       } else if (written[q]) {
-        if (M == kTsit5)
-          tsit5_step(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
-        else
-          rb23_step(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
+        m_step<M>(fs, tl[q], &yl[q * D], ts[q] - tl[q], pj, out);
       } else {
         for (int i = 0; i < D; ++i) out[i] = y0[j * D + i];
       }
@@ -1036,6 +1233,200 @@ int gso_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
   }
   return adjoint_ensemble<kRb23>(field, n, D, P, aux, y0, p, t0, tape, S, yfbar, ts, k, ysbar,
                                  y0bar, pbar, nthreads);
+}
+
+int gso_rodas5p_record(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0, const double* t1,
+                       const double* dt0, double rtol, double atol, int64_t max_steps,
+                       double* y_final, int64_t* n_acc, int64_t* n_rej, int32_t* status,
+                       double* tape, int64_t cap, int nthreads) {
+  FieldSpec fs{field, D, P, aux};
+  if (!field_ok(fs) || n < 0 || (tape && cap < 0)) return -1;
+  if (field != GSO_F_ROBERTSON && field != GSO_F_LINEAR_DECAY && field != GSO_F_LINEAR_LADDER)
+    return -2;
+  const double eps = std::numeric_limits<double>::epsilon();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j) {
+    const double t1j = t1[j];
+    const double dt0j = dt0 ? dt0[j] : (t1j - t0) / 100.0;
+    const double floor = 4.0 * eps * max_(std::fabs(t0), std::fabs(t1j - t0));
+    r5_record_one(fs, y0 + j * D, p + j * P, t0, t1j, dt0j, rtol, atol, max_steps, floor,
+                  y_final + j * D, n_acc + j, n_rej + j, status + j,
+                  tape ? tape + j * cap : nullptr, cap);
+  }
+  return 0;
+}
+
+int gso_rodas5p_replay(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0, const double* tape,
+                       int64_t S, double* y_final, int nthreads) {
+  return replay_ensemble<kRodas5p>(field, n, D, P, aux, y0, p, t0, tape, S, y_final, nthreads);
+}
+
+int gso_rodas5p_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                        const double* y0, const double* p, double t0, const double* tape,
+                        int64_t S, const double* ybar, double* y0bar, double* pbar,
+                        int nthreads) {
+  return adjoint_ensemble<kRodas5p>(field, n, D, P, aux, y0, p, t0, tape, S, ybar, nullptr, 0,
+                                    nullptr, y0bar, pbar, nthreads);
+}
+
+int gso_rodas5p_replay_saveat(int field, int64_t n, int D, int P, const double* aux,
+                              const double* y0, const double* p, double t0, const double* tape,
+                              int64_t S, const double* ts, int64_t k, double* y_final,
+                              double* ys, int nthreads) {
+  return replay_saveat_ensemble<kRodas5p>(field, n, D, P, aux, y0, p, t0, tape, S, ts, k,
+                                          y_final, ys, nthreads);
+}
+
+int gso_rodas5p_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                               const double* y0, const double* p, double t0,
+                               const double* tape, int64_t S, const double* ts, int64_t k,
+                               const double* ysbar, const double* yfbar, double* y0bar,
+                               double* pbar, int nthreads) {
+  std::vector<double> zeros;
+  if (!ysbar) {
+    zeros.assign(static_cast<size_t>(n * k * D), 0.0);
+    ysbar = zeros.data();
+  }
+  return adjoint_ensemble<kRodas5p>(field, n, D, P, aux, y0, p, t0, tape, S, yfbar, ts, k,
+                                    ysbar, y0bar, pbar, nthreads);
+}
+
+// Dense output by the continuous extension (gradsolve/solvers/dense.py:166-227): a save time
+// bracketed by step s (dt > 0, t_s < ts <= t_s + dt, the last such step) is the step's
+// interpolant at theta = (ts - t_s) / dt; at or past the final time it is y_final, and
+// unbracketed times (at or before t0) are y0.
+int gso_rodas5p_replay_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                    const double* y0, const double* p, double t0,
+                                    const double* tape, int64_t S, const double* ts, int64_t k,
+                                    double* y_final, double* ys, int nthreads) {
+  FieldSpec fs{field, D, P, aux};
+  if (!field_ok(fs) || n < 0 || S < 0 || k < 0) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j) {
+    const double* tp = tape + j * S;
+    const double* pj = p + j * P;
+    double* out = ys + j * k * D;
+    std::vector<char> written(k, 0);
+    double y[kMaxD], yn[kMaxD];
+    for (int i = 0; i < D; ++i) y[i] = y0[j * D + i];
+    double t = t0;
+    for (int64_t s = 0; s < S; ++s) {
+      const double dt = tp[s];
+      const double tn = t + dt;
+      R5Stages<double> w;
+      r5_stages(fs, t, y, dt, pj, w);
+      for (int d = 0; d < D; ++d) yn[d] = y[d];
+      for (int i = 0; i < 8; ++i)
+        for (int d = 0; d < D; ++d) yn[d] = yn[d] + r5::kM[i] * w.k[i][d];
+      double dv[3][kMaxD];
+      bool any = false;
+      for (int64_t q = 0; q < k; ++q) any = any || (dt > 0.0 && ts[q] > t && ts[q] <= tn);
+      if (any) r5_dense(D, w, dv);
+      for (int64_t q = 0; q < k; ++q) {
+        if (dt > 0.0 && ts[q] > t && ts[q] <= tn) {
+          r5_dense_eval(D, y, yn, dv, (ts[q] - t) / dt, out + q * D);
+          written[q] = 1;
+        }
+      }
+      for (int d = 0; d < D; ++d) y[d] = yn[d];
+      t = tn;
+    }
+    for (int i = 0; i < D; ++i) y_final[j * D + i] = y[i];
+    for (int64_t q = 0; q < k; ++q) {
+      if (ts[q] >= t) {
+        for (int i = 0; i < D; ++i) out[q * D + i] = y[i];
+      } else if (!written[q]) {
+        for (int i = 0; i < D; ++i) out[q * D + i] = y0[j * D + i];
+      }
+    }
+  }
+  return 0;
+}
+
+// Its adjoint: a bracketed save sends (1 - th) of its cotangent to y_s, th to y_{s+1} and
+// th (1 - th) th^r to the step's dense vector D_{r+1}, which the step VJP carries to the stages.
+int gso_rodas5p_adjoint_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t S, const double* ts, int64_t k,
+                                     const double* ysbar, const double* yfbar, double* y0bar,
+                                     double* pbar, int nthreads) {
+  FieldSpec fs{field, D, P, aux};
+  if (!field_ok(fs) || n < 0 || S < 0 || k < 0 || P > kMaxD) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < n; ++j) {
+    const double* tp = tape + j * S;
+    const double* pj = p + j * P;
+    const double* sb = ysbar + j * k * D;
+    std::vector<double> st((S + 1) * D), tt(S + 1);
+    for (int i = 0; i < D; ++i) st[i] = y0[j * D + i];
+    tt[0] = t0;
+    for (int64_t s = 0; s < S; ++s) {
+      r5_step(fs, tt[s], &st[s * D], tp[s], pj, &st[(s + 1) * D]);
+      tt[s + 1] = tt[s] + tp[s];
+    }
+    const double t_fin = tt[S];
+    std::vector<int64_t> br(k, -1);
+    double lam[kMaxD], pb[kMaxD], extra[kMaxD];
+    for (int i = 0; i < D; ++i) {
+      lam[i] = yfbar ? yfbar[j * D + i] : 0.0;
+      extra[i] = 0.0;
+    }
+    for (int i = 0; i < P; ++i) pb[i] = 0.0;
+    for (int64_t q = 0; q < k; ++q) {
+      for (int64_t s = 0; s < S; ++s)
+        if (tp[s] > 0.0 && ts[q] > tt[s] && ts[q] <= tt[s] + tp[s]) br[q] = s;
+      if (ts[q] >= t_fin) {
+        for (int i = 0; i < D; ++i) lam[i] += sb[q * D + i];
+        br[q] = -2;
+      } else if (br[q] == -1) {
+        for (int i = 0; i < D; ++i) extra[i] += sb[q * D + i];
+      }
+    }
+    for (int64_t s = S - 1; s >= 0; --s) {
+      double dbar[3][kMaxD];
+      bool any = false;
+      for (int r = 0; r < 3; ++r)
+        for (int i = 0; i < D; ++i) dbar[r][i] = 0.0;
+      for (int64_t q = 0; q < k; ++q) {
+        if (br[q] != s) continue;
+        any = true;
+        const double th = (ts[q] - tt[s]) / tp[s], om = 1.0 - th;
+        for (int i = 0; i < D; ++i) {
+          const double g = sb[q * D + i];
+          lam[i] += th * g;
+          dbar[0][i] += th * om * g;
+          dbar[1][i] += th * om * th * g;
+          dbar[2][i] += th * om * th * th * g;
+        }
+      }
+      r5_step_vjp<double>(fs, tt[s], &st[s * D], tp[s], pj, lam, any ? dbar : nullptr, pb);
+      for (int64_t q = 0; q < k; ++q) {
+        if (br[q] != s) continue;
+        const double om = 1.0 - (ts[q] - tt[s]) / tp[s];
+        for (int i = 0; i < D; ++i) lam[i] += om * sb[q * D + i];
+      }
+    }
+    for (int i = 0; i < D; ++i) y0bar[j * D + i] = lam[i] + extra[i];
+    for (int i = 0; i < P; ++i) pbar[j * P + i] = pb[i];
+  }
+  return 0;
+}
+
+// The Rodas5P tableau the oracle and the library use (alpha, Gamma strictly lower, c, d, m,
+// m - mhat, dense weights), for the tableau tests.
+int gso_rodas5p_tableau(double* alpha, double* gamma_off, double* c, double* d, double* m,
+                        double* e, double* hd, double* gamma) {
+  std::memcpy(alpha, r5::kAlpha, sizeof r5::kAlpha);
+  std::memcpy(gamma_off, r5::kGammaOff, sizeof r5::kGammaOff);
+  std::memcpy(c, r5::kC, sizeof r5::kC);
+  std::memcpy(d, r5::kD, sizeof r5::kD);
+  std::memcpy(m, r5::kM, sizeof r5::kM);
+  std::memcpy(e, r5::kE, sizeof r5::kE);
+  std::memcpy(hd, r5::kHD, sizeof r5::kHD);
+  *gamma = r5::kGamma;
+  return 0;
 }
 
 int gso_field(int field, int D, int P, const double* aux, double t, const double* y,

@@ -105,6 +105,45 @@ int gso_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
                             const double* ysbar, const double* yfbar,
                             double* y0bar, double* pbar, int nthreads);
 
+/* Rodas5P (Steinebach 2023; gradsolve/solvers/rodas5p_step.py, rodas5p_replay.py) for
+ * autonomous stiff fields: record with the device recorder's Rodas5P semantics
+ * (gradsolve/solvers/record_jax.py:83-132: status 2 at the underflow floor, a non-finite
+ * trial shrinks h by 0.2), replay, adjoint, bracket-and-re-step saveat and its adjoint, and the
+ * dense saveat of the order-4 continuous extension (dense.py:166-227) and its adjoint. Same
+ * argument conventions as the Tsit5 / Rosenbrock23 functions. */
+int gso_rodas5p_record(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0, const double* t1,
+                       const double* dt0, double rtol, double atol, int64_t max_steps,
+                       double* y_final, int64_t* n_acc, int64_t* n_rej, int32_t* status,
+                       double* tape, int64_t cap, int nthreads);
+int gso_rodas5p_replay(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0, const double* tape,
+                       int64_t S, double* y_final, int nthreads);
+int gso_rodas5p_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                        const double* y0, const double* p, double t0, const double* tape,
+                        int64_t S, const double* ybar, double* y0bar, double* pbar,
+                        int nthreads);
+int gso_rodas5p_replay_saveat(int field, int64_t n, int D, int P, const double* aux,
+                              const double* y0, const double* p, double t0, const double* tape,
+                              int64_t S, const double* ts, int64_t k, double* y_final,
+                              double* ys, int nthreads);
+int gso_rodas5p_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                               const double* y0, const double* p, double t0,
+                               const double* tape, int64_t S, const double* ts, int64_t k,
+                               const double* ysbar, const double* yfbar, double* y0bar,
+                               double* pbar, int nthreads);
+int gso_rodas5p_replay_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                    const double* y0, const double* p, double t0,
+                                    const double* tape, int64_t S, const double* ts, int64_t k,
+                                    double* y_final, double* ys, int nthreads);
+int gso_rodas5p_adjoint_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t S, const double* ts, int64_t k,
+                                     const double* ysbar, const double* yfbar, double* y0bar,
+                                     double* pbar, int nthreads);
+int gso_rodas5p_tableau(double* alpha, double* gamma_off, double* c, double* d, double* m,
+                        double* e, double* hd, double* gamma);
+
 /* Adaptive Rosenbrock23 forward + record (gradsolve_g2_spike_oracle.py:138-184;
  * _warp_rosenbrock.py:383-452), W = I - d*h*J factored once per step. */
 int gso_rb23_record(int field, int64_t n, int D, int P, const double* aux,

@@ -70,6 +70,14 @@ def _declare(L):
     L.gso_rb23_replay_saveat.argtypes = rsv
     L.gso_tsit5_adjoint_saveat.argtypes = asv
     L.gso_rb23_adjoint_saveat.argtypes = asv
+    L.gso_rodas5p_record.argtypes = rec
+    L.gso_rodas5p_replay.argtypes = rep
+    L.gso_rodas5p_adjoint.argtypes = adj
+    L.gso_rodas5p_replay_saveat.argtypes = rsv
+    L.gso_rodas5p_adjoint_saveat.argtypes = asv
+    L.gso_rodas5p_replay_saveat_dense.argtypes = rsv
+    L.gso_rodas5p_adjoint_saveat_dense.argtypes = asv
+    L.gso_rodas5p_tableau.argtypes = [_dp] * 8
     L.gso_tsit5_teacher_forced.argtypes = [i, i64, i, i, _dp, _dp, _dp, d, _dp, d, d, _dp, i64,
                                            _i64p, _dp, _i64p, _i64p, i]
     L.gso_tsit5_teacher_forced.restype = ctypes.c_int
@@ -79,7 +87,10 @@ def _declare(L):
     for name in ("gso_tsit5_tableau", "gso_tsit5_record", "gso_rb23_record", "gso_tsit5_replay",
                  "gso_rb23_replay", "gso_tsit5_adjoint", "gso_rb23_adjoint",
                  "gso_tsit5_replay_saveat", "gso_tsit5_adjoint_saveat", "gso_rb23_replay_saveat",
-                 "gso_rb23_adjoint_saveat", "gso_field",
+                 "gso_rb23_adjoint_saveat", "gso_rodas5p_record", "gso_rodas5p_replay",
+                 "gso_rodas5p_adjoint", "gso_rodas5p_replay_saveat", "gso_rodas5p_adjoint_saveat",
+                 "gso_rodas5p_replay_saveat_dense", "gso_rodas5p_adjoint_saveat_dense",
+                 "gso_rodas5p_tableau", "gso_field",
                  "gso_field_vjp", "gso_count_ops"):
         getattr(L, name).restype = ctypes.c_int
 
@@ -183,6 +194,44 @@ def tsit5_teacher_forced(field, y0, params, t0, t1, tape, n_acc, *, rtol, atol, 
                                           rej_at.ctypes.data_as(_i64p), _threads(nthreads)),
            "teacher_forced")
     return ho, nrej, rej_at
+
+
+def rodas5p_record(field, y0, params, t0, t1, *, rtol, atol, max_steps=50000, cap=8192,
+                   aux=None, dt0=None, nthreads=None):
+    """Rodas5P record with the device recorder's semantics (record_jax.py:83-132)."""
+    return _record(lib().gso_rodas5p_record, field, y0, params, t0, t1, rtol, atol, max_steps,
+                   cap, aux, dt0, nthreads)
+
+
+def rodas5p_tableau():
+    """(alpha [8,8], Gamma strictly lower [8,8], c, d, m, m - mhat, hd [3,8], gamma)."""
+    a, g = np.zeros(64), np.zeros(64)
+    c, d, m, e = np.zeros(8), np.zeros(8), np.zeros(8), np.zeros(8)
+    hd = np.zeros(24)
+    gam = np.zeros(1)
+    _check(lib().gso_rodas5p_tableau(_ptr(a), _ptr(g), _ptr(c), _ptr(d), _ptr(m), _ptr(e),
+                                     _ptr(hd), _ptr(gam)), "rodas5p_tableau")
+    return a.reshape(8, 8), g.reshape(8, 8), c, d, m, e, hd.reshape(3, 8), float(gam[0])
+
+
+def rodas5p_replay(field, y0, params, t0, tape, *, aux=None, nthreads=None):
+    return _replay(lib().gso_rodas5p_replay, field, y0, params, t0, tape, aux, nthreads)
+
+
+def rodas5p_adjoint(field, y0, params, t0, tape, ybar, *, aux=None, nthreads=None):
+    return _adjoint(lib().gso_rodas5p_adjoint, field, y0, params, t0, tape, ybar, aux, nthreads)
+
+
+def rodas5p_replay_saveat(field, y0, params, t0, tape, ts, *, dense=False, aux=None,
+                          nthreads=None):
+    fn = lib().gso_rodas5p_replay_saveat_dense if dense else lib().gso_rodas5p_replay_saveat
+    return _replay_saveat(fn, field, y0, params, t0, tape, ts, aux, nthreads)
+
+
+def rodas5p_adjoint_saveat(field, y0, params, t0, tape, ts, ysbar, yfbar=None, *, dense=False,
+                           aux=None, nthreads=None):
+    fn = lib().gso_rodas5p_adjoint_saveat_dense if dense else lib().gso_rodas5p_adjoint_saveat
+    return _adjoint_saveat(fn, field, y0, params, t0, tape, ts, ysbar, yfbar, aux, nthreads)
 
 
 def tsit5_replay(field, y0, params, t0, tape, *, aux=None, nthreads=None):
