@@ -90,6 +90,22 @@ SIGNATURES = {
     "gsx_rb23_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                      _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _i, _vp]),
+    "gsx_rodas5p_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _d, _d, _i64, _vp,
+                             _vp, _vp, _vp, _vp, _i64, _vp, _i32, _i, _vp]),
+    "gsx_rodas5p_replay": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i32,
+                                _i, _vp]),
+    "gsx_rodas5p_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp, _i32,
+                                 _vp, _vp, _vp, _i, _vp]),
+    "gsx_rodas5p_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i64,
+                                _vp, _vp, _vp, _vp, _i, _vp]),
+    "gsx_rodas5p_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp,
+                                        _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _i, _vp]),
+    "gsx_rodas5p_saveat_dense": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp,
+                                      _i64, _vp, _vp, _vp, _i, _vp]),
+    "gsx_rodas5p_adjoint_saveat_dense": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64,
+                                              _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
+                                              _vp, _vp, _i, _vp]),
     "gsx_field_register": (_i, [ctypes.c_char_p, _i, _i, _i, ctypes.POINTER(_i)]),
     "gsx_field_release": (_i, [_i]),
 }

@@ -5,9 +5,9 @@ Differences from the reference, all reported rather than hidden:
 
 * engines run on the GPU (``device`` "cuda"/"gpu"); ``device="cpu"`` raises, because this
   build has no CPU engine;
-* the reference's general-RHS engines (``tsit5_replay``, ``rodas5p_replay``, ``vern7_replay``,
-  ``fixed_step_*``, ``diffrax``) are not part of this build: a request that needs one raises
-  ``ValueError`` naming it;
+* the reference's Rodas5P engine (``rodas5p_replay``) is the native ``hip_rodas5p`` lane; its
+  other general engines (``tsit5_replay``, ``vern7_replay``, ``fixed_step_*``, ``diffrax``)
+  are not part of this build: a request that needs one raises ``ValueError`` naming it;
 * closures are differentiated with torch.autograd (a ``torch.autograd.Function`` whose
   backward is the native adjoint kernel) -- JAX is not installed here.
 """
@@ -23,8 +23,9 @@ from . import fields as F
 from .base import Route, SolveResult
 
 _VALID_WRT = "'params', 'y0', or ('y0', 'params')"
-_NOT_BUILT = {"tsit5_replay", "rodas5p_replay", "vern7_replay", "fixed_step_tsit5",
-              "fixed_step_imex", "diffrax", "fused_rosenbrock_backward"}
+_NOT_BUILT = {"tsit5_replay", "vern7_replay", "fixed_step_tsit5", "fixed_step_imex", "diffrax",
+              "fused_rosenbrock_backward"}
+_RODAS5P_NAMES = ("rodas5p_replay", "hip_rodas5p")
 
 
 @dataclass(frozen=True)
@@ -122,6 +123,29 @@ def _solve_rb23(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult
     return R.solve(problem, y0, params, rtol=rtol, atol=atol, device=device)
 
 
+def _supports_rodas5p(problem) -> bool:
+    """A field with an analytic Jacobian (a stiff binding), autonomous."""
+    b = F.binding_for(problem)
+    return b is not None and b.stiff and not getattr(b, "time_dependent", False)
+
+
+def _solve_rodas5p(problem, y0, params, *, rtol, atol, device, strict=False,
+                   **kw) -> SolveResult:
+    from . import hip_rodas5p as R5
+    return R5.solve(problem, y0, params, rtol=rtol, atol=atol, device=device, strict=strict)
+
+
+def _reject_incapable_saveat_dense(name: str) -> None:
+    """saveat_dense needs the method's published continuous extension: only the Rodas5P lane
+    has one (gradsolve/api.py:425-435)."""
+    if name != D.HIP_RODAS5P:
+        raise ValueError(
+            f"saveat_dense=True is implemented only on the 'rodas5p_replay' lane, which has a "
+            f"published continuous extension; engine {name!r} has none, and a generic "
+            "interpolant is not offered as a substitute. Use engine='rodas5p_replay', or drop "
+            "saveat_dense for the re-step path.")
+
+
 def _is_kuramoto(b) -> bool:
     from . import _native as N
     return b is not None and b.field_id == N.FIELD_KURAMOTO
@@ -155,6 +179,7 @@ ENGINE_REGISTRY = {
     D.HIP_ROSENBROCK23: EngineSpec(D.HIP_ROSENBROCK23, _supports_rb23, _solve_rb23, "rb23"),
     D.HIP_KURAMOTO: EngineSpec(D.HIP_KURAMOTO, _supports_kuramoto, _solve_kuramoto,
                                "kuramoto"),
+    D.HIP_RODAS5P: EngineSpec(D.HIP_RODAS5P, _supports_rodas5p, _solve_rodas5p, "rodas5p"),
 }
 
 
@@ -163,6 +188,13 @@ def _resolve(engine, problem, need_grad, batch_n, accuracy_target, pre=None):
     if engine == "auto":
         name = D.choose_engine(problem.dim, problem.is_stiff, need_grad, batch_n=batch_n,
                                accuracy_target=accuracy_target)
+        if name == D.RODAS5P_REPLAY:
+            name = D.HIP_RODAS5P
+            reasons.append("alias:rodas5p_replay")
+        elif name == D.DIFFRAX and problem.is_stiff:
+            # the reference substitutes its Rodas5P lane when diffrax is not installed
+            name = D.HIP_RODAS5P
+            reasons.append("diffrax-not-installed")
     elif engine in ENGINE_REGISTRY:
         name = engine
     elif engine in D.ALIASES:
@@ -195,7 +227,7 @@ def _resolve(engine, problem, need_grad, batch_n, accuracy_target, pre=None):
     return name, reasons
 
 
-def _translate(problem, params, fused, reasons) -> None:
+def _translate(problem, params, fused, reasons, engine="auto") -> None:
     """``fused='auto'|True``: a problem with no registered field whose ``f_jax`` traces gets
     one compiled on first use, as the reference translates on first use
     (gradsolve/api.py:291-400 with gradsolve/warp/jax_field.py). ``fused=True`` re-raises an
@@ -209,7 +241,7 @@ def _translate(problem, params, fused, reasons) -> None:
     n_params = int(np.asarray(params).shape[1]) if np.ndim(params) == 2 else 0
     try:
         F.register_jax_field(problem.name, f, int(problem.dim), n_params,
-                             stiff=bool(problem.is_stiff))
+                             stiff=bool(problem.is_stiff) or engine in _RODAS5P_NAMES)
         reasons.append("fused: f_jax compiled into the native kernels")
     except F.UnsupportedRHS as exc:
         if fused is True:
@@ -229,16 +261,17 @@ def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
     """
     if fused not in ("auto", True, False):
         raise ValueError(f"fused must be True, False, or 'auto'; got {fused!r}")
-    if saveat_dense:
-        raise ValueError("saveat_dense is implemented only on the reference's rodas5p_replay "
-                         "lane, which this build does not include")
+    if saveat_dense and saveat is None:
+        raise ValueError("saveat_dense=True requires saveat=...; it is a saveat option.")
     if precision not in ("float64", "float32"):
         raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
     _check_shapes(problem, y0, params)
     pre = []
-    _translate(problem, params, fused, pre)
+    _translate(problem, params, fused, pre, engine)
     name, reasons = _resolve(engine, problem, need_grad=saveat is not None, batch_n=batch_n,
                              accuracy_target=accuracy_target, pre=pre)
+    if saveat_dense:
+        _reject_incapable_saveat_dense(name)
     b = F.binding_for(problem)
     if not _native_ok(b, problem, params):
         raise ValueError(f"params shape {tuple(params.shape)} does not match the field "
@@ -246,7 +279,7 @@ def solve(problem, y0, params, *, saveat=None, saveat_dense: bool = False,
     if saveat is not None:
         from . import dense
         res = dense.solve_saveat(name, b, problem, y0, params, saveat, rtol=rtol, atol=atol,
-                                 device=device, precision=precision)
+                                 device=device, precision=precision, dense=saveat_dense)
     elif precision == "float32":
         from . import fp32
         res = fp32.solve(b, problem, y0, params, rtol=rtol, atol=atol, device=device,
@@ -279,18 +312,21 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
     wrt_t = _normalize_wrt(wrt)
     if fused not in ("auto", True, False):
         raise ValueError(f"fused must be True, False, or 'auto'; got {fused!r}")
-    if saveat_dense:
-        raise ValueError("saveat_dense=True requires the reference's rodas5p_replay lane, "
-                         "which this build does not include")
+    if saveat_dense and saveat is None:
+        raise ValueError(
+            "dense=True is a saveat option (it replaces the per-save re-step with the step's own "
+            "continuous extension); pass saveat=..., or drop dense.")
     if precision not in ("float64", "float32"):
         raise ValueError(f"unknown precision {precision!r}; known: ['float64', 'float32']")
     _check_shapes(problem, y0, params)
     pre = []
-    _translate(problem, params, fused, pre)
+    _translate(problem, params, fused, pre, engine)
     name, reasons = _resolve(engine, problem, need_grad=True, batch_n=batch_n,
                              accuracy_target=accuracy_target, pre=pre)
     if name == D.HIP_TSIT5:
         name = D.HIP_REPLAY
+    if saveat_dense:
+        _reject_incapable_saveat_dense(name)
     b = F.binding_for(problem)
     if precision == "float32":
         from . import fp32
@@ -310,7 +346,7 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         closure = dense.make_saveat_closure(name, b, problem, y0, params, saveat, wrt=wrt_t,
                                             rtol=rtol, atol=atol, device=dev, aux=aux,
                                             precision=precision, ckpt_interval=ckpt_interval,
-                                            strict=strict)
+                                            strict=strict, dense=saveat_dense)
     elif name == D.HIP_KURAMOTO and precision == "float64":
         from .closure import KuramotoClosure
         closure = KuramotoClosure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
@@ -325,6 +361,11 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         closure = make_tsit5_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
                                      device=dev, ckpt_interval=ckpt_interval, strict=strict,
                                      aux=aux)
+    elif name == D.HIP_RODAS5P:
+        from . import hip_rodas5p as R5
+        closure = R5.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
+                                  device=dev, ckpt_interval=ckpt_interval, strict=strict,
+                                  aux=aux)
     else:
         from . import hip_rosenbrock23 as R
         closure = R.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,

@@ -70,6 +70,15 @@ class _Rb23Lane(_Lane):
     METHOD = "rb23"
 
 
+class _Rodas5PLane(_Lane):
+    FWD, REPLAY, ADJ = "gsx_rodas5p_fwd", "gsx_rodas5p_replay", "gsx_rodas5p_adjoint"
+
+    def __init__(self, binding, aux, strict, dense=False):
+        super().__init__(binding, aux, strict)
+        # saveat by the continuous extension (saveat_dense) or by re-stepping
+        self.METHOD = "rodas5p_dense" if dense else "rodas5p"
+
+
 class _ReplayFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, z, p_user, state):
@@ -282,6 +291,26 @@ def make_tsit5_saveat_closure(binding, problem, y0, params, *, wrt, rtol, atol, 
                               ckpt_interval=None, strict=False, aux=None,
                               max_steps=100000) -> Tsit5SaveatClosure:
     lane = _Tsit5Lane(binding, aux, strict)
+    return Tsit5SaveatClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol,
+                              atol=atol, device=device, ts=ts, ckpt_interval=ckpt_interval,
+                              max_steps=max_steps)
+
+
+def make_rodas5p_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
+                         ckpt_interval=None, strict=False, aux=None,
+                         max_steps=50000) -> ReplayClosure:
+    lane = _Rodas5PLane(binding, aux, strict)
+    return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
+                         device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
+
+
+def make_rodas5p_saveat_closure(binding, problem, y0, params, *, wrt, rtol, atol, device, ts,
+                                dense=False, ckpt_interval=None, strict=False, aux=None,
+                                max_steps=50000) -> "Tsit5SaveatClosure":
+    """The Rodas5P series closure: saves by re-stepping, or with ``dense`` by the method's
+    order-4 continuous extension (the reference's ``saveat_dense=True``,
+    gradsolve/solvers/rodas5p_replay.py:179-228)."""
+    lane = _Rodas5PLane(binding, aux, strict, dense=dense)
     return Tsit5SaveatClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol,
                               atol=atol, device=device, ts=ts, ckpt_interval=ckpt_interval,
                               max_steps=max_steps)

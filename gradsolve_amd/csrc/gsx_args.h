@@ -62,6 +62,12 @@ struct AdjArgs {
   const int32_t* save_step = nullptr;  // [nsave][n]
   const double* mu = nullptr;          // [nsave][D][n]
   const double* pbar_in = nullptr;     // [P][n] or nullptr (start from zero)
+  // Dense-output cotangents (the Rodas5P continuous extension): with ysbar set, save q
+  // bracketed by step s = save_step[q] contributes at that step through its interpolant at
+  // theta = (ts[q] - save_t[q]) / h_s; ysbar [nsave][D][n].
+  const double* ts = nullptr;
+  const double* save_t = nullptr;
+  const double* ysbar = nullptr;
 };
 
 // Dense output on a recorded mesh (gradsolve/solvers/dense.py:106-163): for every save time

@@ -227,7 +227,7 @@ namespace {
 
 // Dense output and its gradient for one of the two one-step methods; the Tsit5 and
 // Rosenbrock23 entry points below differ only in the method and the fields they accept.
-enum class DenseMethod { kTsit5, kRb23 };
+enum class DenseMethod { kTsit5, kRb23, kRodas5p };
 
 int saveat_common(DenseMethod m, const char* what, int field, int64_t n, int D, int P,
                   const double* aux, const double* y0, const double* p, double t0,
@@ -252,6 +252,9 @@ int saveat_common(DenseMethod m, const char* what, int field, int64_t n, int D, 
   if (m == DenseMethod::kRb23)
     return strict ? gsx_strict::rb23_saveat_dispatch(field, D, a, st)
                   : rb23_saveat_dispatch(field, D, a, st);
+  if (m == DenseMethod::kRodas5p)
+    return strict ? gsx_strict::r5_saveat_dispatch(field, D, a, false, st)
+                  : r5_saveat_dispatch(field, D, a, false, st);
   return strict ? gsx_strict::tsit5_saveat_dispatch(field, D, a, st)
                 : tsit5_saveat_dispatch(field, D, a, st);
 }
@@ -292,6 +295,9 @@ int adjoint_saveat_common(DenseMethod m, const char* what, int field, int64_t n,
     if (rb)
       rc = strict ? gsx_strict::rb23_saveat_vjp_dispatch(field, D, v, st)
                   : rb23_saveat_vjp_dispatch(field, D, v, st);
+    else if (m == DenseMethod::kRodas5p)
+      rc = strict ? gsx_strict::r5_saveat_vjp_dispatch(field, D, v, st)
+                  : r5_saveat_vjp_dispatch(field, D, v, st);
     else
       rc = strict ? gsx_strict::tsit5_saveat_vjp_dispatch(field, D, v, st)
                   : tsit5_saveat_vjp_dispatch(field, D, v, st);
@@ -311,6 +317,8 @@ int adjoint_saveat_common(DenseMethod m, const char* what, int field, int64_t n,
   if (rb)
     return strict ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
                   : rb23_adj_dispatch(field, D, a, st);
+  if (m == DenseMethod::kRodas5p)
+    return strict ? gsx_strict::r5_adj_dispatch(field, D, a, st) : r5_adj_dispatch(field, D, a, st);
   return strict ? gsx_strict::tsit5_adj_dispatch(field, D, a, st)
                 : tsit5_adj_dispatch(field, D, a, st);
 }
@@ -428,6 +436,164 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, cons
   hipStream_t st = static_cast<hipStream_t>(stream);
   return (flags & GSX_STRICT) ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
                               : rb23_adj_dispatch(field, D, a, st);
+}
+
+// ---------------------------------------------------------------------------------------
+// Rodas5P (gsx_rodas5p.hip): fields with an analytic Jacobian (the built-in stiff fields and
+// fields registered with GSX_FIELD_STIFF), autonomous.
+// ---------------------------------------------------------------------------------------
+static int r5_check(int field, int D, int P, const char* what) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (!fi.autonomous) return set_error(GSX_E_UNSUPPORTED, "%s: Rodas5P is built for autonomous fields", what);
+  if (::gsx::rt::is_runtime(field) && !fi.stiff)
+    return set_error(GSX_E_INVALID, "%s: field %d has no Jacobian (register it with "
+                     "GSX_FIELD_STIFF)", what, field);
+  return GSX_OK;
+}
+
+int gsx_rodas5p_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                    const double* p, double t0, double t1, const double* t1_per, double rtol,
+                    double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
+                    int32_t* n_rej, int32_t* status, double* tape, int64_t tape_cap,
+                    double* ckpt, int32_t ckpt_interval, int flags, void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_fwd");
+  if (rc) return rc;
+  if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
+  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+    return set_error(GSX_E_INVALID, "rodas5p_fwd: null input/output pointer");
+  if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (ckpt && !tape) return set_error(GSX_E_INVALID, "checkpoints need a tape");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  FwdArgs a{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::r5_fwd_dispatch(field, D, a, st)
+                              : r5_fwd_dispatch(field, D, a, st);
+}
+
+int gsx_rodas5p_replay(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                       const double* p, double t0, const double* tape, const int32_t* n_acc,
+                       double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                       void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_replay");
+  if (rc) return rc;
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!y0 || !p || !tape || !n_acc || !y_final))
+    return set_error(GSX_E_INVALID, "rodas5p_replay: null pointer");
+  if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  ReplayArgs a{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::r5_replay_dispatch(field, D, a, st)
+                              : r5_replay_dispatch(field, D, a, st);
+}
+
+int gsx_rodas5p_adjoint(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                        const double* p, double t0, const double* tape, int64_t tape_cap,
+                        const int32_t* n_acc, const double* ckpt, int32_t ckpt_interval,
+                        const double* ybar, double* y0bar, double* pbar, int flags,
+                        void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_adjoint");
+  if (rc) return rc;
+  if (n < 0) return set_error(GSX_E_INVALID, "negative n");
+  if (n > 0 && (!p || !tape || !n_acc || !ckpt || !ybar || !y0bar || !pbar))
+    return set_error(GSX_E_INVALID, "rodas5p_adjoint: null pointer");
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::r5_adj_dispatch(field, D, a, st)
+                              : r5_adj_dispatch(field, D, a, st);
+}
+
+int gsx_rodas5p_saveat(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                       const double* p, double t0, const double* tape, const int32_t* n_acc,
+                       const double* y_final, const double* ts, int64_t nsave, double* ys,
+                       int32_t* save_step, double* save_t, double* states, int flags,
+                       void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_saveat");
+  if (rc) return rc;
+  return saveat_common(DenseMethod::kRodas5p, "rodas5p_saveat", field, n, D, P, aux, y0, p, t0,
+                       tape, n_acc, y_final, ts, nsave, ys, save_step, save_t, states, flags,
+                       stream);
+}
+
+int gsx_rodas5p_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                               const double* y0, const double* p, double t0,
+                               const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                               const double* ckpt, int32_t ckpt_interval, const double* ts,
+                               int64_t nsave, const int32_t* save_step, const double* save_t,
+                               const double* states, const double* ysbar, const double* ybar,
+                               double* mu, double* y0bar, double* pbar, int flags,
+                               void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_adjoint_saveat");
+  if (rc) return rc;
+  return adjoint_saveat_common(DenseMethod::kRodas5p, "rodas5p_adjoint_saveat", field, n, D, P,
+                               aux, y0, p, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ts,
+                               nsave, save_step, save_t, states, ysbar, ybar, mu, y0bar, pbar,
+                               flags, stream);
+}
+
+int gsx_rodas5p_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                             const double* y0, const double* p, double t0, const double* tape,
+                             const int32_t* n_acc, const double* y_final, const double* ts,
+                             int64_t nsave, double* ys, int32_t* save_step, double* save_t,
+                             int flags, void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_saveat_dense");
+  if (rc) return rc;
+  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (n == 0 || nsave == 0) return GSX_OK;
+  if (!y0 || !p || !tape || !n_acc || !ts || !ys || !save_step || !save_t)
+    return set_error(GSX_E_INVALID, "rodas5p_saveat_dense: null pointer");
+  SaveArgs a{n, y0, p, aux, t0, tape, n_acc, ts, nsave, ys, save_step, save_t, nullptr, y_final};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::r5_saveat_dispatch(field, D, a, true, st)
+                              : r5_saveat_dispatch(field, D, a, true, st);
+}
+
+int gsx_rodas5p_adjoint_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t tape_cap,
+                                     const int32_t* n_acc, const double* ckpt,
+                                     int32_t ckpt_interval, const double* ts, int64_t nsave,
+                                     const int32_t* save_step, const double* save_t,
+                                     const double* ysbar, const double* ybar, double* y0bar,
+                                     double* pbar, int flags, void* stream) {
+  int rc = r5_check(field, D, P, "rodas5p_adjoint_saveat_dense");
+  if (rc) return rc;
+  if (n < 0 || nsave < 0) return set_error(GSX_E_INVALID, "negative n or nsave");
+  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
+                     kMaxCkptInterval);
+  if (n == 0) return GSX_OK;
+  if (!p || !tape || !n_acc || !ckpt || !y0bar || !pbar ||
+      (nsave > 0 && (!ts || !save_step || !save_t || !ysbar)))
+    return set_error(GSX_E_INVALID, "rodas5p_adjoint_saveat_dense: null pointer");
+  if (nsave == 0 && !ybar)
+    return set_error(GSX_E_INVALID, "rodas5p_adjoint_saveat_dense: no cotangent");
+  AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
+  if (nsave > 0) {
+    a.nsave = nsave;
+    a.save_step = save_step;
+    a.ts = ts;
+    a.save_t = save_t;
+    a.ysbar = ysbar;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return (flags & GSX_STRICT) ? gsx_strict::r5_adj_dispatch(field, D, a, st)
+                              : r5_adj_dispatch(field, D, a, st);
 }
 
 int64_t gsx_persist_lanes(int waves_per_simd) {

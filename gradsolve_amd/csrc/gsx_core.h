@@ -32,6 +32,7 @@ using ::gsx::FwdArgs;
 using ::gsx::ReplayArgs;
 using ::gsx::SaveArgs;
 using ::gsx::SaveVjpArgs;
+using ::gsx::kDegenerate;
 using ::gsx::kExhausted;
 using ::gsx::kReached;
 using ::gsx::kTapeFull;

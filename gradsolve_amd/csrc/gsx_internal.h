@@ -34,6 +34,11 @@ int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
 int rb23_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s);
 int rb23_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s);
+int r5_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
+int r5_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
+int r5_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int r5_saveat_dispatch(int field, int D, const SaveArgs& a, bool dense, hipStream_t s);
+int r5_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s);
 int tsit5_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
@@ -48,7 +53,8 @@ int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 // Fields registered at run time (gsx_runtime.hip): ids from GSX_FIELD_RUNTIME_BASE up,
 // kernels compiled with hiprtc per family and launched with the built-in argument blocks.
 namespace rt {
-enum Family { kTsit5, kTsit5Strict, kTsit5F32, kRb23, kRb23Strict, kNumFamilies };
+enum Family { kTsit5, kTsit5Strict, kTsit5F32, kRb23, kRb23Strict, kRodas5p, kRodas5pStrict,
+              kNumFamilies };
 bool is_runtime(int field);
 bool info(int field, FieldInfo* out);
 int tsit5_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
@@ -60,6 +66,15 @@ int rb23_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
 int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s);
 int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
 int rb23_saveat(int field, Family fam, const SaveArgs& a, hipStream_t s);
+int r5_launch(int field, Family fam, const char* key, int64_t n, const FwdArgs& a,
+              hipStream_t s);
+int r5_launch(int field, Family fam, const char* key, int64_t n, const ReplayArgs& a,
+              hipStream_t s);
+int r5_launch(int field, Family fam, const char* key, int64_t n, const SaveArgs& a,
+              hipStream_t s);
+int r5_launch(int field, Family fam, const char* key, int64_t n, const SaveVjpArgs& a,
+              hipStream_t s);
+int r5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
 int rb23_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s);
 }  // namespace rt
 
@@ -76,6 +91,11 @@ int rb23_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream
 int rb23_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
 int rb23_saveat_dispatch(int field, int D, const ::gsx::SaveArgs& a, hipStream_t s);
 int rb23_saveat_vjp_dispatch(int field, int D, const ::gsx::SaveVjpArgs& a, hipStream_t s);
+int r5_fwd_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
+int r5_replay_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
+int r5_adj_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);
+int r5_saveat_dispatch(int field, int D, const ::gsx::SaveArgs& a, bool dense, hipStream_t s);
+int r5_saveat_vjp_dispatch(int field, int D, const ::gsx::SaveVjpArgs& a, hipStream_t s);
 int tsit5_fwd_f32_dispatch(int field, int D, const ::gsx::FwdArgs& a, hipStream_t s);
 int tsit5_replay_f32_dispatch(int field, int D, const ::gsx::ReplayArgs& a, hipStream_t s);
 int tsit5_adj_f32_dispatch(int field, int D, const ::gsx::AdjArgs& a, hipStream_t s);

@@ -56,3 +56,6 @@ using ::tanh;
 #define GSX_HD inline
 #define GSX_UNROLL
 #endif
+
+// Finite test usable in all three compilations (hiprtc's std has no isfinite).
+GSX_HD bool gsx_finite(double x) { return __builtin_isfinite(x); }

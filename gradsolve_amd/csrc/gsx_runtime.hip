@@ -81,13 +81,28 @@ const FamilySpec& spec(Family f) {
         {"adj8", "rb23_adj_kernel<%s, 8>"}, {"adj16", "rb23_adj_kernel<%s, 16>"},
         {"adj32", "rb23_adj_kernel<%s, 32>"}, {"saveat", "rb23_saveat_kernel<%s>"},
         {"saveat_vjp", "rb23_saveat_vjp_kernel<%s>"}}},
+      {"gsx", "gsx_rodas5p_kernels.h", {},
+       {{"fwd", "r5_fwd_kernel<%s>"}, {"replay", "r5_replay_kernel<%s>"},
+        {"adj1", "r5_adj_kernel<%s, 1>"}, {"adj4", "r5_adj_kernel<%s, 4>"},
+        {"adj8", "r5_adj_kernel<%s, 8>"}, {"adj16", "r5_adj_kernel<%s, 16>"},
+        {"adj32", "r5_adj_kernel<%s, 32>"}, {"saveat", "r5_saveat_kernel<%s>"},
+        {"saveat_vjp", "r5_saveat_vjp_kernel<%s>"},
+        {"saveat_dense", "r5_saveat_dense_kernel<%s>"}}},
+      {"gsx_strict", "gsx_rodas5p_kernels.h", kStrict,
+       {{"fwd", "r5_fwd_kernel<%s>"}, {"replay", "r5_replay_kernel<%s>"},
+        {"adj1", "r5_adj_kernel<%s, 1>"}, {"adj4", "r5_adj_kernel<%s, 4>"},
+        {"adj8", "r5_adj_kernel<%s, 8>"}, {"adj16", "r5_adj_kernel<%s, 16>"},
+        {"adj32", "r5_adj_kernel<%s, 32>"}, {"saveat", "r5_saveat_kernel<%s>"},
+        {"saveat_vjp", "r5_saveat_vjp_kernel<%s>"},
+        {"saveat_dense", "r5_saveat_dense_kernel<%s>"}}},
   };
   return specs[f];
 }
 
 const char* family_name(Family f) {
   static const char* names[kNumFamilies] = {"tsit5", "tsit5 (strict)", "tsit5 (FP32)",
-                                            "rosenbrock23", "rosenbrock23 (strict)"};
+                                            "rosenbrock23", "rosenbrock23 (strict)",
+                                            "rodas5p", "rodas5p (strict)"};
   return names[f];
 }
 
@@ -336,6 +351,34 @@ int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s) {
 }
 
 int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
+  const char* key = adj_key(a.ckpt_interval, false);
+  if (!key)
+    return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d",
+                     a.ckpt_interval, kMaxCkptInterval);
+  return launch(field, fam, key, a.n, kAdjBlock, a, s);
+}
+
+int r5_launch(int field, Family fam, const char* key, int64_t n, const FwdArgs& a,
+              hipStream_t s) {
+  return launch(field, fam, key, n, kFwdBlock, a, s);
+}
+
+int r5_launch(int field, Family fam, const char* key, int64_t n, const ReplayArgs& a,
+              hipStream_t s) {
+  return launch(field, fam, key, n, kFwdBlock, a, s);
+}
+
+int r5_launch(int field, Family fam, const char* key, int64_t n, const SaveArgs& a,
+              hipStream_t s) {
+  return launch(field, fam, key, n, kFwdBlock, a, s);
+}
+
+int r5_launch(int field, Family fam, const char* key, int64_t n, const SaveVjpArgs& a,
+              hipStream_t s) {
+  return launch(field, fam, key, n, kFwdBlock, a, s);
+}
+
+int r5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
   const char* key = adj_key(a.ckpt_interval, false);
   if (!key)
     return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d",

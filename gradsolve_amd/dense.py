@@ -13,8 +13,10 @@ benchmarked on), on the Tsit5 lanes of every other non-stiff field, built-in or 
 run time (``gsx_tsit5_saveat`` / ``gsx_tsit5_adjoint_saveat``), and on the stiff Rosenbrock23
 lane, whose re-step is one ``rosenbrock23_step`` (``gsx_rb23_saveat`` /
 ``gsx_rb23_adjoint_saveat``; the reference's ``replay_solve_rosenbrock_saveat_jax``,
-gradsolve/warp/warp_replay.py:78-84). FP32 dense output outside the Kuramoto engine raises
-``ValueError`` naming what is built.
+gradsolve/warp/warp_replay.py:78-84), and on the Rodas5P lane, by re-stepping or, with
+``saveat_dense=True``, by the method's order-4 continuous extension (``gsx_rodas5p_saveat``,
+``gsx_rodas5p_saveat_dense`` and their adjoints; gradsolve/solvers/dense.py:166-227). FP32
+dense output outside the Kuramoto engine raises ``ValueError`` naming what is built.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ from . import kuramoto as KU
 from .base import SolveResult
 
 
-def _check(name, b, precision) -> str:
+def _check(name, b, precision, dense=False) -> str:
     if b.field_id == N.FIELD_KURAMOTO and name == D.HIP_KURAMOTO:
         return "kuramoto"
     if name in (D.HIP_TSIT5, D.HIP_REPLAY) and not b.stiff:
@@ -36,6 +38,11 @@ def _check(name, b, precision) -> str:
                 f"saveat with precision='float32' is built for the Kuramoto engine only; field "
                 f"{b.key!r} serves dense output in float64")
         return "tsit5"
+    if name == D.HIP_RODAS5P:
+        if precision == "float32":
+            raise ValueError("rodas5p_replay is float64 throughout; precision='float32' is "
+                             "not supported on it")
+        return "rodas5p_dense" if dense else "rodas5p"
     if name == D.HIP_ROSENBROCK23:
         if precision == "float32":
             raise ValueError(
@@ -44,12 +51,15 @@ def _check(name, b, precision) -> str:
         return "rb23"
     raise ValueError(
         f"saveat is not built for field {b.key!r} on engine {name!r}: this build serves "
-        "dense output on the Tsit5 lanes (hip_tsit5 / hip_replay), the Rosenbrock23 lane "
-        "(hip_rosenbrock23) and the Kuramoto engine (hip_kuramoto). Drop saveat for a "
-        "final-state solve or gradient.")
+        "dense output on the Tsit5 lanes (hip_tsit5 / hip_replay), the Rosenbrock23 and "
+        "Rodas5P lanes (hip_rosenbrock23, hip_rodas5p) and the Kuramoto engine (hip_kuramoto). "
+        "Drop saveat for a final-state solve or gradient.")
 
 
-_FWD_ENTRY = {"tsit5": "gsx_tsit5_fwd", "rb23": "gsx_rb23_fwd"}
+_FWD_ENTRY = {"tsit5": "gsx_tsit5_fwd", "rb23": "gsx_rb23_fwd", "rodas5p": "gsx_rodas5p_fwd",
+              "rodas5p_dense": "gsx_rodas5p_fwd"}
+_SOLVER = {"tsit5": D.HIP_TSIT5, "rb23": D.HIP_ROSENBROCK23, "rodas5p": D.HIP_RODAS5P,
+           "rodas5p_dense": D.HIP_RODAS5P}
 
 
 def _solve_saveat_lane(lane, b, problem, y0, params, ts, *, rtol, atol, dev, max_steps):
@@ -68,19 +78,20 @@ def _solve_saveat_lane(lane, b, problem, y0, params, ts, *, rtol, atol, dev, max
     y_saved[status != 0] = np.nan
     return SolveResult(y_final=y_final, accepted_steps=rec.n_acc.cpu().numpy().astype(np.int64),
                        rejected_steps=rec.n_rej.cpu().numpy().astype(np.int64),
-                       solver=D.HIP_ROSENBROCK23 if lane == "rb23" else D.HIP_TSIT5,
-                       y_saved=y_saved, ts_saved=ts)
+                       solver=_SOLVER[lane], y_saved=y_saved, ts_saved=ts)
 
 
 def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
-                 precision="float64", max_steps=100000) -> SolveResult:
+                 precision="float64", max_steps=100000, dense=False) -> SolveResult:
     from .api import _device
-    lane = _check(name, b, precision)
+    lane = _check(name, b, precision, dense)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
     dev = _device(device)
-    if lane in ("tsit5", "rb23"):
+    if lane in ("tsit5", "rb23", "rodas5p", "rodas5p_dense"):
         if lane == "rb23":
             from .hip_rosenbrock23 import MAX_STEPS as max_steps  # record_rosenbrock's 50000
+        elif lane.startswith("rodas5p"):
+            from .hip_rodas5p import MAX_STEPS as max_steps  # record_rodas5p's 50000
         return _solve_saveat_lane(lane, b, problem, y0, params, ts, rtol=rtol, atol=atol,
                                   dev=dev, max_steps=max_steps)
     y0_soa = torch.as_tensor(np.asarray(y0, np.float64), device=dev).t().contiguous()
@@ -102,10 +113,18 @@ def solve_saveat(name, b, problem, y0, params, saveat, *, rtol, atol, device,
 
 def make_saveat_closure(name, b, problem, y0, params, saveat, *, wrt, rtol, atol, device,
                         aux=None, precision="float64", max_steps=100000, ckpt_interval=None,
-                        strict=False):
-    from .closure import KuramotoClosure, make_rb23_saveat_closure, make_tsit5_saveat_closure
-    lane = _check(name, b, precision)
+                        strict=False, dense=False):
+    from .closure import (KuramotoClosure, make_rb23_saveat_closure,
+                          make_rodas5p_saveat_closure, make_tsit5_saveat_closure)
+    lane = _check(name, b, precision, dense)
     ts = KU.validate_saveat(saveat, problem.t0, problem.t1)
+    if lane.startswith("rodas5p"):
+        from .hip_rodas5p import MAX_STEPS
+        return make_rodas5p_saveat_closure(b, problem, y0, params, wrt=wrt, rtol=rtol,
+                                           atol=atol, device=device, ts=ts,
+                                           dense=lane == "rodas5p_dense",
+                                           ckpt_interval=ckpt_interval, strict=strict, aux=aux,
+                                           max_steps=MAX_STEPS)
     if lane == "rb23":
         from .hip_rosenbrock23 import MAX_STEPS
         return make_rb23_saveat_closure(b, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,

@@ -10,10 +10,13 @@ The codomain names this build's native engines:
                          ``warp_rosenbrock``/``cuda_rosenbrock23``);
 * ``hip_kuramoto``     -- Kuramoto-32 forward + record, saveat and adjoint, 16 trajectories
                          per wavefront with the coupling on MFMA (the reference runs this
-                         problem on its general ``tsit5_replay`` lane).
+                         problem on its general ``tsit5_replay`` lane);
+* ``hip_rodas5p``      -- Rodas5P forward + record, adjoint and dense output (replaces the
+                         reference's ``rodas5p_replay``, which the router names for stiff
+                         gradients above the low-NVAR band and which explicit opt-in reaches).
 
-High-dimensional problems route to the reference's general engines (``tsit5_replay``,
-``rodas5p_replay``, ``diffrax``), which are not part of this build: the API reports them
+High-dimensional non-stiff problems route to the reference's general engines
+(``tsit5_replay``, ``diffrax``), which are not part of this build: the API reports them
 instead of silently substituting.
 """
 from __future__ import annotations
@@ -36,6 +39,7 @@ HIP_ROSENBROCK23 = "hip_rosenbrock23"
 #: output: the API substitutes it for the router's Tsit5 engines when the problem's native
 #: field is the Kuramoto kernel, and records why on ``route.reason``.
 HIP_KURAMOTO = "hip_kuramoto"
+HIP_RODAS5P = "hip_rodas5p"
 TSIT5_REPLAY = "tsit5_replay"
 RODAS5P_REPLAY = "rodas5p_replay"
 DIFFRAX = "diffrax"
@@ -47,6 +51,7 @@ ALIASES = {
     "warp_replay": HIP_REPLAY,
     "warp_rosenbrock": HIP_ROSENBROCK23,
     "cuda_rosenbrock23": HIP_ROSENBROCK23,
+    "rodas5p_replay": HIP_RODAS5P,
 }
 
 

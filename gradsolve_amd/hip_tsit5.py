@@ -292,7 +292,10 @@ class SaveatResult:
 
 
 _DENSE_ENTRIES = {"tsit5": ("gsx_tsit5_saveat", "gsx_tsit5_adjoint_saveat"),
-                  "rb23": ("gsx_rb23_saveat", "gsx_rb23_adjoint_saveat")}
+                  "rb23": ("gsx_rb23_saveat", "gsx_rb23_adjoint_saveat"),
+                  "rodas5p": ("gsx_rodas5p_saveat", "gsx_rodas5p_adjoint_saveat"),
+                  "rodas5p_dense": ("gsx_rodas5p_saveat_dense",
+                                    "gsx_rodas5p_adjoint_saveat_dense")}
 
 
 def _dense_entry(method: str, which: int) -> str:
@@ -321,10 +324,15 @@ def saveat(rec: Tsit5Record, ts: torch.Tensor, *, y0_soa: Optional[torch.Tensor]
     st_t = torch.empty((k, rec.n), dtype=torch.float64, device=dev)
     states = torch.empty((k, rec.D, rec.n), dtype=torch.float64, device=dev) \
         if keep_states else None
-    N.call(_dense_entry(method, 0), rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux), _ptr(y0_soa),
-           _ptr(p_soa), rec.t0, _ptr(rec.tape), _ptr(rec.n_acc), _ptr(y_final), _ptr(ts), k,
-           _ptr(ys),
-           _ptr(step), _ptr(st_t), _ptr(states), flags_for(strict), _stream_handle(stream))
+    entry = _dense_entry(method, 0)
+    args = [rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux), _ptr(y0_soa), _ptr(p_soa), rec.t0,
+            _ptr(rec.tape), _ptr(rec.n_acc), _ptr(y_final), _ptr(ts), k, _ptr(ys), _ptr(step),
+            _ptr(st_t)]
+    if method == "rodas5p_dense":  # the interpolant needs no bracket states
+        states = None
+    else:
+        args.append(_ptr(states))
+    N.call(entry, *args, flags_for(strict), _stream_handle(stream))
     return SaveatResult(ys, step, st_t, states)
 
 
@@ -343,9 +351,16 @@ def adjoint_saveat(rec: Tsit5Record, ts: torch.Tensor, sv: SaveatResult,
     ysbar = ysbar.to(dtype=torch.float64).contiguous()
     if ybar is not None:
         ybar = ybar.to(dtype=torch.float64).contiguous()
-    mu = torch.empty((k, rec.D, rec.n), dtype=torch.float64, device=dev)
     y0bar = torch.empty((rec.D, rec.n), dtype=torch.float64, device=dev)
     pbar = torch.empty((rec.P, rec.n), dtype=torch.float64, device=dev)
+    if method == "rodas5p_dense":  # one reverse sweep, no re-step VJPs
+        N.call(_dense_entry(method, 1), rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux),
+               _ptr(rec.y0), _ptr(p_soa), rec.t0, _ptr(rec.tape), rec.cap, _ptr(rec.n_acc),
+               _ptr(ckpt), rec.ckpt_interval, _ptr(ts), k, _ptr(sv.save_step),
+               _ptr(sv.save_t), _ptr(ysbar), _ptr(ybar), _ptr(y0bar), _ptr(pbar),
+               flags_for(strict), _stream_handle(stream))
+        return y0bar, pbar
+    mu = torch.empty((k, rec.D, rec.n), dtype=torch.float64, device=dev)
     N.call(_dense_entry(method, 1), rec.field, rec.n, rec.D, rec.P, _ptr(rec.aux),
            _ptr(rec.y0), _ptr(p_soa), rec.t0, _ptr(rec.tape), rec.cap, _ptr(rec.n_acc),
            _ptr(ckpt), rec.ckpt_interval, _ptr(ts), k, _ptr(sv.save_step), _ptr(sv.save_t),

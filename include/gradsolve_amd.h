@@ -238,6 +238,80 @@ int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
                             const double* ybar, double* mu, double* y0bar, double* pbar,
                             int flags, void* stream);
 
+/* ---- Rodas5P: the stiff high-order lane ------------------------------------------------
+ * Steinebach's Rodas5P (order-5(4) Rosenbrock-Wanner pair, 8 stages, order-4 continuous
+ * extension) for autonomous fields with an analytic Jacobian (GSX_FIELD_ROBERTSON,
+ * GSX_FIELD_LINEAR_DECAY, and fields registered with GSX_FIELD_STIFF): one LU of
+ * W = I - h*gamma*J per step in registers, reused by the 8 stage solves. Same argument
+ * contracts as the Rosenbrock23 entry points. The forward records with the semantics of
+ * gradsolve's device recorder for Rodas5P (gradsolve/solvers/record_jax.py:83-132 with
+ * RODAS5P_METHOD): status 2 when a step falls below the underflow floor
+ * 4*eps*max(|t0|, |t1 - t0|), a non-finite trial shrinks h by 0.2 without a controller update,
+ * max_steps counts attempts. Replaces record_rodas5p / replay_rodas5p_jax and its jax.grad
+ * (gradsolve/solvers/rodas5p_replay.py:94-176, rodas5p_step.py:119-168), the rodas5p_replay
+ * lane of solve/grad_closure (gradsolve/api.py:509-530, 608-616, 683-691). */
+int gsx_rodas5p_fwd(int field, int64_t n, int D, int P, const double* aux,
+                    const double* y0, const double* p, double t0, double t1,
+                    const double* t1_per, double rtol, double atol, int64_t max_steps,
+                    double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                    double* tape, int64_t tape_cap, double* ckpt, int32_t ckpt_interval,
+                    int flags, void* stream);
+int gsx_rodas5p_replay(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0,
+                       const double* tape, const int32_t* n_acc,
+                       double* y_final, double* ckpt, int32_t ckpt_interval, int flags,
+                       void* stream);
+int gsx_rodas5p_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                        const double* y0, const double* p, double t0,
+                        const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                        const double* ckpt, int32_t ckpt_interval,
+                        const double* ybar, double* y0bar, double* pbar, int flags,
+                        void* stream);
+
+/* Rodas5P saveat by bracket-and-re-step (the default of the reference's rodas5p_replay lane,
+ * dense.vmap_saveat with rodas5p_advance) and its gradient; arguments as gsx_tsit5_saveat /
+ * gsx_tsit5_adjoint_saveat. */
+int gsx_rodas5p_saveat(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0,
+                       const double* tape, const int32_t* n_acc, const double* y_final,
+                       const double* ts, int64_t nsave, double* ys, int32_t* save_step,
+                       double* save_t, double* states, int flags, void* stream);
+int gsx_rodas5p_adjoint_saveat(int field, int64_t n, int D, int P, const double* aux,
+                               const double* y0, const double* p, double t0,
+                               const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                               const double* ckpt, int32_t ckpt_interval,
+                               const double* ts, int64_t nsave, const int32_t* save_step,
+                               const double* save_t, const double* states,
+                               const double* ysbar, const double* ybar, double* mu,
+                               double* y0bar, double* pbar, int flags, void* stream);
+
+/* Rodas5P dense output by its order-4 continuous extension (saveat_dense=True:
+ * gradsolve/solvers/dense.py:166-227, rodas5p_step.py:102-226): a save time in
+ * (t_s, t_s + h_s] is the interpolant (1-th) y_s + th (y_{s+1} + (1-th)(D1 + th (D2 + th D3)))
+ * at th = (ts - t_s)/h_s, D_r = sum_i hd[r][i] k_i of step s; at or past the final time it is
+ * y_final, at or before t0 it is y0 (save_step -1 / n_acc). The replay forms each step's
+ * stages once and no save takes an extra step. ys [nsave][D][n]; save_step / save_t as
+ * gsx_tsit5_saveat. */
+int gsx_rodas5p_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                             const double* y0, const double* p, double t0,
+                             const double* tape, const int32_t* n_acc, const double* y_final,
+                             const double* ts, int64_t nsave, double* ys, int32_t* save_step,
+                             double* save_t, int flags, void* stream);
+
+/* Gradient of a dense Rodas5P series: one reverse sweep over the segments, each bracketed
+ * save sending (1-th) of its cotangent to y_s, th to y_{s+1} and th (1-th) th^r to D_{r+1},
+ * which the step VJP carries to the stages. ysbar [nsave][D][n]; ybar (NULL: zero) dL/dy_final;
+ * save_step / save_t as gsx_rodas5p_saveat_dense wrote them; tape, ckpt as gsx_rodas5p_fwd
+ * wrote them. Replaces jax.grad through vmap_saveat_dense (gradsolve/solvers/dense.py:230-260). */
+int gsx_rodas5p_adjoint_saveat_dense(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t tape_cap,
+                                     const int32_t* n_acc, const double* ckpt,
+                                     int32_t ckpt_interval, const double* ts, int64_t nsave,
+                                     const int32_t* save_step, const double* save_t,
+                                     const double* ysbar, const double* ybar, double* y0bar,
+                                     double* pbar, int flags, void* stream);
+
 /* ---- Persistent work-queue form (ensembles with very uneven step counts) ----------------
  * No reference counterpart as a kernel: the reference integrates one trajectory per thread
  * with one t1 per ensemble (gradsolve/warp/warp_ode.py:292) and leaves a wave idle until its

@@ -7,6 +7,7 @@
 
 #include "../../gradsolve_amd/csrc/gsx_core.h"
 #include "../../gradsolve_amd/csrc/gsx_rb23.h"
+#include "../../gradsolve_amd/csrc/gsx_rodas5p.h"
 #include "../../gradsolve_amd/csrc/gsx_f32.h"
 
 using namespace gsx;
@@ -86,6 +87,12 @@ int run_rb_adj_saveat(const SaveVjpArgs& v, AdjArgs a) {
   a.mu = v.mu;
   a.pbar_in = v.pbar;
   return run_rb_adj<F>(a);
+}
+template <class F>
+int run_r5_adj(const AdjArgs& a) {
+  std::vector<double> seg(static_cast<size_t>(a.ckpt_interval) * ckpt_rows<F>());
+  for (int64_t j = 0; j < a.n; ++j) r5_adj_lane<F>(a, j, seg.data(), 1);
+  return 0;
 }
 }  // namespace
 
@@ -180,6 +187,62 @@ int lh_rb23_adjoint_saveat(int field, int64_t n, const double* p, double t0,
   if (field == 3) return run_rb_adj_saveat<FRobertson>(v, a);
   if (field == 2) return run_rb_adj_saveat<FLinearDecay<2>>(v, a);
   return -1;
+}
+int lh_r5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
+              double rtol, double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
+              int32_t* n_rej, int32_t* status, double* tape, int64_t cap, double* ckpt,
+              int32_t C) {
+  FwdArgs a{n, y0, p, nullptr, t0, t1, nullptr, rtol, atol, max_steps, y_final, n_acc, n_rej,
+            status, tape, tape ? cap : 0, ckpt, ckpt ? C : 1};
+  if (field == 3) { for (int64_t j = 0; j < n; ++j) r5_fwd_lane<FRobertson>(a, j); return 0; }
+  if (field == 2) { for (int64_t j = 0; j < n; ++j) r5_fwd_lane<FLinearDecay<1>>(a, j); return 0; }
+  return -1;
+}
+int lh_r5_replay(int field, int64_t n, const double* y0, const double* p, double t0,
+                 const double* tape, const int32_t* n_acc, double* y_final) {
+  ReplayArgs a{n, y0, p, nullptr, t0, tape, n_acc, y_final, nullptr, 1};
+  if (field == 3) { for (int64_t j = 0; j < n; ++j) r5_replay_lane<FRobertson>(a, j); return 0; }
+  return -1;
+}
+// dense: 0 re-step saveat, 1 the continuous extension
+int lh_r5_saveat(int field, int dense, int64_t n, const double* y0, const double* p, double t0,
+                 const double* tape, const int32_t* n_acc, const double* ts, int64_t k,
+                 double* ys, int32_t* save_step, double* save_t, double* states) {
+  SaveArgs a{n, y0, p, nullptr, t0, tape, n_acc, ts, k, ys, save_step, save_t, states, nullptr};
+  if (field != 3) return -1;
+  for (int64_t j = 0; j < n; ++j) {
+    if (dense)
+      r5_saveat_dense_lane<FRobertson>(a, j);
+    else
+      r5_saveat_lane<FRobertson>(a, j);
+  }
+  return 0;
+}
+// The gradient as gsx_rodas5p_adjoint[_saveat[_dense]] runs it (k = 0: final state only).
+int lh_r5_adjoint(int field, int dense, int64_t n, const double* p, double t0,
+                  const double* tape, int64_t cap, const int32_t* n_acc, const double* ckpt,
+                  int32_t C, const double* ts, int64_t k, const int32_t* save_step,
+                  const double* save_t, const double* states, const double* ysbar,
+                  const double* ybar, double* mu, double* y0bar, double* pbar) {
+  if (field != 3) return -1;
+  AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  if (k > 0 && dense) {
+    a.nsave = k;
+    a.save_step = save_step;
+    a.ts = ts;
+    a.save_t = save_t;
+    a.ysbar = ysbar;
+  } else if (k > 0) {
+    SaveVjpArgs v{n, p, nullptr, n_acc, ts, k, save_step, save_t, states, ysbar, ybar, mu,
+                  y0bar, pbar};
+    for (int64_t j = 0; j < n; ++j) r5_saveat_vjp_lane<FRobertson>(v, j);
+    a.ybar = y0bar;
+    a.nsave = k;
+    a.save_step = save_step;
+    a.mu = mu;
+    a.pbar_in = pbar;
+  }
+  return run_r5_adj<FRobertson>(a);
 }
 int lh_tsit5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
                  const double* t1_per, double rtol, double atol, int64_t max_steps,

@@ -1,0 +1,195 @@
+"""GPU parity of the Rodas5P lane (gsx_rodas5p_*) against the oracle (gradsolve's rodas5p_replay:
+gradsolve/solvers/rodas5p_step.py, rodas5p_replay.py, the device recorder record_jax.py:83-132,
+and dense.py:106-227 for its saves).
+
+Robertson, k = [0.04, 3e7, 1e4] * exp(U(log .8, log 1.25)), y0 = [1, 0, 0], t in [0, 100],
+rtol 1e-6, atol 1e-10:
+  * the record: step counts equal the oracle's on >= 99.9% of trajectories and final states of
+    those within 1e-10 (strict: the device pow is the only difference from the host loop);
+  * the adjoint of sum(y_final * [1, 10, 1]) against the oracle's on the device's own tape
+    (1e-9), at C = 1 and C = 8;
+  * saves by re-stepping and by the continuous extension bit for bit in the strict variant
+    (1e-12 in the default one), and their gradients (1e-9);
+  * the API: engine='rodas5p_replay' routes to hip_rodas5p, saveat_dense=True serves the dense
+    series and its closure gradient (against the oracle), and saveat_dense on another lane
+    raises ValueError;
+  * a stiff field compiled at run time: the Rodas5P closure gradient against central
+    differences of the same frozen-mesh closure (rel 1e-6 beyond the differences' rounding).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+import gradsolve_amd as gs
+from gradsolve_amd import _native as N
+from gradsolve_amd import hip_tsit5 as H
+from gradsolve_amd import problems as P
+from gradsolve_amd import workloads as W
+from gradsolve_amd.fields import register_field
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+W_LOSS = np.array([1.0, 10.0, 1.0])
+TS = np.array([-1.0, 0.0, 1e-5, 0.01, 0.4, 3.0, 3.0, 17.0, 55.5, 99.9, 100.0, 120.0])
+
+
+def _soa(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a, np.float64).T), device=dev)
+
+
+def _record(e, dev, **kw):
+    rec = H.record(N.FIELD_ROBERTSON, _soa(e.y0, dev), _soa(e.params, dev), t0=0.0, t1=100.0,
+                   rtol=1e-6, atol=1e-10, max_steps=50000, entry="gsx_rodas5p_fwd", **kw)
+    torch.cuda.synchronize()
+    return rec
+
+
+def _adjoint(rec, ybar, dev):
+    y0b = torch.empty((3, rec.n), dtype=torch.float64, device=dev)
+    pb = torch.empty((3, rec.n), dtype=torch.float64, device=dev)
+    N.call("gsx_rodas5p_adjoint", N.FIELD_ROBERTSON, rec.n, 3, 3, None, None, rec.p.data_ptr(),
+           0.0, rec.tape.data_ptr(), rec.cap, rec.n_acc.data_ptr(), rec.ckpt.data_ptr(),
+           rec.ckpt_interval, ybar.data_ptr(), y0b.data_ptr(), pb.data_ptr(), 0,
+           torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return y0b.cpu().numpy().T, pb.cpu().numpy().T
+
+
+@pytest.mark.parametrize("strict", [True, False])
+def test_rodas5p_record_matches_oracle(gpu, strict):
+    e = W.robertson(2048, seed=71)
+    print(f"case rodas5p robertson n=2048 strict={strict}", flush=True)
+    rec = _record(e, gpu, ckpt_interval=8, strict=strict)
+    assert int((rec.status != 0).sum().item()) == 0
+    oyf, otape, ona, onr, ost = O.rodas5p_record(O.ROBERTSON, e.y0, e.params, 0.0, 100.0,
+                                                 rtol=1e-6, atol=1e-10)
+    na, nr = rec.n_acc.cpu().numpy(), rec.n_rej.cpu().numpy()
+    same = (na == ona) & (nr == onr)
+    print(f"count mismatches: {(~same).sum()}; mean accepted {na.mean():.1f}", flush=True)
+    assert (~same).mean() < 1e-3
+    yf = rec.y_final.cpu().numpy().T
+    rel = np.abs(yf[same] - oyf[same]) / np.maximum(np.abs(oyf[same]), 1e-300)
+    assert np.max(rel[:, [0, 2]]) < 1e-10 and np.max(rel[:, 1]) < 1e-8
+
+
+@pytest.mark.parametrize("C", [1, 8])
+def test_rodas5p_adjoint_matches_oracle(gpu, C):
+    e = W.robertson(1024, seed=72)
+    print(f"case rodas5p adjoint n=1024 C={C}", flush=True)
+    rec = _record(e, gpu, ckpt_interval=C)
+    ybar = torch.as_tensor(np.tile(W_LOSS[:, None], (1, rec.n)), device=gpu).contiguous()
+    g0, gp = _adjoint(rec, ybar, gpu)
+    tape = rec.padded_tape().cpu().numpy()
+    og0, ogp = O.rodas5p_adjoint(O.ROBERTSON, e.y0, e.params, 0.0, tape,
+                                 np.tile(W_LOSS, (rec.n, 1)))
+    assert np.max(np.abs(gp - ogp) / np.abs(ogp)) < 1e-9
+    assert np.max(np.abs(g0 - og0) / np.abs(og0).max(axis=1, keepdims=True)) < 1e-9
+
+
+@pytest.mark.parametrize("dense", [False, True])
+@pytest.mark.parametrize("strict", [True, False])
+def test_rodas5p_saveat_and_gradient(gpu, dense, strict):
+    e = W.robertson(1024, seed=73)
+    print(f"case rodas5p saveat n=1024 dense={dense} strict={strict}", flush=True)
+    rec = _record(e, gpu, ckpt_interval=8, strict=strict)
+    ts = torch.as_tensor(TS, device=gpu)
+    method = "rodas5p_dense" if dense else "rodas5p"
+    sv = H.saveat(rec, ts, strict=strict, method=method)
+    torch.cuda.synchronize()
+    ys = sv.ys.permute(2, 0, 1).cpu().numpy()
+    tape = rec.padded_tape().cpu().numpy()
+    _, oys = O.rodas5p_replay_saveat(O.ROBERTSON, e.y0, e.params, 0.0, tape, TS, dense=dense)
+    if strict:
+        assert np.array_equal(ys, oys)
+    else:
+        np.testing.assert_allclose(ys, oys, rtol=1e-12, atol=1e-12 * np.abs(oys).max())
+    yf = rec.y_final.t().cpu().numpy()
+    assert np.array_equal(ys[:, -1], yf) and np.array_equal(ys[:, 0], e.y0)
+    assert np.max(np.abs(ys.sum(axis=2) - 1.0)) < 1e-12
+    rng = np.random.default_rng(74)
+    ysbar = rng.standard_normal((1024, TS.shape[0], 3))
+    ybar = rng.standard_normal((1024, 3))
+    y0b, pb = H.adjoint_saveat(rec, ts, sv,
+                               torch.as_tensor(ysbar.transpose(1, 2, 0).copy(), device=gpu),
+                               torch.as_tensor(ybar.T.copy(), device=gpu), strict=strict,
+                               method=method)
+    torch.cuda.synchronize()
+    og0, ogp = O.rodas5p_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, tape, TS, ysbar, ybar,
+                                        dense=dense)
+    g0, gp = y0b.cpu().numpy().T, pb.cpu().numpy().T
+    assert np.max(np.abs(gp - ogp) / np.abs(ogp).max(axis=1, keepdims=True)) < 1e-9
+    assert np.max(np.abs(g0 - og0) / np.abs(og0).max(axis=1, keepdims=True)) < 1e-9
+
+
+def test_rodas5p_through_the_api(gpu):
+    prob = P.Robertson()
+    e = W.robertson(256, seed=75)
+    ts = np.linspace(0.0, 100.0, 21)
+    print("case rodas5p api n=256", flush=True)
+    res = gs.solve(prob, e.y0, e.params, engine="rodas5p_replay", rtol=1e-6, atol=1e-10)
+    assert res.route.actual == "hip_rodas5p" and "alias:rodas5p_replay" in res.route.reason
+    assert np.all(np.isfinite(res.y_final)) and np.all(res.accepted_steps > 10)
+    sd = gs.solve(prob, e.y0, e.params, engine="rodas5p_replay", saveat=ts, saveat_dense=True,
+                  rtol=1e-6, atol=1e-10)
+    sr = gs.solve(prob, e.y0, e.params, engine="rodas5p_replay", saveat=ts, rtol=1e-6,
+                  atol=1e-10)
+    np.testing.assert_array_equal(sd.y_final, sr.y_final)
+    np.testing.assert_array_equal(sd.y_saved[:, -1], sd.y_final)
+    assert np.max(np.abs(sd.y_saved - sr.y_saved)) < 1e-6
+    with pytest.raises(ValueError, match="saveat_dense=True is implemented only"):
+        gs.solve(prob, e.y0, e.params, saveat=ts, saveat_dense=True, rtol=1e-6, atol=1e-10)
+    clo = gs.grad_closure(prob, e.y0, e.params, engine="rodas5p_replay", saveat=ts,
+                          saveat_dense=True, rtol=1e-6, atol=1e-10)
+    assert clo.route.actual == "hip_rodas5p"
+    pt = torch.tensor(e.params, dtype=torch.float64, device=gpu, requires_grad=True)
+    wts = torch.as_tensor(np.tile([1.0, 1e4, 1.0], (256, 21, 1)), device=gpu)
+    (clo(pt) * wts).sum().backward()
+    og0, ogp = O.rodas5p_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, clo.dts_padded(), ts,
+                                        wts.cpu().numpy(), dense=True)
+    g = pt.grad.cpu().numpy()
+    assert np.max(np.abs(g - ogp) / np.abs(ogp).max(axis=1, keepdims=True)) < 1e-9
+
+
+@dataclasses.dataclass
+class Prob:
+    name: str
+    dim: int
+    is_stiff: bool
+    f_jax: object
+    t0: float = 0.0
+    t1: float = 3.0
+
+
+def stiff_pair(t, y, p):
+    return np.array([-p[0] * (y[0] - np.cos(y[1])), -p[1] * y[1] + 0.1 * y[0] * y[0]])
+
+
+def test_rodas5p_runtime_field_matches_finite_differences(gpu):
+    register_field("r5_stiff_pair", stiff_pair, 2, 2, stiff=True)
+    prob = Prob("r5_stiff_pair", 2, True, stiff_pair)
+    rng = np.random.default_rng(76)
+    n = 32
+    y0 = np.column_stack([rng.uniform(-1.0, 1.0, n), rng.uniform(-0.5, 0.5, n)])
+    params = np.column_stack([rng.uniform(500.0, 2000.0, n), rng.uniform(0.5, 1.5, n)])
+    print("case rodas5p runtime stiff_pair n=32", flush=True)
+    clo = gs.grad_closure(prob, y0, params, engine="rodas5p_replay", rtol=1e-7, atol=1e-9)
+    assert clo.route.actual == "hip_rodas5p"
+    w = rng.standard_normal((n, 2))
+    pt = torch.tensor(params, dtype=torch.float64, device=gpu, requires_grad=True)
+    (clo(pt) * torch.as_tensor(w, device=gpu)).sum().backward()
+    g = pt.grad.cpu().numpy()
+    g_fd = np.zeros_like(params)
+    for c in range(2):
+        eps = 1e-6 * np.abs(params[:, c])
+        pp, pm = params.copy(), params.copy()
+        pp[:, c] += eps
+        pm[:, c] -= eps
+        g_fd[:, c] = ((clo(pp) - clo(pm)) * w).sum(axis=1) / (2 * eps)
+    loss = np.abs((clo(params) * w).sum(axis=1))[:, None]
+    fd_noise = 2.2e-16 * loss / (2e-6 * params)
+    err = np.abs(g - g_fd) - 4.0 * fd_noise
+    for c in range(2):
+        assert np.max(err[:, c]) / np.abs(g_fd[:, c]).max() < 1e-6, c

@@ -361,3 +361,74 @@ def test_rb23_lanes_and_saveat_match_oracle(lh, C):
     sg0, sgp = O.rb23_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts, ysbar, ybar)
     np.testing.assert_allclose(gq.a.T, sgp, rtol=1e-12, atol=1e-12 * np.abs(sgp).max())
     np.testing.assert_allclose(g0.a.T, sg0, rtol=1e-12, atol=1e-12 * np.abs(sg0).max())
+
+
+@pytest.mark.parametrize("C", [1, 4])
+def test_rodas5p_lanes_match_oracle(lh, C):
+    """The Rodas5P lane bodies (gsx_rodas5p_*, strict build) against the oracle on Robertson:
+    the record (counts, tape, final state) and the replay bit for bit, the re-step and the
+    dense saves bit for bit (saves before t0, on the mesh, at t1 and past it), and the
+    gradients of the final state, of a re-step series and of a dense series to 1e-12, every
+    buffer bounds-guarded."""
+    vp, i64, i32, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, \
+        ctypes.c_int
+    lh.lh_r5_fwd.argtypes = [i, i64, vp, vp, d, d, d, d, i64, vp, vp, vp, vp, vp, i64, vp, i32]
+    lh.lh_r5_replay.argtypes = [i, i64, vp, vp, d, vp, vp, vp]
+    lh.lh_r5_saveat.argtypes = [i, i, i64, vp, vp, d, vp, vp, vp, i64, vp, vp, vp, vp]
+    lh.lh_r5_adjoint.argtypes = [i, i, i64, vp, d, vp, i64, vp, vp, i32, vp, i64, vp, vp, vp,
+                                 vp, vp, vp, vp, vp]
+    e = W.robertson(12, seed=35)
+    n, cap, t1 = 12, 512, 20.0
+    y0s = np.ascontiguousarray(e.y0.T)
+    ps = np.ascontiguousarray(e.params.T)
+    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+        Guarded((n,), np.int32)
+    tape, ck = Guarded((cap, n)), Guarded((cap // C + 1, 3, n))
+    assert lh.lh_r5_fwd(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, 1e-6, 1e-10, 50000,
+                        yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C) == 0
+    assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
+    oyf, otape, ona, onr, ost = O.rodas5p_record(O.ROBERTSON, e.y0, e.params, 0.0, t1,
+                                                 rtol=1e-6, atol=1e-10)
+    assert np.all(st.a == 0) and np.array_equal(na.a, ona) and np.array_equal(nr.a, onr)
+    assert np.array_equal(yf.a.T, oyf)
+    S = int(na.a.max())
+    padded = np.zeros((n, S))
+    for j in range(n):
+        padded[j, :na.a[j]] = tape.a[:na.a[j], j]
+    assert np.array_equal(padded, otape[:, :S])
+    ry = Guarded((3, n))
+    lh.lh_r5_replay(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr)
+    assert ry.intact() and np.array_equal(ry.a, yf.a)
+    rng = np.random.default_rng(36)
+    ybar = rng.standard_normal((n, 3))
+    ybar_soa = np.ascontiguousarray(ybar.T)
+    g0, gp = Guarded((3, n)), Guarded((3, n))
+    lh.lh_r5_adjoint(3, 0, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C, None, 0,
+                     None, None, None, None, ybar_soa.ctypes.data, None, g0.ptr, gp.ptr)
+    assert g0.intact() and gp.intact()
+    og0, ogp = O.rodas5p_adjoint(O.ROBERTSON, e.y0, e.params, 0.0, padded, ybar)
+    np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=1e-12 * np.abs(ogp).max())
+    np.testing.assert_allclose(g0.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
+    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0])
+    k = ts.shape[0]
+    ysbar = rng.standard_normal((n, k, 3))
+    ysb_soa = np.ascontiguousarray(ysbar.transpose(1, 2, 0))
+    for dense in (0, 1):
+        ys, sstep, st_t, states = Guarded((k, 3, n)), Guarded((k, n), np.int32), \
+            Guarded((k, n)), Guarded((k, 3, n))
+        lh.lh_r5_saveat(3, dense, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
+                        ts.ctypes.data, k, ys.ptr, sstep.ptr, st_t.ptr, states.ptr)
+        assert all(g.intact() for g in (ys, sstep, st_t))
+        _, oys = O.rodas5p_replay_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts,
+                                         dense=bool(dense))
+        assert np.array_equal(ys.a.transpose(2, 0, 1), oys), dense
+        assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, e.y0)
+        mu, h0, hp = Guarded((k, 3, n)), Guarded((3, n)), Guarded((3, n))
+        lh.lh_r5_adjoint(3, dense, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+                         ts.ctypes.data, k, sstep.ptr, st_t.ptr, states.ptr,
+                         ysb_soa.ctypes.data, ybar_soa.ctypes.data, mu.ptr, h0.ptr, hp.ptr)
+        assert all(g.intact() for g in (mu, h0, hp))
+        sg0, sgp = O.rodas5p_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts, ysbar,
+                                            ybar, dense=bool(dense))
+        np.testing.assert_allclose(hp.a.T, sgp, rtol=1e-12, atol=1e-12 * np.abs(sgp).max())
+        np.testing.assert_allclose(h0.a.T, sg0, rtol=1e-12, atol=1e-12 * np.abs(sg0).max())

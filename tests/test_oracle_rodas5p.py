@@ -72,7 +72,7 @@ def test_order_five_linear_and_nonlinear():
                 p[0, 0] * y[0] - p[0, 2] * y[1] * y[2] - p[0, 1] * y[1] ** 2,
                 p[0, 1] * y[1] ** 2]
 
-    hs = np.array([0.4, 0.2, 0.1, 0.05])
+    hs = np.array([0.2, 0.1, 0.05, 0.025])
     errs = []
     for h in hs:
         ref = si.solve_ivp(f, (0.0, h), yA[0], method="Radau", rtol=1e-14, atol=1e-16).y[:, -1]
