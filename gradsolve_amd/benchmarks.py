@@ -136,17 +136,26 @@ def uneven(args, dev):
 
 def robertson(args, dev):
     """Stiff: Robertson, N = 262,144, Rosenbrock23 rtol 1e-6 atol 1e-10, t in [0, 100];
-    gradient of sum(y_final * [1, 10, 1]) w.r.t. k1..k3."""
+    gradient of sum(y_final * [1, 10, 1]) w.r.t. k1..k3. The same workload on the Rodas5P
+    lane is reported beside it (``rodas5p``)."""
+    out = _stiff_lane(args, dev, "rb23", "robertson_rb23")
+    out["rodas5p"] = _stiff_lane(args, dev, "rodas5p", "robertson_rodas5p")
+    return out
+
+
+def _stiff_lane(args, dev, method, name):
+    fwd_entry, adj_entry = f"gsx_{method}_fwd", f"gsx_{method}_adjoint"
     e = W.robertson(args.n or (1 << 18), seed=0)
     n = e.y0.shape[0]
     y0, p = _soa(e.y0, dev), _soa(e.params, dev)
     rec = H.record(N.FIELD_ROBERTSON, y0, p, t0=0.0, t1=100.0, rtol=1e-6, atol=1e-10,
-                   max_steps=50000, tape_cap=1024, ckpt_interval=8, entry="gsx_rb23_fwd")
+                   max_steps=50000, tape_cap=1024, ckpt_interval=8, entry=fwd_entry)
     torch.cuda.synchronize()
     S = rec.max_steps_taken()
     acc = int(rec.n_acc.to(torch.int64).sum().item())
-    out = {"workload": "robertson_rb23", "n": n, "mean_accepted": round(acc / n, 2),
-           "max_accepted": S}
+    rej = int(rec.n_rej.to(torch.int64).sum().item())
+    out = {"workload": name, "n": n, "mean_accepted": round(acc / n, 2),
+           "mean_rejected": round(rej / n, 2), "max_accepted": S}
     wvec = torch.tensor([1.0, 10.0, 1.0], dtype=torch.float64, device=dev)[:, None]
     best = None
     for C in (4, 8, 16):
@@ -154,7 +163,7 @@ def robertson(args, dev):
         ybar = (wvec * torch.ones((3, n), dtype=torch.float64, device=dev)).contiguous()
 
         def fwd():
-            N.call("gsx_rb23_fwd", N.FIELD_ROBERTSON, n, 3, 3, None, y0.data_ptr(),
+            N.call(fwd_entry, N.FIELD_ROBERTSON, n, 3, 3, None, y0.data_ptr(),
                    p.data_ptr(), 0.0, 100.0, None, 1e-6, 1e-10, 50000, ws.y_final.data_ptr(),
                    ws.n_acc.data_ptr(), ws.n_rej.data_ptr(), ws.status.data_ptr(),
                    ws.tape.data_ptr(), ws.cap, ws.ckpt.data_ptr(), C, 0,
@@ -162,7 +171,7 @@ def robertson(args, dev):
 
         def both():
             fwd()
-            N.call("gsx_rb23_adjoint", N.FIELD_ROBERTSON, n, 3, 3, None, None, p.data_ptr(),
+            N.call(adj_entry, N.FIELD_ROBERTSON, n, 3, 3, None, None, p.data_ptr(),
                    0.0, ws.tape.data_ptr(), ws.cap, ws.n_acc.data_ptr(), ws.ckpt.data_ptr(), C,
                    ybar.data_ptr(), ws.y0bar.data_ptr(), ws.pbar.data_ptr(), 0,
                    H._stream_handle(None))
