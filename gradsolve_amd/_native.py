@@ -26,6 +26,7 @@ FIELD_ROBERTSON = 3
 FIELD_KURAMOTO = 4
 FIELD_VDP = 5
 FIELD_LORENZ3 = 6
+FIELD_FORCED_STIFF = 8
 FIELD_RUNTIME_BASE = 1000  # ids of fields registered at run time start here
 FIELD_STIFF = 1            # gsx_field_register flags
 FIELD_TIME_DEPENDENT = 2

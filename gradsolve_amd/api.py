@@ -124,9 +124,10 @@ def _solve_rb23(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult
 
 
 def _supports_rodas5p(problem) -> bool:
-    """A field with an analytic Jacobian (a stiff binding), autonomous."""
+    """A field with an analytic Jacobian (a stiff binding); one that reads t also carries its
+    generated df/dt (gradsolve/solvers/rodas5p_step.py:119-146)."""
     b = F.binding_for(problem)
-    return b is not None and b.stiff and not getattr(b, "time_dependent", False)
+    return b is not None and b.stiff
 
 
 def _solve_rodas5p(problem, y0, params, *, rtol, atol, device, strict=False,

@@ -95,6 +95,9 @@ def header() -> str:
            "// c_i (time abscissae) and d_i = sum_j Gamma_ij (df/dt weights), as published\n",
            _arr("kC", ABSCISSAE),
            _arr("kD", GAMMA_SUMS),
+           "// sum_j Gamma_ij of the converted Gamma (NumPy's row sum, diagonal included): the\n",
+           "// df/dt weight of stage i for fields that depend on t\n",
+           _arr("kGammaSum", gam.sum(axis=1)),
            "// m_i (solution weights) and m_i - mhat_i (error weights)\n",
            _arr("kM", m),
            _arr("kE", err),

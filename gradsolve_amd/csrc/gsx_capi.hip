@@ -37,6 +37,7 @@ bool field_info(int field, int D, FieldInfo* out) {
     case GSX_FIELD_KURAMOTO: *out = {32, 32, true, false}; return D == 32;
     case GSX_FIELD_VDP: *out = {2, 1, true, false}; return D == 2;
     case GSX_FIELD_LORENZ3: *out = {3, 3, true, false}; return D == 3;
+    case GSX_FIELD_FORCED_STIFF: *out = {2, 3, false, true}; return D == 2;
     default: return false;
   }
 }
@@ -440,13 +441,12 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, cons
 
 // ---------------------------------------------------------------------------------------
 // Rodas5P (gsx_rodas5p.hip): fields with an analytic Jacobian (the built-in stiff fields and
-// fields registered with GSX_FIELD_STIFF), autonomous.
+// fields registered with GSX_FIELD_STIFF); a field that reads t also supplies df/dt.
 // ---------------------------------------------------------------------------------------
 static int r5_check(int field, int D, int P, const char* what) {
   FieldInfo fi;
   int rc = check_field(field, D, P, &fi);
   if (rc) return rc;
-  if (!fi.autonomous) return set_error(GSX_E_UNSUPPORTED, "%s: Rodas5P is built for autonomous fields", what);
   if (::gsx::rt::is_runtime(field) && !fi.stiff)
     return set_error(GSX_E_INVALID, "%s: field %d has no Jacobian (register it with "
                      "GSX_FIELD_STIFF)", what, field);

@@ -372,7 +372,7 @@ struct FLinearDecay {
     GSX_UNROLL for (int i = 0; i < D; ++i) J[i * D + i] = -p[0];
   }
   template <class T>
-  GSX_HD static void jac_vjp(const T*, const T*, const T* Jb, T*, T* pb, const double*) {
+  GSX_HD static void jac_vjp(T, const T*, const T*, const T* Jb, T*, T* pb, const double*) {
     T tr = T(0);
     GSX_UNROLL for (int i = 0; i < D; ++i) tr += Jb[i * D + i];
     pb[0] -= tr;
@@ -417,7 +417,7 @@ struct FRobertson {
     J[8] = T(0.0);
   }
   template <class T>
-  GSX_HD static void jac_vjp(const T* y, const T* p, const T* Jb, T* yb, T* pb,
+  GSX_HD static void jac_vjp(T, const T* y, const T* p, const T* Jb, T* yb, T* pb,
                              const double*) {
     const T k2 = p[1], k3 = p[2];
     const T y1 = y[1], y2 = y[2];
@@ -426,6 +426,65 @@ struct FRobertson {
     pb[0] += Jb[3] - Jb[0];
     pb[1] += T(2.0) * y1 * (Jb[7] - Jb[4]);
     pb[2] += y2 * Jb[1] + y1 * Jb[2] - y2 * Jb[4] - y1 * Jb[5];
+  }
+};
+
+// A stiff field that reads t (D=2, P=3: a, b, c), for the Rodas5P lane's df/dt term:
+//   f0 = -a (y0 - y1^2) + 0.4 sin 3t,  f1 = y0 - y1 - b y1^3 + 0.2 cos t + c sin(t) y0.
+// At (a, b, c) = (30, 0.7, 0) it is the time-dependent stiff test field of gradsolve's
+// tests/test_rodas5p_dense.py:94-103; c makes J depend on t and df/dt on y0 and c.
+// dfdt gives df/dt at (t, y); dfdt_vjp adds d/d(y, p) of ftb . df/dt.
+struct FForcedStiff {
+  static constexpr int D = 2, P = 3;
+  static constexpr bool autonomous = false;
+  template <class T>
+  GSX_HD static void f(T t, const T* y, const T* p, T* dy, const double*) {
+    const T a = p[0], b = p[1], c = p[2];
+    const T st = sin(t);
+    dy[0] = -a * (y[0] - y[1] * y[1]) + T(0.4) * sin(T(3.0) * t);
+    dy[1] = y[0] - y[1] - b * y[1] * y[1] * y[1] + T(0.2) * cos(t) + c * st * y[0];
+  }
+  template <class T>
+  GSX_HD static void vjp(T t, const T* y, const T* p, const T* v, T* yb, T* pb, const double*) {
+    const T a = p[0], b = p[1], c = p[2];
+    const T st = sin(t);
+    yb[0] += -a * v[0] + (T(1.0) + c * st) * v[1];
+    yb[1] += T(2.0) * a * y[1] * v[0] - (T(1.0) + T(3.0) * b * y[1] * y[1]) * v[1];
+    pb[0] -= (y[0] - y[1] * y[1]) * v[0];
+    pb[1] -= y[1] * y[1] * y[1] * v[1];
+    pb[2] += st * y[0] * v[1];
+  }
+  template <class T>
+  GSX_HD static void jac(T t, const T* y, const T* p, T* J, const double*) {
+    const T a = p[0], b = p[1], c = p[2];
+    const T y1 = y[1];
+    J[0] = -a;
+    J[1] = T(2.0) * a * y1;
+    J[2] = T(1.0) + c * sin(t);
+    J[3] = -T(1.0) - T(3.0) * b * y1 * y1;
+  }
+  template <class T>
+  GSX_HD static void jac_vjp(T t, const T* y, const T* p, const T* Jb, T* yb, T* pb,
+                             const double*) {
+    const T a = p[0], b = p[1];
+    const T y1 = y[1];
+    yb[1] += T(2.0) * a * Jb[1] - T(6.0) * b * y1 * Jb[3];
+    pb[0] += T(2.0) * y1 * Jb[1] - Jb[0];
+    pb[1] -= T(3.0) * y1 * y1 * Jb[3];
+    pb[2] += sin(t) * Jb[2];
+  }
+  template <class T>
+  GSX_HD static void dfdt(T t, const T* y, const T* p, T* ft, const double*) {
+    const T c = p[2];
+    ft[0] = T(1.2) * cos(T(3.0) * t);
+    ft[1] = -T(0.2) * sin(t) + c * cos(t) * y[0];
+  }
+  template <class T>
+  GSX_HD static void dfdt_vjp(T t, const T* y, const T* p, const T* ftb, T* yb, T* pb,
+                              const double*) {
+    const T ct = cos(t);
+    yb[0] += p[2] * ct * ftb[1];
+    pb[2] += ct * y[0] * ftb[1];
   }
 };
 

@@ -198,7 +198,7 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
   const double dh = h * rb_d();
   double Jb[D * D];
   GSX_UNROLL for (int i = 0; i < D * D; ++i) Jb[i] = -dh * Wb[i];
-  F::jac_vjp(y, p, Jb, yb, pb, aux);
+  F::jac_vjp(t, y, p, Jb, yb, pb, aux);
   GSX_UNROLL for (int i = 0; i < D; ++i) lam[i] = yb[i];
 }
 

@@ -75,6 +75,7 @@ constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23;
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_ROBERTSON: return launch_fwd<FRobertson>(a, s);
+    case GSX_FIELD_FORCED_STIFF: return launch_fwd<FForcedStiff>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_fwd<FLinearDecay<1>>(a, s);
       if (D == 2) return launch_fwd<FLinearDecay<2>>(a, s);
@@ -91,6 +92,7 @@ int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
 int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_ROBERTSON: return launch_replay<FRobertson>(a, s);
+    case GSX_FIELD_FORCED_STIFF: return launch_replay<FForcedStiff>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_replay<FLinearDecay<1>>(a, s);
       if (D == 2) return launch_replay<FLinearDecay<2>>(a, s);
@@ -107,6 +109,7 @@ int rb23_replay_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
 int rb23_adj_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
   switch (field) {
     case GSX_FIELD_ROBERTSON: return launch_adj<FRobertson>(a, s);
+    case GSX_FIELD_FORCED_STIFF: return launch_adj<FForcedStiff>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_adj<FLinearDecay<1>>(a, s);
       if (D == 2) return launch_adj<FLinearDecay<2>>(a, s);
@@ -124,6 +127,7 @@ int rb23_saveat_dispatch(int field, int D, const SaveArgs& a, hipStream_t s) {
   if (a.n == 0) return GSX_OK;
   switch (field) {
     case GSX_FIELD_ROBERTSON: return launch_saveat<FRobertson>(a, s);
+    case GSX_FIELD_FORCED_STIFF: return launch_saveat<FForcedStiff>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_saveat<FLinearDecay<1>>(a, s);
       if (D == 2) return launch_saveat<FLinearDecay<2>>(a, s);
@@ -141,6 +145,7 @@ int rb23_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t
   if (a.n == 0) return GSX_OK;
   switch (field) {
     case GSX_FIELD_ROBERTSON: return launch_saveat_vjp<FRobertson>(a, s);
+    case GSX_FIELD_FORCED_STIFF: return launch_saveat_vjp<FForcedStiff>(a, s);
     case GSX_FIELD_LINEAR_DECAY:
       if (D == 1) return launch_saveat_vjp<FLinearDecay<1>>(a, s);
       if (D == 2) return launch_saveat_vjp<FLinearDecay<2>>(a, s);

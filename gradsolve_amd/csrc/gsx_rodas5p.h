@@ -1,11 +1,13 @@
 // gsx_rodas5p.h -- per-lane Rodas5P (Steinebach 2023: order-5(4) Rosenbrock-Wanner pair, 8
-// stages, order-4 continuous extension) for autonomous stiff fields with an analytic Jacobian:
-// forward + record, replay, adjoint, and the dense output of the continuous extension with its
-// adjoint. gradsolve's Rodas5P lane is rodas5p_step.py / rodas5p_replay.py, recorded by
+// stages, order-4 continuous extension) for stiff fields with an analytic Jacobian: forward +
+// record, replay, adjoint, and the dense output of the continuous extension with its adjoint.
+// gradsolve's Rodas5P lane is rodas5p_step.py / rodas5p_replay.py, recorded by
 // record_jax.py:83-132 (the recorder's status 2 at the underflow floor and its FAC_MIN shrink
 // after a non-finite trial are kept). The tableau (gsx_rodas5p_tab.h) is in alpha/Gamma form:
-//   W = I - (h gamma) J(y);  u_i = y + sum_{j<i} alpha_ij k_j,  a_i = sum_{j<i} Gamma_ij k_j,
-//   k_i = W \ (h f(u_i) + h J a_i);  y_next = y + sum m_i k_i;  error = sum (m_i - mhat_i) k_i.
+//   J = df/dy(t, y), f_t = df/dt(t, y) (fields that read t; rodas5p_step.py:119-146);
+//   W = I - (h gamma) J;  u_i = y + sum_{j<i} alpha_ij k_j,  a_i = sum_{j<i} Gamma_ij k_j,
+//   k_i = W \ (h f(t + c_i h, u_i) + h J a_i + h^2 g_i f_t),  g_i = sum_j Gamma_ij;
+//   y_next = y + sum m_i k_i;  error = sum (m_i - mhat_i) k_i.
 // One LU of W per step (in registers, partial pivoting), reused by the 8 stage solves.
 #pragma once
 
@@ -48,24 +50,27 @@ GSX_HD void r5_stage_input(const double* y, const double (&k)[8][F::D], int i, d
 }
 
 template <class F>
-GSX_HD void r5_stages(const double* y, double h, const double* p, const double* aux,
+GSX_HD void r5_stages(double t, const double* y, double h, const double* p, const double* aux,
                       R5Stages<F>& w) {
   constexpr int D = F::D;
-  F::jac(0.0, y, p, w.J, aux);
+  F::jac(t, y, p, w.J, aux);
   const double hg = h * r5t::kGamma;
   GSX_UNROLL for (int a = 0; a < D; ++a) {
     GSX_UNROLL for (int b = 0; b < D; ++b)
       w.W[a * D + b] = (a == b ? 1.0 : 0.0) - hg * w.J[a * D + b];
   }
   lu_factor<D>(w.W, w.piv);
+  double ft[D];
+  if constexpr (!F::autonomous) F::dfdt(t, y, p, ft, aux);
   GSX_UNROLL for (int i = 0; i < 8; ++i) {
     double u[D], acc[D], fv[D], rhs[D];
     r5_stage_input<F>(y, w.k, i, u, acc);
-    F::f(0.0, u, p, fv, aux);
+    F::f(t + r5t::kC[i] * h, u, p, fv, aux);
     GSX_UNROLL for (int r = 0; r < D; ++r) {
       double ja = 0.0;
       GSX_UNROLL for (int c = 0; c < D; ++c) ja = ja + w.J[r * D + c] * acc[c];
       rhs[r] = h * fv[r] + h * ja;
+      if constexpr (!F::autonomous) rhs[r] = rhs[r] + (h * h * r5t::kGammaSum[i]) * ft[r];
     }
     lu_solve<D>(w.W, w.piv, rhs, w.k[i]);
   }
@@ -82,9 +87,9 @@ GSX_HD void r5_combine(const double* y, const R5Stages<F>& w, double* ynew) {
 
 // Value-only step rodas5p_advance (rodas5p_step.py:149-155).
 template <class F>
-GSX_HD void r5_advance(double* y, double h, const double* p, const double* aux) {
+GSX_HD void r5_advance(double t, double* y, double h, const double* p, const double* aux) {
   R5Stages<F> w;
-  r5_stages<F>(y, h, p, aux, w);
+  r5_stages<F>(t, y, h, p, aux, w);
   double yn[F::D];
   r5_combine<F>(y, w, yn);
   GSX_UNROLL for (int d = 0; d < F::D; ++d) y[d] = yn[d];
@@ -94,11 +99,11 @@ GSX_HD void r5_advance(double* y, double h, const double* p, const double* aux) 
 // controller exponent -1/5 is Tsit5's too). finite: y_next, the error vector and the measure
 // are all finite.
 template <class F>
-GSX_HD double r5_trial(const double* y, double h, const double* p, const double* aux,
+GSX_HD double r5_trial(double t, const double* y, double h, const double* p, const double* aux,
                        double* ynew, double rtol, double atol, bool& finite) {
   constexpr int D = F::D;
   R5Stages<F> w;
-  r5_stages<F>(y, h, p, aux, w);
+  r5_stages<F>(t, y, h, p, aux, w);
   double e[D];
   GSX_UNROLL for (int d = 0; d < D; ++d) {
     ynew[d] = y[d];
@@ -138,19 +143,20 @@ GSX_HD void r5_dense_eval(const double* yl, const double* yr, const double (&dv)
     out[d] = om * yl[d] + th * (yr[d] + om * (dv[0][d] + th * (dv[1][d] + th * dv[2][d])));
 }
 
-// Adjoint of r5_advance at (y, h): lam <- (dy_next/dy)^T lam (+ the dense vectors' cotangents
-// dbar, when given, carried to the stages), pb += (d/dp)^T. Reverse over the stages: each solve
-// k_i = W^-1 rhs_i sends W^-T kbar_i to rhs_i and -W^-T kbar_i k_i^T to W; rhs_i sends its
-// cotangent to f at u_i (then to y and the earlier k via alpha), to J (outer product with a_i)
-// and to the earlier k via Gamma J^T; W = I - h gamma J sends -h gamma Wbar to J, and J's
-// cotangent reaches y and p through jac_vjp.
+// Adjoint of r5_advance at (t, y, h): lam <- (dy_next/dy)^T lam (+ the dense vectors'
+// cotangents dbar, when given, carried to the stages), pb += (d/dp)^T. Reverse over the
+// stages: each solve k_i = W^-1 rhs_i sends W^-T kbar_i to rhs_i and -W^-T kbar_i k_i^T to W;
+// rhs_i sends its cotangent to f at u_i (then to y and the earlier k via alpha), to J (outer
+// product with a_i), to the earlier k via Gamma J^T, and h^2 g_i of it to f_t; W = I - h gamma J
+// sends -h gamma Wbar to J; J's cotangent reaches y and p through jac_vjp, f_t's through
+// dfdt_vjp.
 template <class F>
-GSX_HD void r5_step_vjp(const double* y, double h, const double* p, double* lam, double* pb,
-                        const double* aux, const double (*dbar)[F::D] = nullptr) {
+GSX_HD void r5_step_vjp(double t, const double* y, double h, const double* p, double* lam,
+                        double* pb, const double* aux, const double (*dbar)[F::D] = nullptr) {
   constexpr int D = F::D;
   R5Stages<F> w;
-  r5_stages<F>(y, h, p, aux, w);
-  double kb[8][D], yb[D], Wb[D * D], Jb[D * D];
+  r5_stages<F>(t, y, h, p, aux, w);
+  double kb[8][D], yb[D], Wb[D * D], Jb[D * D], ftb[D];
   GSX_UNROLL for (int d = 0; d < D; ++d) yb[d] = lam[d];
   GSX_UNROLL for (int i = 0; i < 8; ++i) {
     GSX_UNROLL for (int d = 0; d < D; ++d) {
@@ -164,10 +170,14 @@ GSX_HD void r5_step_vjp(const double* y, double h, const double* p, double* lam,
     Wb[x] = 0.0;
     Jb[x] = 0.0;
   }
+  GSX_UNROLL for (int d = 0; d < D; ++d) ftb[d] = 0.0;
   GSX_UNROLL for (int i = 7; i >= 0; --i) {
     double u[D], acc[D], rb[D], fb[D], ub[D], accb[D];
     r5_stage_input<F>(y, w.k, i, u, acc);
     lu_solve_t<D>(w.W, w.piv, kb[i], rb);
+    if constexpr (!F::autonomous) {
+      GSX_UNROLL for (int d = 0; d < D; ++d) ftb[d] = ftb[d] + (h * h * r5t::kGammaSum[i]) * rb[d];
+    }
     GSX_UNROLL for (int a = 0; a < D; ++a) {
       GSX_UNROLL for (int b = 0; b < D; ++b) Wb[a * D + b] = Wb[a * D + b] - rb[a] * w.k[i][b];
     }
@@ -175,7 +185,7 @@ GSX_HD void r5_step_vjp(const double* y, double h, const double* p, double* lam,
       fb[d] = h * rb[d];
       ub[d] = -0.0;  // x + (-0.0) == x: the first add folds
     }
-    F::vjp(0.0, u, p, fb, ub, pb, aux);
+    F::vjp(t + r5t::kC[i] * h, u, p, fb, ub, pb, aux);
     GSX_UNROLL for (int a = 0; a < D; ++a) {
       GSX_UNROLL for (int b = 0; b < D; ++b) Jb[a * D + b] = Jb[a * D + b] + h * rb[a] * acc[b];
     }
@@ -195,7 +205,8 @@ GSX_HD void r5_step_vjp(const double* y, double h, const double* p, double* lam,
   }
   const double hg = h * r5t::kGamma;
   GSX_UNROLL for (int x = 0; x < D * D; ++x) Jb[x] = Jb[x] - hg * Wb[x];
-  F::jac_vjp(y, p, Jb, yb, pb, aux);
+  F::jac_vjp(t, y, p, Jb, yb, pb, aux);
+  if constexpr (!F::autonomous) F::dfdt_vjp(t, y, p, ftb, yb, pb, aux);
   GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = yb[d];
 }
 
@@ -203,14 +214,12 @@ GSX_HD void r5_step_vjp(const double* y, double h, const double* p, double* lam,
 struct MethodRodas5P {
   template <class F>
   GSX_HD static void advance(double t, double* y, double h, const double* p, const double* aux) {
-    (void)t;
-    r5_advance<F>(y, h, p, aux);
+    r5_advance<F>(t, y, h, p, aux);
   }
   template <class F>
   GSX_HD static void step_vjp(double t, const double* y, double h, const double* p,
                               double* lam, double* pb, const double* aux) {
-    (void)t;
-    r5_step_vjp<F>(y, h, p, lam, pb, aux);
+    r5_step_vjp<F>(t, y, h, p, lam, pb, aux);
   }
 };
 
@@ -220,7 +229,6 @@ struct MethodRodas5P {
 // rejection and shrinks h by FAC_MIN without a controller update; max_steps counts attempts.
 template <class F>
 GSX_HD void r5_fwd_lane(const FwdArgs& a, int64_t j) {
-  static_assert(F::autonomous, "the Rodas5P lane is built for autonomous fields");
   constexpr int D = F::D, P = F::P;
   const int64_t n = a.n;
   double y[D], p[P], ynew[D];
@@ -246,7 +254,7 @@ GSX_HD void r5_fwd_lane(const FwdArgs& a, int64_t j) {
       break;
     }
     bool finite = true;
-    const double err = r5_trial<F>(y, h, p, a.aux, ynew, a.rtol, a.atol, finite);
+    const double err = r5_trial<F>(t, y, h, p, a.aux, ynew, a.rtol, a.atol, finite);
     ++attempts;
     if (finite && err <= 1.0) {
       if (rec && acc >= a.tape_cap) {
@@ -292,7 +300,7 @@ GSX_HD void r5_replay_lane(const ReplayArgs& a, int64_t j) {
   int32_t ck_left = a.ckpt_interval;
   for (int32_t s = 0; s < na; ++s) {
     const double h = a.tape[static_cast<int64_t>(s) * n + j];
-    r5_advance<F>(y, h, p, a.aux);
+    r5_advance<F>(t, y, h, p, a.aux);
     t = t + h;
     if (ck && --ck_left == 0) {
       ckpt_store<F>(a.ckpt, (s + 1) / a.ckpt_interval, n, j, y, t);
@@ -368,24 +376,32 @@ GSX_HD void r5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstrid
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     double y[D];
+    double t = 0.0;
     const double* base = a.ckpt + static_cast<int64_t>(m) * DCK * n + j;
     GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = base[d * n];
+    if constexpr (!F::autonomous) t = base[D * n];
     for (int32_t s = s0; s < s1; ++s) {
       double* slot = seg + static_cast<int64_t>(s - s0) * DCK * sstride;
       GSX_UNROLL for (int d = 0; d < D; ++d) slot[d * sstride] = y[d];
-      if (s + 1 < s1) r5_advance<F>(y, a.tape[static_cast<int64_t>(s) * n + j], p, a.aux);
+      if constexpr (!F::autonomous) slot[D * sstride] = t;
+      if (s + 1 < s1) {
+        const double h = a.tape[static_cast<int64_t>(s) * n + j];
+        r5_advance<F>(t, y, h, p, a.aux);
+        t = t + h;
+      }
     }
     for (int32_t s = s1 - 1; s >= s0; --s) {
       const double* slot = seg + static_cast<int64_t>(s - s0) * DCK * sstride;
       GSX_UNROLL for (int d = 0; d < D; ++d) y[d] = slot[d * sstride];
+      if constexpr (!F::autonomous) t = slot[D * sstride];
       const double h = a.tape[static_cast<int64_t>(s) * n + j];
       if (dense) {
         double dbar[3][D], post[D];
         const bool any = r5_dense_cotangents<D>(a, j, s, h, qs, lam, dbar, post);
-        r5_step_vjp<F>(y, h, p, lam, pb, a.aux, any ? dbar : nullptr);
+        r5_step_vjp<F>(t, y, h, p, lam, pb, a.aux, any ? dbar : nullptr);
         GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] += post[d];
       } else {
-        r5_step_vjp<F>(y, h, p, lam, pb, a.aux);
+        r5_step_vjp<F>(t, y, h, p, lam, pb, a.aux);
         inject_saves<D>(a, j, s, qs, lam);
       }
     }
@@ -431,7 +447,7 @@ GSX_HD void r5_saveat_dense_lane(const SaveArgs& a, int64_t j) {
     const double h = a.tape[static_cast<int64_t>(s) * n + j];
     const double tn = t + h;
     R5Stages<F> w;
-    r5_stages<F>(y, h, p, a.aux, w);
+    r5_stages<F>(t, y, h, p, a.aux, w);
     double yn[D];
     r5_combine<F>(y, w, yn);
     if (q < k && a.ts[q] <= tn && !(s == na - 1 && a.ts[q] >= tn)) {

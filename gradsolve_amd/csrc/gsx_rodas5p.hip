@@ -77,6 +77,7 @@ constexpr ::gsx::rt::Family kRtOwn = ::gsx::rt::kRodas5p;
 #define GSX_R5_FIELDS(CALL, ...)                                                  \
   switch (field) {                                                                \
     case GSX_FIELD_ROBERTSON: return CALL<FRobertson>(__VA_ARGS__);               \
+    case GSX_FIELD_FORCED_STIFF: return CALL<FForcedStiff>(__VA_ARGS__);          \
     case GSX_FIELD_LINEAR_DECAY:                                                  \
       if (D == 1) return CALL<FLinearDecay<1>>(__VA_ARGS__);                      \
       if (D == 2) return CALL<FLinearDecay<2>>(__VA_ARGS__);                      \

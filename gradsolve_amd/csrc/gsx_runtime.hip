@@ -409,9 +409,6 @@ int gsx_field_register(const char* source, int D, int P, int flags, int* field_i
     return set_error(GSX_E_INVALID, "field_register: unknown flags 0x%x", flags);
   const bool stiff = (flags & GSX_FIELD_STIFF) != 0;
   const bool autonomous = (flags & GSX_FIELD_TIME_DEPENDENT) == 0;
-  if (stiff && !autonomous)
-    return set_error(GSX_E_UNSUPPORTED,
-                     "field_register: the Rosenbrock23 kernels take autonomous fields only");
   std::lock_guard<std::mutex> lk(rt::g_mu);
   auto f = std::make_unique<rt::RtField>();
   f->D = D;
@@ -420,12 +417,17 @@ int gsx_field_register(const char* source, int D, int P, int flags, int* field_i
   f->autonomous = autonomous;
   f->source = source;
   // The recording forward of the primary family is compiled now, so that errors in the
-  // source surface at registration with the compiler's log.
+  // source surface at registration with the compiler's log; a stiff field that reads t also
+  // compiles the Rodas5P forward, the one user of its dfdt.
   const int id = GSX_FIELD_RUNTIME_BASE + static_cast<int>(rt::g_fields.size());
   rt::Module* m = nullptr;
-  const int rc = stiff ? rt::compile(*f, rt::kRb23, "fwd", &m)
-                       : rt::compile(*f, rt::kTsit5, "fwd_rec", &m);
+  int rc = stiff ? rt::compile(*f, rt::kRb23, "fwd", &m)
+                 : rt::compile(*f, rt::kTsit5, "fwd_rec", &m);
   if (rc != GSX_OK) return rt::fail(*m, id);
+  if (stiff && !autonomous) {
+    rc = rt::compile(*f, rt::kRodas5p, "fwd", &m);
+    if (rc != GSX_OK) return rt::fail(*m, id);
+  }
   rt::g_fields.push_back(std::move(f));
   *field_id = id;
   return GSX_OK;

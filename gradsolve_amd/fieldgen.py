@@ -547,6 +547,45 @@ def _inputs(tr: Traced, extra: Dict[str, str]) -> Dict[int, str]:
     return names
 
 
+def _emit_time_derivative(tr: Traced, wrt) -> list:
+    """``dfdt`` (df/dt at (t, y)) and ``dfdt_vjp`` (yb += d/dy ftb . df/dt, pb likewise) of
+    a stiff field that reads t: the df/dt term of the Rodas5P stages
+    (gradsolve/solvers/rodas5p_step.py:119-146) and its adjoint."""
+    g, D = tr.g, tr.dim
+    one = g.const(1.0)
+    ft = [vjp_nodes(g, [tr.out[i]], [one], [tr.t])[0] for i in range(D)]
+    em = _Emitter(_inputs(tr, {}))
+    em.emit([x for x in ft if x is not None])
+    lines = ["  template <class T>",
+             "  GSX_HD static void dfdt(T t, const T* y, const T* p, T* ft, const double*) {",
+             "    (void)t; (void)y; (void)p;"]
+    lines += em.lines
+    lines += [f"    ft[{i}] = {em.ref(x) if x is not None else 'T(0.0)'};"
+              for i, x in enumerate(ft)]
+    lines += ["  }"]
+    roots, seeds = [], []
+    for i, x in enumerate(ft):
+        if x is not None and x.op != "const":
+            roots.append(x)
+            seeds.append(g.var(f"ftb{i}"))
+    fg = vjp_nodes(g, roots, seeds, wrt) if roots else [None] * len(wrt)
+    em = _Emitter(_inputs(tr, {f"ftb{i}": f"ftb[{i}]" for i in range(D)}))
+    em.emit([x for x in fg if x is not None])
+    lines += ["  template <class T>",
+              "  GSX_HD static void dfdt_vjp(T t, const T* y, const T* p, const T* ftb, T* yb, "
+              "T* pb,",
+              "                              const double*) {",
+              "    (void)t; (void)y; (void)p; (void)ftb; (void)yb; (void)pb;"]
+    lines += em.lines
+    for k, gk in enumerate(fg):
+        if gk is None:
+            continue
+        tgt = f"yb[{k}]" if k < D else f"pb[{k - D}]"
+        lines.append(f"    {tgt} += {em.ref(gk)};")
+    lines += ["  }"]
+    return lines
+
+
 def emit_field(tr: Traced, stiff: bool = False) -> str:
     """The ``struct Field`` source for ``gsx_field_register``."""
     g, D = tr.g, tr.dim
@@ -582,9 +621,6 @@ def emit_field(tr: Traced, stiff: bool = False) -> str:
         lines.append(f"    {tgt} += {em.ref(gk)};")
     lines += ["  }"]
     if stiff:
-        if not autonomous:
-            raise UnsupportedRHS("time", "stiff fields must be autonomous (the Rosenbrock23 "
-                                         "kernels take no time derivative)")
         one = g.const(1.0)
         J = []
         for i in range(D):
@@ -612,10 +648,10 @@ def emit_field(tr: Traced, stiff: bool = False) -> str:
         em = _Emitter(_inputs(tr, {f"Jb{k}": f"Jb[{k}]" for k in range(D * D)}))
         em.emit([x for x in jg if x is not None])
         lines += ["  template <class T>",
-                  "  GSX_HD static void jac_vjp(const T* y, const T* p, const T* Jb, T* yb, "
-                  "T* pb,",
+                  "  GSX_HD static void jac_vjp(T t, const T* y, const T* p, const T* Jb, "
+                  "T* yb, T* pb,",
                   "                             const double*) {",
-                  "    (void)y; (void)p; (void)Jb; (void)yb; (void)pb;"]
+                  "    (void)t; (void)y; (void)p; (void)Jb; (void)yb; (void)pb;"]
         lines += em.lines
         for k, gk in enumerate(jg):
             if gk is None:
@@ -623,6 +659,8 @@ def emit_field(tr: Traced, stiff: bool = False) -> str:
             tgt = f"yb[{k}]" if k < D else f"pb[{k - D}]"
             lines.append(f"    {tgt} += {em.ref(gk)};")
         lines += ["  }"]
+        if not autonomous:
+            lines += _emit_time_derivative(tr, wrt)
     lines += ["};", ""]
     return "\n".join(lines)
 

@@ -5,8 +5,9 @@ Mirrors the reference's ``rodas5p_replay`` engine (gradsolve/solvers/rodas5p_rep
 94-251: ``record_rodas5p`` through the device recorder, ``replay_rodas5p_jax`` and its
 ``jax.grad``; dense output by the continuous extension, gradsolve/solvers/dense.py:166-227)
 with the native entry points ``gsx_rodas5p_*``. Fields need an analytic Jacobian (the
-built-in stiff fields, or a field registered with ``stiff=True``) and must be autonomous.
-``max_steps`` defaults to the reference's 50000.
+built-in stiff fields, or a field registered with ``stiff=True``); a field that reads t
+brings its generated df/dt, which the stages carry as the reference's do. ``max_steps``
+defaults to the reference's 50000.
 """
 from __future__ import annotations
 

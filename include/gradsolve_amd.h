@@ -44,15 +44,19 @@ enum gsx_field {
   GSX_FIELD_ROBERTSON = 3,    /* D=3, P=3 (k1, k2, k3); stiff */
   GSX_FIELD_KURAMOTO = 4,     /* D=32, P=32 (omega); aux = K[32*32] row-major */
   GSX_FIELD_VDP = 5,          /* D=2, P=1 (mu) */
-  GSX_FIELD_LORENZ3 = 6       /* D=3, P=3 (sigma, beta, rho) */
+  GSX_FIELD_LORENZ3 = 6,      /* D=3, P=3 (sigma, beta, rho) */
+  GSX_FIELD_FORCED_STIFF = 8  /* D=2, P=3 (a, b, c); stiff, reads t:
+                                 f0 = -a (y0 - y1^2) + 0.4 sin 3t,
+                                 f1 = y0 - y1 - b y1^3 + 0.2 cos t + c sin(t) y0 */
 };
 
 /* Fields registered at run time (gsx_field_register) take ids from GSX_FIELD_RUNTIME_BASE. */
 enum { GSX_FIELD_RUNTIME_BASE = 1000 };
 
-/* gsx_field_register flags: a stiff field (served by the Rosenbrock23 entry points; must be
- * autonomous) and a field whose right-hand side reads t (Tsit5 only; checkpoints then carry
- * t in an extra row, see gsx_ckpt_rows). */
+/* gsx_field_register flags: a stiff field (with its analytic Jacobian; served by the
+ * Rosenbrock23 and Rodas5P entry points) and a field whose right-hand side reads t
+ * (checkpoints then carry t in an extra row, see gsx_ckpt_rows; a stiff one also supplies
+ * df/dt for Rodas5P). */
 enum gsx_field_flags { GSX_FIELD_STIFF = 1, GSX_FIELD_TIME_DEPENDENT = 2 };
 
 /* Arithmetic flags. Default (0): the kernels contract a*b + c into fused multiply-adds.
@@ -179,10 +183,13 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                              const double* ybar, double* mu, double* y0bar, double* pbar,
                              int flags, void* stream);
 
-/* Fused adaptive Rosenbrock23 (ode23s, autonomous) forward + accepted-h record for stiff
- * fields with an analytic Jacobian: W = I - d*h*J(y, p) factorised once per step (LU with
- * partial pivoting in registers) and reused by the three stage solves; I-controller with
- * exponent -1/3. Same argument contract as gsx_tsit5_fwd. Replaces the XLA FFI handler
+/* Fused adaptive Rosenbrock23 (ode23s, autonomous specialisation) forward + accepted-h
+ * record for stiff fields with an analytic Jacobian: W = I - d*h*J(t, y, p) factorised once
+ * per step (LU with partial pivoting in registers) and reused by the three stage solves;
+ * I-controller with exponent -1/3. As in the reference, every stage evaluates the field at the
+ * step's start time t and no df/dt term is added (a field that reads t gets the reference's
+ * autonomous formula; Rodas5P carries the df/dt term). Same argument contract as
+ * gsx_tsit5_fwd. Replaces the XLA FFI handler
  * Rosenbrock23Fwd (gradsolve/cuda/kernels/rosenbrock23_template.cu.in:151-223) and the Warp
  * kernel _rosenbrock23_adaptive (gradsolve/warp/_warp_rosenbrock.py:383-452, driven by
  * warp_rosenbrock._launch, gradsolve/warp/warp_rosenbrock.py:166-218). */
@@ -240,9 +247,12 @@ int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
 
 /* ---- Rodas5P: the stiff high-order lane ------------------------------------------------
  * Steinebach's Rodas5P (order-5(4) Rosenbrock-Wanner pair, 8 stages, order-4 continuous
- * extension) for autonomous fields with an analytic Jacobian (GSX_FIELD_ROBERTSON,
- * GSX_FIELD_LINEAR_DECAY, and fields registered with GSX_FIELD_STIFF): one LU of
- * W = I - h*gamma*J per step in registers, reused by the 8 stage solves. Same argument
+ * extension) for fields with an analytic Jacobian (GSX_FIELD_ROBERTSON,
+ * GSX_FIELD_LINEAR_DECAY, GSX_FIELD_FORCED_STIFF, and fields registered with
+ * GSX_FIELD_STIFF): one LU of W = I - h*gamma*J per step in registers, reused by the 8 stage
+ * solves. A field that reads t adds h^2 g_i df/dt to stage i (g_i = sum_j Gamma_ij,
+ * rodas5p_step.py:119-146) and evaluates stage i at t + c_i h; J and df/dt are taken at the
+ * step's start. Same argument
  * contracts as the Rosenbrock23 entry points. The forward records with the semantics of
  * gradsolve's device recorder for Rodas5P (gradsolve/solvers/record_jax.py:83-132 with
  * RODAS5P_METHOD): status 2 when a step falls below the underflow floor

@@ -127,8 +127,18 @@ bool field_ok(const FieldSpec& fs) {
     case GSO_F_KURAMOTO: return fs.D >= 1 && fs.D <= kMaxD && fs.P >= fs.D && fs.aux != nullptr;
     case GSO_F_VDP: return fs.D == 2 && fs.P >= 1;
     case GSO_F_LINEAR_LADDER: return fs.D >= 1 && fs.D <= kMaxD;
+    case GSO_F_FORCED_STIFF: return fs.D == 2 && fs.P >= 3;
     default: return false;
   }
+}
+
+// Fields whose right-hand side reads t (the Rodas5P stages then carry the df/dt term).
+bool field_time_dependent(const FieldSpec& fs) { return fs.id == GSO_F_FORCED_STIFF; }
+
+// Fields with an analytic Jacobian (field_jac), which the Rosenbrock methods need.
+bool field_has_jac(const FieldSpec& fs) {
+  return fs.id == GSO_F_ROBERTSON || fs.id == GSO_F_LINEAR_DECAY ||
+         fs.id == GSO_F_LINEAR_LADDER || fs.id == GSO_F_FORCED_STIFF;
 }
 
 template <class T>
@@ -190,6 +200,13 @@ void field_eval(const FieldSpec& fs, T t, const T* y, const double* p, T* dy) {
     }
     case GSO_F_LINEAR_LADDER: {
       for (int i = 0; i < fs.D; ++i) dy[i] = T(1.01) * y[i];
+      break;
+    }
+    case GSO_F_FORCED_STIFF: {
+      const T a = p[0], b = p[1], c = p[2];
+      const T st = sin_(t);
+      dy[0] = -a * (y[0] - y[1] * y[1]) + T(0.4) * sin_(T(3.0) * t);
+      dy[1] = y[0] - y[1] - b * y[1] * y[1] * y[1] + T(0.2) * cos_(t) + c * st * y[0];
       break;
     }
     default: break;
@@ -285,6 +302,17 @@ void field_vjp(const FieldSpec& fs, T t, const T* y, const double* p, const T* v
       for (int i = 0; i < fs.D; ++i) ybar[i] = ybar[i] + T(1.01) * v[i];
       break;
     }
+    case GSO_F_FORCED_STIFF: {
+      const T a = p[0], b = p[1];
+      const T st = sin_(t);
+      // J = [[-a, 2 a y1], [1 + c sin t, -1 - 3 b y1^2]]
+      ybar[0] = ybar[0] + (-a * v[0] + (T(1.0) + T(p[2]) * st) * v[1]);
+      ybar[1] = ybar[1] + (T(2.0) * a * y[1] * v[0] - (T(1.0) + T(3.0) * b * y[1] * y[1]) * v[1]);
+      pbar[0] = pbar[0] - (y[0] - y[1] * y[1]) * v[0];
+      pbar[1] = pbar[1] - y[1] * y[1] * y[1] * v[1];
+      pbar[2] = pbar[2] + st * y[0] * v[1];
+      break;
+    }
     default: break;
   }
 }
@@ -293,7 +321,6 @@ void field_vjp(const FieldSpec& fs, T t, const T* y, const double* p, const T* v
 // Robertson: _warp_rosenbrock.py:132-149 / gradsolve_g2_spike_oracle.py:53-69.
 template <class T>
 bool field_jac(const FieldSpec& fs, T t, const T* y, const double* p, T* J) {
-  (void)t;
   const int D = fs.D;
   switch (fs.id) {
     case GSO_F_ROBERTSON: {
@@ -314,6 +341,15 @@ bool field_jac(const FieldSpec& fs, T t, const T* y, const double* p, T* J) {
       for (int i = 0; i < D; ++i) J[i * D + i] = 1.01;
       return true;
     }
+    case GSO_F_FORCED_STIFF: {
+      const T a = p[0], b = p[1], c = p[2];
+      const T y1 = y[1];
+      J[0] = -a;
+      J[1] = T(2.0) * a * y1;
+      J[2] = T(1.0) + c * sin_(t);
+      J[3] = -T(1.0) - T(3.0) * b * y1 * y1;
+      return true;
+    }
     default: return false;
   }
 }
@@ -321,9 +357,20 @@ bool field_jac(const FieldSpec& fs, T t, const T* y, const double* p, T* J) {
 // ybar += d/dy [ sum_ab Jbar_ab J_ab(y, p) ], pbar += d/dp [ same ] (second-derivative terms
 // of the Rosenbrock adjoint, which differentiates through W = I - d h J(y, p)).
 template <class T>
-void field_jac_vjp(const FieldSpec& fs, const T* y, const double* p, const T* Jb, T* ybar,
+void field_jac_vjp(const FieldSpec& fs, T t, const T* y, const double* p, const T* Jb, T* ybar,
                    T* pbar) {
   switch (fs.id) {
+    case GSO_F_FORCED_STIFF: {
+      const T a = p[0], b = p[1];
+      const T y1 = y[1];
+      // dJ/dy1: J01 = 2 a, J11 = -6 b y1 ; dJ/da: J00 = -1, J01 = 2 y1 ; dJ/db: J11 = -3 y1^2 ;
+      // dJ/dc: J10 = sin t
+      ybar[1] = ybar[1] + (T(2.0) * a * Jb[1] - T(6.0) * b * y1 * Jb[3]);
+      pbar[0] = pbar[0] + (T(2.0) * y1 * Jb[1] - Jb[0]);
+      pbar[1] = pbar[1] - T(3.0) * y1 * y1 * Jb[3];
+      pbar[2] = pbar[2] + sin_(t) * Jb[2];
+      break;
+    }
     case GSO_F_ROBERTSON: {
       const T k2 = p[1], k3 = p[2];
       const T y1 = y[1], y2 = y[2];
@@ -341,6 +388,37 @@ void field_jac_vjp(const FieldSpec& fs, const T* y, const double* p, const T* Jb
       T tr = 0.0;
       for (int i = 0; i < fs.D; ++i) tr = tr + Jb[i * fs.D + i];
       pbar[0] = pbar[0] - tr;
+      break;
+    }
+    default: break;
+  }
+}
+
+// df/dt at (t, y) for the time-dependent fields (the Rodas5P stages' time-derivative term).
+template <class T>
+void field_dfdt(const FieldSpec& fs, T t, const T* y, const double* p, T* ft) {
+  switch (fs.id) {
+    case GSO_F_FORCED_STIFF: {
+      const T c = p[2];
+      ft[0] = T(1.2) * cos_(T(3.0) * t);
+      ft[1] = -T(0.2) * sin_(t) + c * cos_(t) * y[0];
+      break;
+    }
+    default:
+      for (int i = 0; i < fs.D; ++i) ft[i] = T(0.0);
+      break;
+  }
+}
+
+// ybar += (d(df/dt)/dy)^T ftb, pbar += (d(df/dt)/dp)^T ftb.
+template <class T>
+void field_dfdt_vjp(const FieldSpec& fs, T t, const T* y, const double* p, const T* ftb,
+                    T* ybar, T* pbar) {
+  switch (fs.id) {
+    case GSO_F_FORCED_STIFF: {
+      const T ct = cos_(t);
+      ybar[0] = ybar[0] + T(p[2]) * ct * ftb[1];
+      pbar[2] = pbar[2] + ct * y[0] * ftb[1];
       break;
     }
     default: break;
@@ -657,17 +735,19 @@ void rb23_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T
   const T dh = h * T(RB_D);
   T Jb[kMaxD * kMaxD];
   for (int i = 0; i < D * D; ++i) Jb[i] = -dh * Wb[i];
-  field_jac_vjp(fs, y, p, Jb, ybar, pbar);
+  field_jac_vjp(fs, t, y, p, Jb, ybar, pbar);
   for (int i = 0; i < D; ++i) lam[i] = ybar[i];
 }
 
 // ---------------------------------------------------------------------------------------
-// Rodas5P (gradsolve/solvers/rodas5p_step.py:119-168), autonomous fields. The alpha/Gamma
-// tableau is the library's generated header (gsx_rodas5p_tab.h: NumPy's conversion of the
-// published a/C form, rodas5p_step.py:63-75); it is pinned separately by its generator test,
-// the published row sums and the observed orders.
+// Rodas5P (gradsolve/solvers/rodas5p_step.py:119-168). The alpha/Gamma tableau is the
+// library's generated header (gsx_rodas5p_tab.h: NumPy's conversion of the published a/C
+// form, rodas5p_step.py:63-77); it is pinned separately by its generator test, the published
+// row sums, the (a, C)-form stage vectors and the observed orders.
+//   J = df/dy(t, y), f_t = df/dt(t, y) (zero for fields that do not read t);
 //   W = I - (h gamma) J;  for i = 0..7:  u_i = y + sum_{j<i} alpha_ij k_j,
-//   a_i = 0 + sum_{j<i} Gamma_ij k_j,  k_i = W \ (h f(u_i) + h (J a_i));
+//   a_i = 0 + sum_{j<i} Gamma_ij k_j,
+//   k_i = W \ (h f(t + c_i h, u_i) + h (J a_i) + (h h g_i) f_t),  g_i = sum_j Gamma_ij;
 //   y_next = y + sum_i m_i k_i,  err vector = 0 + sum_i (m_i - mhat_i) k_i.
 // ---------------------------------------------------------------------------------------
 namespace r5 = gsx_rodas5p_tab;
@@ -699,6 +779,9 @@ void r5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, R5Sta
     for (int j = 0; j < D; ++j)
       w.lu.a[i * D + j] = (i == j ? T(1.0) : T(0.0)) - hg * w.J[i * D + j];
   lu_factor(w.lu);
+  const bool td = field_time_dependent(fs);
+  T ft[kMaxD];
+  if (td) field_dfdt(fs, t, y, p, ft);
   for (int i = 0; i < 8; ++i) {
     T u[kMaxD], acc[kMaxD], F[kMaxD], rhs[kMaxD];
     r5_stage_input(D, y, w.k, i, u, acc);
@@ -707,6 +790,7 @@ void r5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, R5Sta
       T ja = T(0.0);
       for (int c = 0; c < D; ++c) ja = ja + w.J[r * D + c] * acc[c];
       rhs[r] = h * F[r] + h * ja;
+      if (td) rhs[r] = rhs[r] + (h * h * T(r5::kGammaSum[i])) * ft[r];
     }
     lu_solve(w.lu, rhs, w.k[i]);
   }
@@ -777,11 +861,16 @@ void r5_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* 
       if (dbar)
         for (int r = 0; r < 3; ++r) kb[i][d] = kb[i][d] + T(r5::kHD[r * 8 + i]) * dbar[r][d];
     }
+  const bool td = field_time_dependent(fs);
+  T ftb[kMaxD];
   for (int i = 0; i < D * D; ++i) { Wb[i] = T(0.0); Jb[i] = T(0.0); }
+  for (int d = 0; d < D; ++d) ftb[d] = T(0.0);
   for (int i = 7; i >= 0; --i) {
     T u[kMaxD], acc[kMaxD], rb[kMaxD], Fb[kMaxD], ub[kMaxD], accb[kMaxD];
     r5_stage_input(D, y, w.k, i, u, acc);
     lu_solve_t(w.lu, kb[i], rb);  // k_i = W^-1 rhs_i
+    if (td)
+      for (int d = 0; d < D; ++d) ftb[d] = ftb[d] + (h * h * T(r5::kGammaSum[i])) * rb[d];
     for (int a = 0; a < D; ++a)
       for (int b = 0; b < D; ++b) Wb[a * D + b] = Wb[a * D + b] - rb[a] * w.k[i][b];
     for (int d = 0; d < D; ++d) { Fb[d] = h * rb[d]; ub[d] = T(0.0); }
@@ -801,7 +890,8 @@ void r5_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* 
   }
   const T hg = h * T(r5::kGamma);
   for (int i = 0; i < D * D; ++i) Jb[i] = Jb[i] - hg * Wb[i];
-  field_jac_vjp(fs, y, p, Jb, yb, pbar);
+  field_jac_vjp(fs, t, y, p, Jb, yb, pbar);
+  if (td) field_dfdt_vjp(fs, t, y, p, ftb, yb, pbar);
   for (int d = 0; d < D; ++d) lam[d] = yb[d];
 }
 
@@ -930,9 +1020,7 @@ int record_ensemble(int field, int64_t n, int D, int P, const double* aux, const
                     int64_t cap, int nthreads) {
   FieldSpec fs{field, D, P, aux};
   if (!field_ok(fs) || n < 0 || (tape && cap < 0)) return -1;
-  if (M == kRb23 && field != GSO_F_ROBERTSON && field != GSO_F_LINEAR_DECAY &&
-      field != GSO_F_LINEAR_LADDER)
-    return -2;
+  if (M == kRb23 && !field_has_jac(fs)) return -2;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j) {
     const double t1j = t1[j];
@@ -1242,8 +1330,7 @@ int gso_rodas5p_record(int field, int64_t n, int D, int P, const double* aux,
                        double* tape, int64_t cap, int nthreads) {
   FieldSpec fs{field, D, P, aux};
   if (!field_ok(fs) || n < 0 || (tape && cap < 0)) return -1;
-  if (field != GSO_F_ROBERTSON && field != GSO_F_LINEAR_DECAY && field != GSO_F_LINEAR_LADDER)
-    return -2;
+  if (!field_has_jac(fs)) return -2;
   const double eps = std::numeric_limits<double>::epsilon();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t j = 0; j < n; ++j) {
@@ -1426,6 +1513,19 @@ int gso_rodas5p_tableau(double* alpha, double* gamma_off, double* c, double* d, 
   std::memcpy(e, r5::kE, sizeof r5::kE);
   std::memcpy(hd, r5::kHD, sizeof r5::kHD);
   *gamma = r5::kGamma;
+  return 0;
+}
+
+int gso_rodas5p_stages(int field, int D, int P, const double* aux, double t, const double* y,
+                       double h, const double* p, double* k) {
+  FieldSpec fs{field, D, P, aux};
+  if (!field_ok(fs) || D > kMaxD) return -1;
+  double J[kMaxD * kMaxD];
+  if (!field_jac(fs, t, y, p, J)) return -1;
+  R5Stages<double> w;
+  r5_stages(fs, t, y, h, p, w);
+  for (int i = 0; i < 8; ++i)
+    for (int d = 0; d < D; ++d) k[i * D + d] = w.k[i][d];
   return 0;
 }
 

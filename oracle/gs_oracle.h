@@ -37,7 +37,12 @@ enum {
   GSO_F_KURAMOTO = 4,      /* any D, P=D (omega); aux = K[D*D] row-major */
   GSO_F_VDP = 5,           /* D=2, P=1 (mu) (_warp_kernel.py:158-174) */
   GSO_F_LORENZ3 = 6,       /* D=3, P=3 (sigma, beta, rho) (warp_ode.py:552-589 columns) */
-  GSO_F_LINEAR_LADDER = 7  /* any D, P ignored: f = 1.01*y (_warp_kernel.py:133-155) */
+  GSO_F_LINEAR_LADDER = 7, /* any D, P ignored: f = 1.01*y (_warp_kernel.py:133-155) */
+  GSO_F_FORCED_STIFF = 8   /* D=2, P=3 (a, b, c), time-dependent, stiff for large a:
+                              f0 = -a (y0 - y1^2) + 0.4 sin 3t,
+                              f1 = y0 - y1 - b y1^3 + 0.2 cos t + c sin(t) y0;
+                              at (a, b, c) = (30, 0.7, 0) the field of
+                              tests/test_rodas5p_dense.py:94-103 */
 };
 
 /* Tsitouras (2011) tableau as the oracle holds it: c2..c6, a21..a65 (15), b1..b6, e1..e7. */
@@ -143,6 +148,9 @@ int gso_rodas5p_adjoint_saveat_dense(int field, int64_t n, int D, int P, const d
                                      double* pbar, int nthreads);
 int gso_rodas5p_tableau(double* alpha, double* gamma_off, double* c, double* d, double* m,
                         double* e, double* hd, double* gamma);
+/* The eight stage vectors k [8][D] of one Rodas5P step from (t, y) with step h. */
+int gso_rodas5p_stages(int field, int D, int P, const double* aux, double t, const double* y,
+                       double h, const double* p, double* k);
 
 /* Adaptive Rosenbrock23 forward + record (gradsolve_g2_spike_oracle.py:138-184;
  * _warp_rosenbrock.py:383-452), W = I - d*h*J factored once per step. */

@@ -24,6 +24,7 @@ KURAMOTO = 4
 VDP = 5
 LORENZ3 = 6
 LINEAR_LADDER = 7
+FORCED_STIFF = 8
 
 _lib = None
 
@@ -78,6 +79,7 @@ def _declare(L):
     L.gso_rodas5p_replay_saveat_dense.argtypes = rsv
     L.gso_rodas5p_adjoint_saveat_dense.argtypes = asv
     L.gso_rodas5p_tableau.argtypes = [_dp] * 8
+    L.gso_rodas5p_stages.argtypes = [i, i, i, _dp, d, _dp, d, _dp, _dp]
     L.gso_tsit5_teacher_forced.argtypes = [i, i64, i, i, _dp, _dp, _dp, d, _dp, d, d, _dp, i64,
                                            _i64p, _dp, _i64p, _i64p, i]
     L.gso_tsit5_teacher_forced.restype = ctypes.c_int
@@ -90,7 +92,7 @@ def _declare(L):
                  "gso_rb23_adjoint_saveat", "gso_rodas5p_record", "gso_rodas5p_replay",
                  "gso_rodas5p_adjoint", "gso_rodas5p_replay_saveat", "gso_rodas5p_adjoint_saveat",
                  "gso_rodas5p_replay_saveat_dense", "gso_rodas5p_adjoint_saveat_dense",
-                 "gso_rodas5p_tableau", "gso_field",
+                 "gso_rodas5p_tableau", "gso_rodas5p_stages", "gso_field",
                  "gso_field_vjp", "gso_count_ops"):
         getattr(L, name).restype = ctypes.c_int
 
@@ -212,6 +214,17 @@ def rodas5p_tableau():
     _check(lib().gso_rodas5p_tableau(_ptr(a), _ptr(g), _ptr(c), _ptr(d), _ptr(m), _ptr(e),
                                      _ptr(hd), _ptr(gam)), "rodas5p_tableau")
     return a.reshape(8, 8), g.reshape(8, 8), c, d, m, e, hd.reshape(3, 8), float(gam[0])
+
+
+def rodas5p_stages(field, t, y, h, params, *, aux=None):
+    """The stage vectors k [8, D] of one step of size h from (t, y)."""
+    y = np.ascontiguousarray(y, np.float64)
+    p = np.ascontiguousarray(params, np.float64)
+    k = np.zeros((8, y.shape[0]))
+    a = None if aux is None else np.ascontiguousarray(aux, np.float64)
+    _check(lib().gso_rodas5p_stages(field, y.shape[0], p.shape[0], _ptr(a), float(t), _ptr(y),
+                                    float(h), _ptr(p), _ptr(k)), "rodas5p_stages")
+    return k
 
 
 def rodas5p_replay(field, y0, params, t0, tape, *, aux=None, nthreads=None):

@@ -147,6 +147,7 @@ int lh_rb23_fwd(int field, int64_t n, const double* y0, const double* p, double 
   FwdArgs a{n, y0, p, nullptr, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
             status, tape, tape ? cap : 0, ckpt, ckpt ? C : 1};
   if (field == 3) return run_rb_fwd<FRobertson>(a);
+  if (field == 8) return run_rb_fwd<FForcedStiff>(a);
   if (field == 2) return run_rb_fwd<FLinearDecay<2>>(a);
   return -1;
 }
@@ -155,6 +156,7 @@ int lh_rb23_replay(int field, int64_t n, const double* y0, const double* p, doub
                    int32_t C) {
   ReplayArgs a{n, y0, p, nullptr, t0, tape, n_acc, y_final, ckpt, ckpt ? C : 1};
   if (field == 3) return run_rb_replay<FRobertson>(a);
+  if (field == 8) return run_rb_replay<FForcedStiff>(a);
   if (field == 2) return run_rb_replay<FLinearDecay<2>>(a);
   return -1;
 }
@@ -163,6 +165,7 @@ int lh_rb23_adjoint(int field, int64_t n, const double* p, double t0, const doub
                     const double* ybar, double* y0bar, double* pbar) {
   AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
   if (field == 3) return run_rb_adj<FRobertson>(a);
+  if (field == 8) return run_rb_adj<FForcedStiff>(a);
   if (field == 2) return run_rb_adj<FLinearDecay<2>>(a);
   return -1;
 }
@@ -172,6 +175,7 @@ int lh_rb23_saveat(int field, int64_t n, const double* y0, const double* p, doub
   SaveArgs a{n, y0, p, nullptr, t0, tape, n_acc, ts, k, ys, save_step, save_t, states,
              nullptr};
   if (field == 3) return run_rb_saveat<FRobertson>(a);
+  if (field == 8) return run_rb_saveat<FForcedStiff>(a);
   if (field == 2) return run_rb_saveat<FLinearDecay<2>>(a);
   return -1;
 }
@@ -185,38 +189,67 @@ int lh_rb23_adjoint_saveat(int field, int64_t n, const double* p, double t0,
                 y0bar, pbar};
   AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
   if (field == 3) return run_rb_adj_saveat<FRobertson>(v, a);
+  if (field == 8) return run_rb_adj_saveat<FForcedStiff>(v, a);
   if (field == 2) return run_rb_adj_saveat<FLinearDecay<2>>(v, a);
   return -1;
 }
+}  // extern "C"
+namespace {
+template <class F>
+int r5_fwd_t(const FwdArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) r5_fwd_lane<F>(a, j);
+  return 0;
+}
+template <class F>
+int r5_replay_t(const ReplayArgs& a) {
+  for (int64_t j = 0; j < a.n; ++j) r5_replay_lane<F>(a, j);
+  return 0;
+}
+template <class F>
+int r5_saveat_t(const SaveArgs& a, int dense) {
+  for (int64_t j = 0; j < a.n; ++j) {
+    if (dense)
+      r5_saveat_dense_lane<F>(a, j);
+    else
+      r5_saveat_lane<F>(a, j);
+  }
+  return 0;
+}
+template <class F>
+int r5_adjoint_t(AdjArgs a, const SaveVjpArgs* v) {
+  if (v) {
+    for (int64_t j = 0; j < a.n; ++j) r5_saveat_vjp_lane<F>(*v, j);
+  }
+  return run_r5_adj<F>(a);
+}
+}  // namespace
+extern "C" {
+// Rodas5P fields of the harness: 3 Robertson, 8 the forced stiff field (reads t).
+#define LH_R5(CALL, ...)                                          \
+  if (field == 3) return CALL<FRobertson>(__VA_ARGS__);           \
+  if (field == 8) return CALL<FForcedStiff>(__VA_ARGS__);         \
+  return -1;
+
 int lh_r5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
               double rtol, double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
               int32_t* n_rej, int32_t* status, double* tape, int64_t cap, double* ckpt,
               int32_t C) {
   FwdArgs a{n, y0, p, nullptr, t0, t1, nullptr, rtol, atol, max_steps, y_final, n_acc, n_rej,
             status, tape, tape ? cap : 0, ckpt, ckpt ? C : 1};
-  if (field == 3) { for (int64_t j = 0; j < n; ++j) r5_fwd_lane<FRobertson>(a, j); return 0; }
-  if (field == 2) { for (int64_t j = 0; j < n; ++j) r5_fwd_lane<FLinearDecay<1>>(a, j); return 0; }
-  return -1;
+  if (field == 2) return r5_fwd_t<FLinearDecay<1>>(a);
+  LH_R5(r5_fwd_t, a)
 }
 int lh_r5_replay(int field, int64_t n, const double* y0, const double* p, double t0,
                  const double* tape, const int32_t* n_acc, double* y_final) {
   ReplayArgs a{n, y0, p, nullptr, t0, tape, n_acc, y_final, nullptr, 1};
-  if (field == 3) { for (int64_t j = 0; j < n; ++j) r5_replay_lane<FRobertson>(a, j); return 0; }
-  return -1;
+  LH_R5(r5_replay_t, a)
 }
 // dense: 0 re-step saveat, 1 the continuous extension
 int lh_r5_saveat(int field, int dense, int64_t n, const double* y0, const double* p, double t0,
                  const double* tape, const int32_t* n_acc, const double* ts, int64_t k,
                  double* ys, int32_t* save_step, double* save_t, double* states) {
   SaveArgs a{n, y0, p, nullptr, t0, tape, n_acc, ts, k, ys, save_step, save_t, states, nullptr};
-  if (field != 3) return -1;
-  for (int64_t j = 0; j < n; ++j) {
-    if (dense)
-      r5_saveat_dense_lane<FRobertson>(a, j);
-    else
-      r5_saveat_lane<FRobertson>(a, j);
-  }
-  return 0;
+  LH_R5(r5_saveat_t, a, dense)
 }
 // The gradient as gsx_rodas5p_adjoint[_saveat[_dense]] runs it (k = 0: final state only).
 int lh_r5_adjoint(int field, int dense, int64_t n, const double* p, double t0,
@@ -224,8 +257,10 @@ int lh_r5_adjoint(int field, int dense, int64_t n, const double* p, double t0,
                   int32_t C, const double* ts, int64_t k, const int32_t* save_step,
                   const double* save_t, const double* states, const double* ysbar,
                   const double* ybar, double* mu, double* y0bar, double* pbar) {
-  if (field != 3) return -1;
   AdjArgs a{n, nullptr, p, nullptr, t0, tape, cap, n_acc, ckpt, C, ybar, y0bar, pbar};
+  SaveVjpArgs v{n, p, nullptr, n_acc, ts, k, save_step, save_t, states, ysbar, ybar, mu,
+                y0bar, pbar};
+  const SaveVjpArgs* vp = nullptr;
   if (k > 0 && dense) {
     a.nsave = k;
     a.save_step = save_step;
@@ -233,17 +268,16 @@ int lh_r5_adjoint(int field, int dense, int64_t n, const double* p, double t0,
     a.save_t = save_t;
     a.ysbar = ysbar;
   } else if (k > 0) {
-    SaveVjpArgs v{n, p, nullptr, n_acc, ts, k, save_step, save_t, states, ysbar, ybar, mu,
-                  y0bar, pbar};
-    for (int64_t j = 0; j < n; ++j) r5_saveat_vjp_lane<FRobertson>(v, j);
+    vp = &v;
     a.ybar = y0bar;
     a.nsave = k;
     a.save_step = save_step;
     a.mu = mu;
     a.pbar_in = pbar;
   }
-  return run_r5_adj<FRobertson>(a);
+  LH_R5(r5_adjoint_t, a, vp)
 }
+#undef LH_R5
 int lh_tsit5_fwd(int field, int64_t n, const double* y0, const double* p, double t0, double t1,
                  const double* t1_per, double rtol, double atol, int64_t max_steps,
                  double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,

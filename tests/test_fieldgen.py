@@ -88,11 +88,17 @@ class HostField:
                      'double* yb, double* pb) '
                      '{ gsx_user::Field::vjp<double>(t, y, p, v, yb, pb, nullptr); }\n')
             if stiff:
-                fh.write('void hf_jac(const double* y, const double* p, double* J) '
-                         '{ gsx_user::Field::jac<double>(0.0, y, p, J, nullptr); }\n'
-                         'void hf_jac_vjp(const double* y, const double* p, const double* Jb, '
-                         'double* yb, double* pb) '
-                         '{ gsx_user::Field::jac_vjp<double>(y, p, Jb, yb, pb, nullptr); }\n')
+                fh.write('void hf_jac(double t, const double* y, const double* p, double* J) '
+                         '{ gsx_user::Field::jac<double>(t, y, p, J, nullptr); }\n'
+                         'void hf_jac_vjp(double t, const double* y, const double* p, '
+                         'const double* Jb, double* yb, double* pb) '
+                         '{ gsx_user::Field::jac_vjp<double>(t, y, p, Jb, yb, pb, nullptr); }\n')
+            if stiff and tr.time_dependent:
+                fh.write('void hf_dfdt(double t, const double* y, const double* p, double* ft) '
+                         '{ gsx_user::Field::dfdt<double>(t, y, p, ft, nullptr); }\n'
+                         'void hf_dfdt_vjp(double t, const double* y, const double* p, '
+                         'const double* ftb, double* yb, double* pb) '
+                         '{ gsx_user::Field::dfdt_vjp<double>(t, y, p, ftb, yb, pb, nullptr); }\n')
             fh.write("}\n")
         so = os.path.join(self.dir, "h.so")
         subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
@@ -116,15 +122,29 @@ class HostField:
                       yb.ctypes.data_as(ctypes.c_void_p), pb.ctypes.data_as(ctypes.c_void_p))
         return yb, pb
 
-    def jac(self, y, p):
+    def jac(self, y, p, t=0.0):
         J = np.zeros(self.D * self.D)
-        self.L.hf_jac(self._p(y), self._p(p), J.ctypes.data_as(ctypes.c_void_p))
+        self.L.hf_jac(ctypes.c_double(t), self._p(y), self._p(p),
+                      J.ctypes.data_as(ctypes.c_void_p))
         return J.reshape(self.D, self.D)
 
-    def jac_vjp(self, y, p, Jb):
+    def jac_vjp(self, y, p, Jb, t=0.0):
         yb, pb = np.zeros(self.D), np.zeros(self.P)
-        self.L.hf_jac_vjp(self._p(y), self._p(p), self._p(Jb),
+        self.L.hf_jac_vjp(ctypes.c_double(t), self._p(y), self._p(p), self._p(Jb),
                           yb.ctypes.data_as(ctypes.c_void_p), pb.ctypes.data_as(ctypes.c_void_p))
+        return yb, pb
+
+    def dfdt(self, t, y, p):
+        ft = np.zeros(self.D)
+        self.L.hf_dfdt(ctypes.c_double(t), self._p(y), self._p(p),
+                       ft.ctypes.data_as(ctypes.c_void_p))
+        return ft
+
+    def dfdt_vjp(self, t, y, p, ftb):
+        yb, pb = np.zeros(self.D), np.zeros(self.P)
+        self.L.hf_dfdt_vjp(ctypes.c_double(t), self._p(y), self._p(p), self._p(ftb),
+                           yb.ctypes.data_as(ctypes.c_void_p),
+                           pb.ctypes.data_as(ctypes.c_void_p))
         return yb, pb
 
 
@@ -223,9 +243,48 @@ def test_wrong_output_length_raises():
     assert ei.value.primitive_name == "shape"
 
 
-def test_stiff_field_must_be_autonomous():
-    with pytest.raises(UnsupportedRHS):
-        FG.emit_field(FG.trace(forced_pendulum, 2, 2), stiff=True)
+def forced_stiff(t, y, p):
+    a, b, c = p[0], p[1], p[2]
+    return np.stack([-a * (y[0] - y[1] * y[1]) + 0.4 * np.sin(3.0 * t),
+                     y[0] - y[1] - b * y[1] ** 3 + 0.2 * np.cos(t) + c * np.sin(t) * y[0]])
+
+
+def _cs_t(f, t, y, p):
+    """Complex-step df/dt of the Python right-hand side."""
+    h = 1e-30
+    return np.imag(np.asarray(f(complex(t, h), y.astype(complex), p.astype(complex)),
+                              dtype=complex)) / h
+
+
+def test_time_dependent_stiff_field_jacobian_and_time_derivative():
+    """A stiff field that reads t: J(t) against the complex-step Jacobian, df/dt against the
+    complex step in t, and the VJPs of both (jac_vjp, dfdt_vjp) against central differences of
+    the generated J and df/dt themselves."""
+    hf = HostField(forced_stiff, 2, 3, stiff=True)
+    assert hf.time_dependent
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        t = rng.uniform(0.0, 5.0)
+        y = rng.uniform(0.1, 1.0, 2)
+        p = np.array([rng.uniform(10.0, 500.0), rng.uniform(0.3, 0.9), rng.uniform(-1.0, 1.0)])
+        J = hf.jac(y, p, t)
+        np.testing.assert_allclose(J, cs_jac(forced_stiff, t, y, p, "y"), rtol=1e-14,
+                                   atol=1e-14 * np.abs(J).max())
+        ft = hf.dfdt(t, y, p)
+        np.testing.assert_allclose(ft, _cs_t(forced_stiff, t, y, p), rtol=1e-14, atol=1e-15)
+        Jb = rng.standard_normal(4)
+        ftb = rng.standard_normal(2)
+        for fn, vjp, seed in ((lambda yy, pp: hf.jac(yy, pp, t).reshape(-1),
+                               lambda: hf.jac_vjp(y, p, Jb, t), Jb),
+                              (lambda yy, pp: hf.dfdt(t, yy, pp),
+                               lambda: hf.dfdt_vjp(t, y, p, ftb), ftb)):
+            yb, pb = vjp()
+            fd_y = np.array([np.dot(seed, fn(y + e, p) - fn(y - e, p)) / 2e-6
+                             for e in np.eye(2) * 1e-6])
+            fd_p = np.array([np.dot(seed, fn(y, p + e) - fn(y, p - e)) / (2e-6 * max(1.0, p[k]))
+                             for k, e in enumerate(np.eye(3) * 1e-6 * np.maximum(1.0, p))])
+            np.testing.assert_allclose(yb, fd_y, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(fd_y).max()))
+            np.testing.assert_allclose(pb, fd_p, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(fd_p).max()))
 
 
 def test_library_registers_generated_field_and_reports_compile_errors():
@@ -239,6 +298,11 @@ def test_library_registers_generated_field_and_reports_compile_errors():
     src_t = FG.emit_field(FG.trace(forced_pendulum, 2, 2))
     fid_t = N.field_register(src_t, 2, 2, N.FIELD_TIME_DEPENDENT)
     assert fid_t > fid and N.call("gsx_ckpt_rows", fid_t, 2) == 3
+    # a stiff field that reads t compiles its Rosenbrock23 and Rodas5P forwards (df/dt)
+    src_s = FG.emit_field(FG.trace(forced_stiff, 2, 3), stiff=True)
+    fid_s = N.field_register(src_s, 2, 3, N.FIELD_STIFF | N.FIELD_TIME_DEPENDENT)
+    assert N.call("gsx_ckpt_rows", fid_s, 2) == 3
+    assert N.call("gsx_field_release", fid_s) == 0
     with pytest.raises(ValueError, match="differs from the registered D=4"):
         N.field_register(src, 4, 1, 0)
     with pytest.raises(ValueError, match="error"):

@@ -193,3 +193,100 @@ def test_rodas5p_runtime_field_matches_finite_differences(gpu):
     err = np.abs(g - g_fd) - 4.0 * fd_noise
     for c in range(2):
         assert np.max(err[:, c]) / np.abs(g_fd[:, c]).max() < 1e-6, c
+
+
+# -- a stiff field that reads t (the df/dt term of rodas5p_step.py:119-146) --------------------
+# GSX_FIELD_FORCED_STIFF: f0 = -a (y0 - y1^2) + 0.4 sin 3t, f1 = y0 - y1 - b y1^3 + 0.2 cos t +
+# c sin(t) y0 (at c = 0 the time-dependent stiff field of the reference's
+# tests/test_rodas5p_dense.py:94-103). The device's sin/cos round within 1 ulp of the host's, so
+# the strict variant is compared with tolerances here rather than bit for bit.
+TS_F = np.array([-1.0, 0.0, 1e-5, 0.01, 0.4, 1.3, 1.3, 2.9, 4.99, 5.0, 6.0])
+
+
+def _forced(n, seed):
+    rng = np.random.default_rng(seed)
+    y0 = np.column_stack([rng.uniform(0.2, 0.6, n), rng.uniform(0.4, 0.8, n)])
+    prm = np.column_stack([rng.uniform(20.0, 500.0, n), rng.uniform(0.5, 0.9, n),
+                           rng.uniform(-0.5, 0.5, n)])
+    return y0, prm
+
+
+@pytest.mark.parametrize("strict", [True, False])
+def test_rodas5p_time_dependent_field(gpu, strict):
+    n = 1024
+    y0, prm = _forced(n, 77)
+    print(f"case rodas5p forced_stiff n={n} strict={strict}", flush=True)
+    rec = H.record(N.FIELD_FORCED_STIFF, _soa(y0, gpu), _soa(prm, gpu), t0=0.0, t1=5.0,
+                   rtol=1e-7, atol=1e-9, max_steps=50000, entry="gsx_rodas5p_fwd",
+                   ckpt_interval=8, strict=strict)
+    torch.cuda.synchronize()
+    assert int((rec.status != 0).sum().item()) == 0
+    oyf, _, ona, onr, _ = O.rodas5p_record(O.FORCED_STIFF, y0, prm, 0.0, 5.0, rtol=1e-7,
+                                           atol=1e-9)
+    na, nr = rec.n_acc.cpu().numpy(), rec.n_rej.cpu().numpy()
+    same = (na == ona) & (nr == onr)
+    print(f"count mismatches: {(~same).sum()}; mean accepted {na.mean():.1f}", flush=True)
+    assert (~same).mean() < 1e-2
+    yf = rec.y_final.cpu().numpy().T
+    assert np.max(np.abs(yf[same] - oyf[same])) < 1e-10
+    tape = rec.padded_tape().cpu().numpy()
+    rng = np.random.default_rng(78)
+    ybar = rng.standard_normal((n, 2))
+    y0b = torch.empty((2, n), dtype=torch.float64, device=gpu)
+    pb = torch.empty((3, n), dtype=torch.float64, device=gpu)
+    ybt = torch.as_tensor(ybar.T.copy(), device=gpu)
+    N.call("gsx_rodas5p_adjoint", N.FIELD_FORCED_STIFF, n, 2, 3, None, None, rec.p.data_ptr(),
+           0.0, rec.tape.data_ptr(), rec.cap, rec.n_acc.data_ptr(), rec.ckpt.data_ptr(),
+           rec.ckpt_interval, ybt.data_ptr(), y0b.data_ptr(), pb.data_ptr(),
+           N.STRICT if strict else 0, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    og0, ogp = O.rodas5p_adjoint(O.FORCED_STIFF, y0, prm, 0.0, tape, ybar)
+    g0, gp = y0b.cpu().numpy().T, pb.cpu().numpy().T
+    assert np.max(np.abs(gp - ogp) / np.abs(ogp).max(axis=1, keepdims=True)) < 1e-9
+    assert np.max(np.abs(g0 - og0) / np.abs(og0).max(axis=1, keepdims=True)) < 1e-9
+    ts = torch.as_tensor(TS_F, device=gpu)
+    ysbar = rng.standard_normal((n, TS_F.shape[0], 2))
+    for dense in (False, True):
+        method = "rodas5p_dense" if dense else "rodas5p"
+        sv = H.saveat(rec, ts, strict=strict, method=method)
+        torch.cuda.synchronize()
+        ys = sv.ys.permute(2, 0, 1).cpu().numpy()
+        _, oys = O.rodas5p_replay_saveat(O.FORCED_STIFF, y0, prm, 0.0, tape, TS_F, dense=dense)
+        np.testing.assert_allclose(ys, oys, rtol=1e-11, atol=1e-11 * np.abs(oys).max())
+        yb0, pbs = H.adjoint_saveat(rec, ts, sv,
+                                    torch.as_tensor(ysbar.transpose(1, 2, 0).copy(), device=gpu),
+                                    ybt, strict=strict, method=method)
+        torch.cuda.synchronize()
+        sg0, sgp = O.rodas5p_adjoint_saveat(O.FORCED_STIFF, y0, prm, 0.0, tape, TS_F, ysbar,
+                                            ybar, dense=dense)
+        g0, gp = yb0.cpu().numpy().T, pbs.cpu().numpy().T
+        assert np.max(np.abs(gp - sgp) / np.abs(sgp).max(axis=1, keepdims=True)) < 1e-9, dense
+        assert np.max(np.abs(g0 - sg0) / np.abs(sg0).max(axis=1, keepdims=True)) < 1e-9, dense
+
+
+def forced_stiff(t, y, p):
+    return np.array([-p[0] * (y[0] - y[1] * y[1]) + 0.4 * np.sin(3.0 * t),
+                     y[0] - y[1] - p[1] * y[1] ** 3 + 0.2 * np.cos(t) + p[2] * np.sin(t) * y[0]])
+
+
+def test_time_dependent_stiff_field_compiled_at_run_time(gpu):
+    """register_field(stiff=True) on a right-hand side that reads t: the generated df/dt and its
+    VJP drive the Rodas5P closure, whose gradient matches the oracle's adjoint on the closure's
+    own mesh; Rosenbrock23 (the router's low-dimension stiff lane) runs it with the reference's
+    autonomous formula, also against the oracle."""
+    register_field("r5_forced_stiff", forced_stiff, 2, 3, stiff=True)
+    n = 64
+    y0, prm = _forced(n, 79)
+    rng = np.random.default_rng(80)
+    w = rng.standard_normal((n, 2))
+    for engine, t1, oracle_adj in (("rodas5p_replay", 5.0, O.rodas5p_adjoint),
+                                   ("auto", 0.3, O.rb23_adjoint)):
+        print(f"case runtime forced_stiff engine={engine} n={n}", flush=True)
+        prob = Prob("r5_forced_stiff", 2, True, forced_stiff, t1=t1)
+        clo = gs.grad_closure(prob, y0, prm, engine=engine, rtol=1e-7, atol=1e-9)
+        assert clo.route.actual == ("hip_rodas5p" if engine != "auto" else "hip_rosenbrock23")
+        pt = torch.tensor(prm, dtype=torch.float64, device=gpu, requires_grad=True)
+        (clo(pt) * torch.as_tensor(w, device=gpu)).sum().backward()
+        _, ogp = oracle_adj(O.FORCED_STIFF, y0, prm, 0.0, clo.dts_padded(), w)
+        g = pt.grad.cpu().numpy()
+        assert np.max(np.abs(g - ogp) / np.abs(ogp).max(axis=1, keepdims=True)) < 1e-8, engine

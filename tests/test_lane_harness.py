@@ -291,8 +291,25 @@ def test_saveat_and_its_gradient_match_oracle(lh, C):
     np.testing.assert_allclose(g0.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
 
 
+def _stiff_case(case, n, seed, t1_forced=5.0):
+    """(lane field id, oracle field id, D, checkpoint rows, y0 [n, D], params [n, P], t1) of a
+    stiff harness case: Robertson, or the forced stiff field that reads t (its checkpoints
+    carry t in an extra row). Rosenbrock23 is the autonomous specialisation (no df/dt term,
+    every stage at the step's start time, as the reference's), so on the forced field its
+    error estimate keeps an O(h^2) time term and it needs a shorter span."""
+    if case == "robertson":
+        e = W.robertson(n, seed=seed)
+        return 3, O.ROBERTSON, 3, 3, e.y0, e.params, 20.0
+    rng = np.random.default_rng(seed)
+    y0 = np.column_stack([rng.uniform(0.2, 0.6, n), rng.uniform(0.4, 0.8, n)])
+    prm = np.column_stack([rng.uniform(20.0, 500.0, n), rng.uniform(0.5, 0.9, n),
+                           rng.uniform(-0.5, 0.5, n)])
+    return 8, O.FORCED_STIFF, 2, 3, y0, prm, t1_forced
+
+
+@pytest.mark.parametrize("case", ["robertson", "forced"])
 @pytest.mark.parametrize("C", [1, 4])
-def test_rb23_lanes_and_saveat_match_oracle(lh, C):
+def test_rb23_lanes_and_saveat_match_oracle(lh, C, case):
     """The Rosenbrock23 lane bodies (strict build) against the oracle on Robertson: the record
     (counts, tape and final state bit for bit), the replay, the adjoint (1e-12), the dense
     output of gsx_rb23_saveat bit for bit, and the gradient of gsx_rb23_adjoint_saveat (save
@@ -306,18 +323,19 @@ def test_rb23_lanes_and_saveat_match_oracle(lh, C):
     lh.lh_rb23_saveat.argtypes = [i, i64, vp, vp, d, vp, vp, vp, i64, vp, vp, vp, vp]
     lh.lh_rb23_adjoint_saveat.argtypes = [i, i64, vp, d, vp, i64, vp, vp, i32, vp, i64, vp,
                                           vp, vp, vp, vp, vp, vp, vp]
-    e = W.robertson(12, seed=33)
-    n, cap, t1 = 12, 1024, 20.0
-    y0s = np.ascontiguousarray(e.y0.T)
-    ps = np.ascontiguousarray(e.params.T)
-    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+    fid, oid, D, R, y0, prm, t1 = _stiff_case(case, 12, 33, t1_forced=0.5)
+    P = prm.shape[1]
+    n, cap = 12, 1024
+    y0s = np.ascontiguousarray(y0.T)
+    ps = np.ascontiguousarray(prm.T)
+    yf, na, nr, st = Guarded((D, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
         Guarded((n,), np.int32)
     tape = Guarded((cap, n))
-    ck = Guarded((cap // C + 1, 3, n))
-    lh.lh_rb23_fwd(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, None, 1e-6, 1e-10, 50000,
+    ck = Guarded((cap // C + 1, R, n))
+    lh.lh_rb23_fwd(fid, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, None, 1e-6, 1e-10, 50000,
                    yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C)
     assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
-    oyf, otape, ona, onr, ost = O.rb23_record(O.ROBERTSON, e.y0, e.params, 0.0, t1, rtol=1e-6,
+    oyf, otape, ona, onr, ost = O.rb23_record(oid, y0, prm, 0.0, t1, rtol=1e-6,
                                               atol=1e-10)
     assert np.all(st.a == 0) and np.array_equal(na.a, ona) and np.array_equal(nr.a, onr)
     assert np.array_equal(yf.a.T, oyf)
@@ -325,46 +343,47 @@ def test_rb23_lanes_and_saveat_match_oracle(lh, C):
     for j in range(n):
         padded[j, :na.a[j]] = tape.a[:na.a[j], j]
     assert np.array_equal(padded, otape[:, :padded.shape[1]])
-    ry, rck = Guarded((3, n)), Guarded((cap // C + 1, 3, n))
-    lh.lh_rb23_replay(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr,
+    ry, rck = Guarded((D, n)), Guarded((cap // C + 1, R, n))
+    lh.lh_rb23_replay(fid, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr,
                       rck.ptr, C)
     assert ry.intact() and rck.intact() and np.array_equal(ry.a, yf.a)
-    w = np.ascontiguousarray(np.tile([[1.0], [10.0], [1.0]], (1, n)))
-    gy, gp = Guarded((3, n)), Guarded((3, n))
-    lh.lh_rb23_adjoint(3, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+    w = np.ascontiguousarray(np.tile([[1.0], [10.0], [1.0]][:D], (1, n)))
+    gy, gp = Guarded((D, n)), Guarded((P, n))
+    lh.lh_rb23_adjoint(fid, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
                        w.ctypes.data, gy.ptr, gp.ptr)
     assert gy.intact() and gp.intact()
-    og0, ogp = O.rb23_adjoint(O.ROBERTSON, e.y0, e.params, 0.0, padded, w.T.copy())
+    og0, ogp = O.rb23_adjoint(oid, y0, prm, 0.0, padded, w.T.copy())
     np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=0)
     np.testing.assert_allclose(gy.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
-    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0])
+    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0]) * (t1 / 20.0)
     k = ts.shape[0]
-    ys, sstep, st_t, states = Guarded((k, 3, n)), Guarded((k, n), np.int32), \
-        Guarded((k, n)), Guarded((k, 3, n))
-    lh.lh_rb23_saveat(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
+    ys, sstep, st_t, states = Guarded((k, D, n)), Guarded((k, n), np.int32), \
+        Guarded((k, n)), Guarded((k, D, n))
+    lh.lh_rb23_saveat(fid, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
                       ts.ctypes.data, k, ys.ptr, sstep.ptr, st_t.ptr, states.ptr)
     assert all(g.intact() for g in (ys, sstep, st_t, states))
-    _, oys = O.rb23_replay_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts)
+    _, oys = O.rb23_replay_saveat(oid, y0, prm, 0.0, padded, ts)
     assert np.array_equal(ys.a.transpose(2, 0, 1), oys)
-    assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, e.y0)
+    assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, y0)
     rng = np.random.default_rng(34)
-    ysbar = rng.standard_normal((n, k, 3))
-    ybar = rng.standard_normal((n, 3))
-    mu = Guarded((k, 3, n))
-    g0, gq = Guarded((3, n)), Guarded((3, n))
+    ysbar = rng.standard_normal((n, k, D))
+    ybar = rng.standard_normal((n, D))
+    mu = Guarded((k, D, n))
+    g0, gq = Guarded((D, n)), Guarded((P, n))
     ysb_soa = np.ascontiguousarray(ysbar.transpose(1, 2, 0))  # kept alive across the call
     ybar_soa = np.ascontiguousarray(ybar.T)
-    lh.lh_rb23_adjoint_saveat(3, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+    lh.lh_rb23_adjoint_saveat(fid, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
                               ts.ctypes.data, k, sstep.ptr, st_t.ptr, states.ptr,
                               ysb_soa.ctypes.data, ybar_soa.ctypes.data, mu.ptr, g0.ptr, gq.ptr)
     assert all(g.intact() for g in (mu, g0, gq))
-    sg0, sgp = O.rb23_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts, ysbar, ybar)
+    sg0, sgp = O.rb23_adjoint_saveat(oid, y0, prm, 0.0, padded, ts, ysbar, ybar)
     np.testing.assert_allclose(gq.a.T, sgp, rtol=1e-12, atol=1e-12 * np.abs(sgp).max())
     np.testing.assert_allclose(g0.a.T, sg0, rtol=1e-12, atol=1e-12 * np.abs(sg0).max())
 
 
+@pytest.mark.parametrize("case", ["robertson", "forced"])
 @pytest.mark.parametrize("C", [1, 4])
-def test_rodas5p_lanes_match_oracle(lh, C):
+def test_rodas5p_lanes_match_oracle(lh, C, case):
     """The Rodas5P lane bodies (gsx_rodas5p_*, strict build) against the oracle on Robertson:
     the record (counts, tape, final state) and the replay bit for bit, the re-step and the
     dense saves bit for bit (saves before t0, on the mesh, at t1 and past it), and the
@@ -377,17 +396,18 @@ def test_rodas5p_lanes_match_oracle(lh, C):
     lh.lh_r5_saveat.argtypes = [i, i, i64, vp, vp, d, vp, vp, vp, i64, vp, vp, vp, vp]
     lh.lh_r5_adjoint.argtypes = [i, i, i64, vp, d, vp, i64, vp, vp, i32, vp, i64, vp, vp, vp,
                                  vp, vp, vp, vp, vp]
-    e = W.robertson(12, seed=35)
-    n, cap, t1 = 12, 512, 20.0
-    y0s = np.ascontiguousarray(e.y0.T)
-    ps = np.ascontiguousarray(e.params.T)
-    yf, na, nr, st = Guarded((3, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
+    fid, oid, D, R, y0, prm, t1 = _stiff_case(case, 12, 35)
+    P = prm.shape[1]
+    n, cap = 12, 512
+    y0s = np.ascontiguousarray(y0.T)
+    ps = np.ascontiguousarray(prm.T)
+    yf, na, nr, st = Guarded((D, n)), Guarded((n,), np.int32), Guarded((n,), np.int32), \
         Guarded((n,), np.int32)
-    tape, ck = Guarded((cap, n)), Guarded((cap // C + 1, 3, n))
-    assert lh.lh_r5_fwd(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, 1e-6, 1e-10, 50000,
+    tape, ck = Guarded((cap, n)), Guarded((cap // C + 1, R, n))
+    assert lh.lh_r5_fwd(fid, n, y0s.ctypes.data, ps.ctypes.data, 0.0, t1, 1e-6, 1e-10, 50000,
                         yf.ptr, na.ptr, nr.ptr, st.ptr, tape.ptr, cap, ck.ptr, C) == 0
     assert all(g.intact() for g in (yf, na, nr, st, tape, ck))
-    oyf, otape, ona, onr, ost = O.rodas5p_record(O.ROBERTSON, e.y0, e.params, 0.0, t1,
+    oyf, otape, ona, onr, ost = O.rodas5p_record(oid, y0, prm, 0.0, t1,
                                                  rtol=1e-6, atol=1e-10)
     assert np.all(st.a == 0) and np.array_equal(na.a, ona) and np.array_equal(nr.a, onr)
     assert np.array_equal(yf.a.T, oyf)
@@ -396,39 +416,39 @@ def test_rodas5p_lanes_match_oracle(lh, C):
     for j in range(n):
         padded[j, :na.a[j]] = tape.a[:na.a[j], j]
     assert np.array_equal(padded, otape[:, :S])
-    ry = Guarded((3, n))
-    lh.lh_r5_replay(3, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr)
+    ry = Guarded((D, n))
+    lh.lh_r5_replay(fid, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr, ry.ptr)
     assert ry.intact() and np.array_equal(ry.a, yf.a)
     rng = np.random.default_rng(36)
-    ybar = rng.standard_normal((n, 3))
+    ybar = rng.standard_normal((n, D))
     ybar_soa = np.ascontiguousarray(ybar.T)
-    g0, gp = Guarded((3, n)), Guarded((3, n))
-    lh.lh_r5_adjoint(3, 0, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C, None, 0,
+    g0, gp = Guarded((D, n)), Guarded((P, n))
+    lh.lh_r5_adjoint(fid, 0, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C, None, 0,
                      None, None, None, None, ybar_soa.ctypes.data, None, g0.ptr, gp.ptr)
     assert g0.intact() and gp.intact()
-    og0, ogp = O.rodas5p_adjoint(O.ROBERTSON, e.y0, e.params, 0.0, padded, ybar)
+    og0, ogp = O.rodas5p_adjoint(oid, y0, prm, 0.0, padded, ybar)
     np.testing.assert_allclose(gp.a.T, ogp, rtol=1e-12, atol=1e-12 * np.abs(ogp).max())
     np.testing.assert_allclose(g0.a.T, og0, rtol=1e-12, atol=1e-12 * np.abs(og0).max())
-    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0])
+    ts = np.array([-1.0, 0.0, 1e-5, 0.3, 0.3, 2.0, 7.5, 19.99, 20.0, 25.0]) * (t1 / 20.0)
     k = ts.shape[0]
-    ysbar = rng.standard_normal((n, k, 3))
+    ysbar = rng.standard_normal((n, k, D))
     ysb_soa = np.ascontiguousarray(ysbar.transpose(1, 2, 0))
     for dense in (0, 1):
-        ys, sstep, st_t, states = Guarded((k, 3, n)), Guarded((k, n), np.int32), \
-            Guarded((k, n)), Guarded((k, 3, n))
-        lh.lh_r5_saveat(3, dense, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
+        ys, sstep, st_t, states = Guarded((k, D, n)), Guarded((k, n), np.int32), \
+            Guarded((k, n)), Guarded((k, D, n))
+        lh.lh_r5_saveat(fid, dense, n, y0s.ctypes.data, ps.ctypes.data, 0.0, tape.ptr, na.ptr,
                         ts.ctypes.data, k, ys.ptr, sstep.ptr, st_t.ptr, states.ptr)
         assert all(g.intact() for g in (ys, sstep, st_t))
-        _, oys = O.rodas5p_replay_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts,
+        _, oys = O.rodas5p_replay_saveat(oid, y0, prm, 0.0, padded, ts,
                                          dense=bool(dense))
         assert np.array_equal(ys.a.transpose(2, 0, 1), oys), dense
-        assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, e.y0)
-        mu, h0, hp = Guarded((k, 3, n)), Guarded((3, n)), Guarded((3, n))
-        lh.lh_r5_adjoint(3, dense, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
+        assert np.array_equal(ys.a[-2].T, yf.a.T) and np.array_equal(ys.a[1].T, y0)
+        mu, h0, hp = Guarded((k, D, n)), Guarded((D, n)), Guarded((P, n))
+        lh.lh_r5_adjoint(fid, dense, n, ps.ctypes.data, 0.0, tape.ptr, cap, na.ptr, ck.ptr, C,
                          ts.ctypes.data, k, sstep.ptr, st_t.ptr, states.ptr,
                          ysb_soa.ctypes.data, ybar_soa.ctypes.data, mu.ptr, h0.ptr, hp.ptr)
         assert all(g.intact() for g in (mu, h0, hp))
-        sg0, sgp = O.rodas5p_adjoint_saveat(O.ROBERTSON, e.y0, e.params, 0.0, padded, ts, ysbar,
+        sg0, sgp = O.rodas5p_adjoint_saveat(oid, y0, prm, 0.0, padded, ts, ysbar,
                                             ybar, dense=bool(dense))
         np.testing.assert_allclose(hp.a.T, sgp, rtol=1e-12, atol=1e-12 * np.abs(sgp).max())
         np.testing.assert_allclose(h0.a.T, sg0, rtol=1e-12, atol=1e-12 * np.abs(sg0).max())
