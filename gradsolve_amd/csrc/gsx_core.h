@@ -1048,7 +1048,9 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
 // adjoint needs. A missing second step (odd n_acc) runs with h = 0, which leaves lam and pb
 // unchanged exactly, so every lane follows the same control flow. `m` is wavefront-uniform in
 // the kernel: all active lanes read checkpoint row m and tape rows 2m, 2m+1 together.
-template <class F>
+// kSaves = false drops the save-cotangent injection and pbar_in (the plain final-state
+// gradient), which keeps the lane's register footprint at that of the bare adjoint.
+template <class F, bool kSaves = true>
 GSX_HD void tsit5_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32_t na,
                                    const double* p, double* lam, double* pb, int32_t& qs) {
   constexpr int D = F::D;
@@ -1070,29 +1072,31 @@ GSX_HD void tsit5_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32
     tsit5_stage_inputs<F>(t + h0, y1, h1, p, z1, nullptr, a.aux);
     tsit5_step_vjp_z<F>(t + h0, z1, h1, p, lam, pb, a.aux);
   }
-  if (s0 + 1 < na) inject_saves<D>(a, j, s0 + 1, qs, lam);
+  if constexpr (kSaves) {
+    if (s0 + 1 < na) inject_saves<D>(a, j, s0 + 1, qs, lam);
+  }
   tsit5_step_vjp_z<F>(t, z0, h0, p, lam, pb, a.aux);
-  inject_saves<D>(a, j, s0, qs, lam);
+  if constexpr (kSaves) inject_saves<D>(a, j, s0, qs, lam);
 }
 
 // The pair adjoint of one lane. `mmax` is at least the lane's segment count; the kernel passes
 // the wavefront's largest, so every lane runs the loop in step and idles above its own count.
-template <class F>
+template <class F, bool kSaves = true>
 GSX_HD void tsit5_adj_pair_lane(const AdjArgs& a, int64_t j, int32_t na, int32_t mmax) {
   constexpr int D = F::D, P = F::P;
   const int64_t n = a.n;
   double p[P], pb[P], lam[D];
   GSX_UNROLL for (int q = 0; q < P; ++q) {
     p[q] = a.p[q * n + j];
-    pb[q] = a.pbar_in ? a.pbar_in[q * n + j] : 0.0;
+    pb[q] = (kSaves && a.pbar_in) ? a.pbar_in[q * n + j] : 0.0;
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
-  int32_t qs = static_cast<int32_t>(a.nsave) - 1;
+  int32_t qs = kSaves ? static_cast<int32_t>(a.nsave) - 1 : -1;
   const int32_t nseg = (na + 1) / 2;
   for (int32_t m = mmax - 1; m >= 0; --m) {
-    if (m < nseg) tsit5_adj_pair_segment<F>(a, j, m, na, p, lam, pb, qs);
+    if (m < nseg) tsit5_adj_pair_segment<F, kSaves>(a, j, m, na, p, lam, pb, qs);
   }
-  inject_saves<D>(a, j, -1, qs, lam);
+  if constexpr (kSaves) inject_saves<D>(a, j, -1, qs, lam);
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }

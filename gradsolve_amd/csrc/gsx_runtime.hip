@@ -53,14 +53,16 @@ const FamilySpec& spec(Family f) {
       {"gsx", "gsx_tsit5_kernels.h", {},
        {{"fwd_rec", "tsit5_fwd_kernel<%s, true>"}, {"fwd", "tsit5_fwd_kernel<%s, false>"},
         {"replay", "tsit5_replay_kernel<%s>"}, {"adj1", "tsit5_adj_kernel<%s, 1>"},
-        {"adj_pair", "tsit5_adj_pair_kernel<%s>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
+        {"adj_pair", "tsit5_adj_pair_kernel<%s, true>"},
+        {"adj_pair_plain", "tsit5_adj_pair_kernel<%s, false>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
         {"adj8", "tsit5_adj_kernel<%s, 8>"}, {"adj16", "tsit5_adj_kernel<%s, 16>"},
         {"adj32", "tsit5_adj_kernel<%s, 32>"}, {"saveat", "tsit5_saveat_kernel<%s>"},
         {"saveat_vjp", "tsit5_saveat_vjp_kernel<%s>"}}},
       {"gsx_strict", "gsx_tsit5_kernels.h", kStrict,
        {{"fwd_rec", "tsit5_fwd_kernel<%s, true>"}, {"fwd", "tsit5_fwd_kernel<%s, false>"},
         {"replay", "tsit5_replay_kernel<%s>"}, {"adj1", "tsit5_adj_kernel<%s, 1>"},
-        {"adj_pair", "tsit5_adj_pair_kernel<%s>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
+        {"adj_pair", "tsit5_adj_pair_kernel<%s, true>"},
+        {"adj_pair_plain", "tsit5_adj_pair_kernel<%s, false>"}, {"adj4", "tsit5_adj_kernel<%s, 4>"},
         {"adj8", "tsit5_adj_kernel<%s, 8>"}, {"adj16", "tsit5_adj_kernel<%s, 16>"},
         {"adj32", "tsit5_adj_kernel<%s, 32>"}, {"saveat", "tsit5_saveat_kernel<%s>"},
         {"saveat_vjp", "tsit5_saveat_vjp_kernel<%s>"}}},
@@ -153,7 +155,7 @@ const char* kernel_template(const FamilySpec& sp, const char* key) {
 constexpr int kSegLdsBytes = 65536;
 
 int seg_cmax(const char* key) {
-  if (std::strncmp(key, "adj", 3) != 0 || std::strcmp(key, "adj_pair") == 0) return 0;
+  if (std::strncmp(key, "adj", 3) != 0 || std::strncmp(key, "adj_pair", 8) == 0) return 0;
   return std::atoi(key + 3);
 }
 
@@ -331,6 +333,7 @@ int tsit5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
     return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d",
                      a.ckpt_interval, kMaxCkptInterval);
   const bool is_pair = std::strcmp(key, "adj_pair") == 0;
+  if (is_pair && a.nsave == 0 && !a.pbar_in) key = "adj_pair_plain";
   return launch(field, fam, key, a.n, is_pair ? kFwdBlock : kAdjBlock, a, s);
 }
 

@@ -49,7 +49,10 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
   if (C <= 1) return launch_adj_c<F, 1>(a, s);
   if (C == 2) {
     const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
-    tsit5_adj_pair_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+    if (a.nsave > 0 || a.pbar_in)
+      tsit5_adj_pair_kernel<F, true><<<grid, kFwdBlock, 0, s>>>(a);
+    else
+      tsit5_adj_pair_kernel<F, false><<<grid, kFwdBlock, 0, s>>>(a);
     return check_launch("tsit5_adj_pair_kernel");
   }
   if (C <= 4) return launch_adj_c<F, 4>(a, s);

@@ -56,7 +56,9 @@ __global__ __launch_bounds__(kFwdBlock) void tsit5_replay_kernel(ReplayArgs a) {
   tsit5_replay_lane<F>(a, j);
 }
 
-// Checkpoint interval 2: the pair adjoint (tsit5_adj_pair_segment; no LDS). The segment loop
+// Checkpoint interval 2: the pair adjoint (tsit5_adj_pair_segment; no LDS; kSaves = false when
+// no save cotangents or incoming pbar are given, and only that variant takes the VGPR cap below:
+// with the save injection the cap would spill 24 VGPRs). The segment loop
 // runs in step across the wavefront, from its largest segment count down, so the active lanes
 // read the same checkpoint and tape rows; a lane idles above its own count. Fields with at
 // most three states and one parameter are held to 128 VGPRs (four waves per SIMD; Lorenz:
@@ -65,8 +67,9 @@ __global__ __launch_bounds__(kFwdBlock) void tsit5_replay_kernel(ReplayArgs a) {
 template <class F>
 constexpr int pair_waves_per_eu() { return (F::D <= 3 && F::P == 1) ? 4 : 1; }
 
-template <class F>
-__global__ __launch_bounds__(kFwdBlock) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<F>())))
+template <class F, bool kSaves>
+__global__ __launch_bounds__(kFwdBlock)
+__attribute__((amdgpu_waves_per_eu(kSaves ? 1 : pair_waves_per_eu<F>())))
 void tsit5_adj_pair_kernel(AdjArgs a) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
   const bool valid = j < a.n;
@@ -77,7 +80,7 @@ void tsit5_adj_pair_kernel(AdjArgs a) {
     mmax = o > mmax ? o : mmax;
   }
   if (!valid) return;
-  tsit5_adj_pair_lane<F>(a, j, na, mmax);
+  tsit5_adj_pair_lane<F, kSaves>(a, j, na, mmax);
 }
 
 // Dense output on a recorded mesh (tsit5_saveat_lane) and the VJPs of its re-steps
