@@ -183,10 +183,18 @@ def _stiff_lane(args, dev, method, name):
             best = (tb, tf, C)
         del ws
     tb, tf, C = best
+    trial_fl, adj_fl = ((WC.ROBERTSON_RB23_TRIAL_FLOPS, WC.ROBERTSON_RB23_ADJ_STEP_FLOPS)
+                        if method == "rb23" else
+                        (WC.ROBERTSON_RODAS5P_TRIAL_FLOPS, WC.ROBERTSON_RODAS5P_ADJ_STEP_FLOPS))
+    fl_fwd = trial_fl * (acc + rej)
+    fl_all = fl_fwd + adj_fl * acc
     out.update({"ckpt_interval": C, "fwd_traj_per_s": round(n / tf, 1),
                 "fwd_adj_traj_per_s": round(n / tb, 1),
                 "fwd_acc_steps_per_s": round(acc / tf, 1),
-                "fwd_adj_acc_steps_per_s": round(acc / tb, 1)})
+                "fwd_adj_acc_steps_per_s": round(acc / tb, 1),
+                "fwd_tflops": round(fl_fwd / tf / 1e12, 3),
+                "fwd_adj_tflops": round(fl_all / tb / 1e12, 3),
+                "roofline_frac": round(fl_all / tb / 1e12 / WC.PEAK_FP64_TFLOPS, 4)})
     return out
 
 

@@ -12,6 +12,14 @@ LORENZ_TSIT5_TRIAL_POWS = 1
 LORENZ_TSIT5_REPLAY_STEP_FLOPS = 264  # one tsit5_step (7 field calls)
 LORENZ_TSIT5_ADJ_STEP_FLOPS = 517   # stage recompute (no k7) + vector-Jacobian products
 
+# Robertson (D=3, P=3) on the stiff lanes: one adaptive trial step (Jacobian, one LU of W, the
+# stage solves, the error norm and the controller) and one adjoint step (stage recompute and
+# the transposed products through the solves, the field and the Jacobian).
+ROBERTSON_RB23_TRIAL_FLOPS = 198
+ROBERTSON_RB23_ADJ_STEP_FLOPS = 302
+ROBERTSON_RODAS5P_TRIAL_FLOPS = 948
+ROBERTSON_RODAS5P_ADJ_STEP_FLOPS = 2560
+
 # HBM bytes (algorithmic) for Lorenz, float64.
 TAPE_BYTES_PER_ACCEPTED_STEP = 8
 LORENZ_FWD_IO_BYTES = 24 + 8 + 24 + 12   # y0, rho in; y_final, n_acc/n_rej/status out

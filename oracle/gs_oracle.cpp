@@ -811,7 +811,7 @@ T r5_trial(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* ynew, 
       e[d] = e[d] + T(r5::kE[i]) * w.k[i][d];
     }
   bool ok = true;
-  for (int d = 0; d < D; ++d) ok = ok && std::isfinite(double(ynew[d])) && std::isfinite(double(e[d]));
+  for (int d = 0; d < D; ++d) ok = ok && std::isfinite(val(ynew[d])) && std::isfinite(val(e[d]));
   *finite = ok;
   return wrms(D, e, y, ynew, rtol, atol);
 }
@@ -1582,6 +1582,19 @@ int gso_count_ops(int which, int field, int D, int P, const double* aux, double*
       break;
     }
     case 4: rb23_step_vjp(fs, t, y, h, pv.data(), lam, pb); break;
+    case 5: {
+      // One Rodas5P trial step (J, one LU, 8 stages, solution and error combinations, the
+      // WRMS norm) plus the controller update.
+      bool finite = true;
+      const OpD err = r5_trial(fs, t, y, h, pv.data(), yn, rtol, atol, &finite);
+      const OpD fac = min_(max_(OpD(SAFETY) * pow_(max_(err, OpD(ERR_FLOOR)), OpD(TSIT5_EXP)),
+                                OpD(FAC_MIN)), OpD(FAC_MAX));
+      OpD dtn = h * fac;
+      (void)dtn;
+      break;
+    }
+    case 6: r5_step_vjp(fs, t, y, h, pv.data(), lam, static_cast<const OpD (*)[kMaxD]>(nullptr),
+                        pb); break;
     default: return -1;
   }
   *flops = g_flops;

@@ -181,7 +181,8 @@ int gso_field_vjp(int field, int D, int P, const double* aux, double t, const do
  * a fused multiply-add would be 2 (none is formed: contraction is off); pow counted
  * apart. which: 0 = one Tsit5 trial step (FSAL: 6 field calls), 1 = one replay step
  * (tsit5_step, 7 field calls), 2 = one adjoint step (stage recompute + VJP, no k7),
- * 3 = one Rosenbrock23 trial step. */
+ * 3 = one Rosenbrock23 trial step, 4 = one Rosenbrock23 adjoint step, 5 = one Rodas5P trial
+ * step (J, one LU, 8 stages, error norm and controller), 6 = one Rodas5P adjoint step. */
 int gso_count_ops(int which, int field, int D, int P, const double* aux,
                   double* flops, double* pows);
 
