@@ -34,7 +34,7 @@ def _parse():
     ap.add_argument("--ckpt", type=int, default=0, help="checkpoint interval (0 = autotune)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 17)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-torch-plain", action="store_true",
                     help="time only the HIP plain arm (the torch replay arm is slow)")
@@ -324,9 +324,19 @@ def _cpu_baseline(args, ens):
                                           nthreads=cores)
     O.tsit5_adjoint(O.LORENZ, y0, prm, 0.0, tape, 2.0 * yf, nthreads=cores)
     dt = time.perf_counter() - t
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(m / dt, 1), "unit": "traj/s", "cores": cores, "kind": "port",
             "sample": f"first {m} trajectories of the workload, record + adjoint, "
-                      f"{dt:.2f} s wall"}
+                      f"{dt:.2f} s wall ({dt * cores:.1f} thread-seconds)",
+            "cpu_model": model, "nproc": os.cpu_count()}
 
 
 if __name__ == "__main__":
