@@ -19,6 +19,16 @@ GSX_HD double rb_e32() { return 6.0 + std::sqrt(2.0); }
 // In-place LU with partial pivoting of a D x D row-major matrix held in registers. Row swaps
 // and the pivot search use compile-time indices with data-dependent selects, so nothing is
 // indexed dynamically (no scratch memory). piv[c] is the row swapped with row c at step c.
+// The default variant keeps the reciprocal of U's diagonal in place of the diagonal after
+// factoring: one division per pivot, and the multipliers and every triangular solve multiply
+// by it (within an ulp or so of the quotients). The strict variant keeps U's diagonal and
+// divides, as LAPACK's factorisation and triangular solves do.
+#if defined(GSX_STRICT_ARITH)
+constexpr bool kLuRecipDiag = false;
+#else
+constexpr bool kLuRecipDiag = true;
+#endif
+
 template <int D>
 GSX_HD void lu_factor(double (&a)[D * D], int (&piv)[D]) {
   GSX_UNROLL for (int c = 0; c < D; ++c) {
@@ -40,11 +50,13 @@ GSX_HD void lu_factor(double (&a)[D * D], int (&piv)[D]) {
         a[r * D + k] = sw ? x : z;
       }
     }
+    const double inv = kLuRecipDiag ? 1.0 / a[c * D + c] : 0.0;
     GSX_UNROLL for (int r = c + 1; r < D; ++r) {
-      const double l = a[r * D + c] / a[c * D + c];
+      const double l = kLuRecipDiag ? a[r * D + c] * inv : a[r * D + c] / a[c * D + c];
       a[r * D + c] = l;
       GSX_UNROLL for (int k = c + 1; k < D; ++k) a[r * D + k] = a[r * D + k] - l * a[c * D + k];
     }
+    if (kLuRecipDiag) a[c * D + c] = inv;
   }
 }
 
@@ -67,7 +79,7 @@ GSX_HD void lu_solve(const double (&a)[D * D], const int (&piv)[D], const double
   GSX_UNROLL for (int i = D - 1; i >= 0; --i) {
     double s = w[i];
     GSX_UNROLL for (int k = i + 1; k < D; ++k) s = s - a[i * D + k] * x[k];
-    x[i] = s / a[i * D + i];
+    x[i] = kLuRecipDiag ? s * a[i * D + i] : s / a[i * D + i];
   }
 }
 
@@ -79,7 +91,7 @@ GSX_HD void lu_solve_t(const double (&a)[D * D], const int (&piv)[D], const doub
   GSX_UNROLL for (int i = 0; i < D; ++i) {
     double s = b[i];
     GSX_UNROLL for (int k = 0; k < i; ++k) s = s - a[k * D + i] * z[k];
-    z[i] = s / a[i * D + i];
+    z[i] = kLuRecipDiag ? s * a[i * D + i] : s / a[i * D + i];
   }
   GSX_UNROLL for (int i = D - 1; i >= 0; --i) {
     double s = z[i];
