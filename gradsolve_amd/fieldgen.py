@@ -16,11 +16,13 @@ Supported: + - * / unary -, ``**`` with a numeric exponent (integer exponents up
 magnitude become products, as XLA's integer_pow does) or a symbolic one, abs, the functions
 sin cos tan exp log sqrt tanh sinh cosh arctan log1p expm1 square (as NumPy ufuncs or the
 functions of this module), comparisons < <= > >= (traced 0/1 values, as the reference's
-translator emits its lt/le/gt/ge primitives), and this module's ``where``, ``maximum``,
-``minimum`` and ``sign`` (the reference's select, max, min, sign). Anything that needs a
-symbol's truth value (``if``, ``max()``/``min()``, ``np.maximum`` on object arrays) raises
-``UnsupportedRHS`` naming the construct, as the reference's translator does for primitives
-outside its subset (``cond``).
+translator emits its lt/le/gt/ge primitives, on scalars and on the y and p arrays), and
+select / max / min / sign (the reference's select, max, min, sign primitives) as NumPy's own
+``np.where``, ``np.maximum``, ``np.minimum``, ``np.fmax``, ``np.fmin``, ``np.clip`` and
+``np.sign`` (y and p are handed in as ``TracedArray``s, whose NumPy hooks trace these) or this
+module's ``where``, ``maximum``, ``minimum`` and ``sign``. Anything that needs a symbol's
+truth value (``if``, Python's ``max()``/``min()``) raises ``UnsupportedRHS`` naming the
+construct, as the reference's translator does for primitives outside its subset (``cond``).
 """
 from __future__ import annotations
 
@@ -139,11 +141,36 @@ class Sym:
     def conjugate(self): return self
     def absolute(self): return abs(self)
 
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        # NumPy's max/min/sign/clip on a traced scalar trace (see TracedArray); any other
+        # ufunc runs NumPy's object loop on 0-d object arrays, which calls this class's
+        # operators and methods as it does without this hook.
+        if kwargs.get("out", 0) is None:
+            kwargs = {k: v for k, v in kwargs.items() if k != "out"}
+        if method == "__call__" and not kwargs and ufunc.__name__ in _TRACED_UFUNCS:
+            return _TRACED_UFUNCS[ufunc.__name__](*[_plain(x) for x in inputs])
+        args = []
+        for x in inputs:
+            if isinstance(x, Sym):
+                box = np.empty((), dtype=object)
+                box[()] = x
+                x = box
+            args.append(_plain(x))
+        out = getattr(ufunc, method)(*args, **kwargs)
+        if isinstance(out, np.ndarray) and out.ndim == 0 and out.dtype == object:
+            return out[()]
+        return _traced(out)
+
+    def __array_function__(self, func, types, args, kwargs):
+        # np.where / np.clip with a traced scalar among their arguments trace; other NumPy
+        # functions run their own implementation.
+        return _array_function(func, args, kwargs)
+
     def _no(self, what):
         raise UnsupportedRHS(what, f"{what} on a traced value: the right-hand side must be "
                                    "straight-line arithmetic; branch with "
-                                   "gradsolve_amd.fieldgen.where / maximum / minimum "
-                                   "instead of if, max(), min() or np.maximum")
+                                   "np.where / np.maximum / np.minimum / np.clip "
+                                   "instead of if, max() or min()")
 
     def __bool__(self): self._no("cond")
 
@@ -238,6 +265,71 @@ def sign(x):
     return np.sign(x)
 
 
+def _clip(a, lo, hi):
+    """minimum(maximum(a, lo), hi), as jnp.clip computes it."""
+    return minimum(maximum(a, lo), hi)
+
+
+# NumPy ufuncs whose object loops would ask a symbol for its truth value, traced instead (the
+# reference translator's max, min, sign and select primitives).
+_TRACED_UFUNCS = {"maximum": maximum, "fmax": maximum, "minimum": minimum, "fmin": minimum,
+                  "sign": sign, "clip": _clip}
+
+
+_COMPARISONS = ("greater", "greater_equal", "less", "less_equal")
+
+
+def _plain(x):
+    if isinstance(x, TracedArray):
+        return x.view(np.ndarray)
+    if isinstance(x, (list, tuple)):
+        return type(x)(_plain(v) for v in x)
+    return x
+
+
+def _traced(x):
+    if isinstance(x, np.ndarray) and not isinstance(x, TracedArray) and x.dtype == object:
+        return x.view(TracedArray)
+    if isinstance(x, tuple):
+        return tuple(_traced(v) for v in x)
+    return x
+
+
+class TracedArray(np.ndarray):
+    """The object arrays a traced right-hand side receives (y and p). NumPy's maximum,
+    minimum, fmax, fmin, sign, clip and where on them trace as this module's maximum,
+    minimum, sign and where; every other operation is NumPy's object-array arithmetic on the
+    symbols."""
+
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        args = [_plain(x) for x in inputs]
+        if kwargs.get("out", 0) is None:
+            kwargs = {k: v for k, v in kwargs.items() if k != "out"}
+        if method == "__call__" and not kwargs and ufunc.__name__ in _TRACED_UFUNCS:
+            return _traced(_TRACED_UFUNCS[ufunc.__name__](*args))
+        if method == "__call__" and not kwargs and ufunc.__name__ in _COMPARISONS:
+            # the object-result loop keeps each comparison a traced 0/1 value
+            return _traced(ufunc(*args, dtype=object))
+        if "out" in kwargs:
+            kwargs["out"] = _plain(kwargs["out"])
+        return _traced(getattr(ufunc, method)(*args, **kwargs))
+
+    def __array_function__(self, func, types, args, kwargs):
+        return _array_function(func, args, kwargs)
+
+
+def _array_function(func, args, kwargs):
+    args, kwargs = _plain(args), {k: _plain(v) for k, v in kwargs.items()}
+    kwargs = {k: v for k, v in kwargs.items() if not (k == "out" and v is None)}
+    if func is np.where and len(args) == 3 and not kwargs:
+        return _traced(where(*args))
+    if func is np.clip and len(args) == 3 and not kwargs:
+        return _traced(_clip(*args))
+    impl = getattr(func, "_implementation", func)
+    return _traced(impl(*args, **kwargs))
+
+
+
 sin, cos, tan, exp, log, sqrt = (_fn(n) for n in ("sin", "cos", "tan", "exp", "log", "sqrt"))
 tanh, sinh, cosh, arctan, log1p, expm1, square = (
     _fn(n) for n in ("tanh", "sinh", "cosh", "arctan", "log1p", "expm1", "square"))
@@ -283,13 +375,14 @@ def trace(f, dim: int, n_params: int) -> Traced:
     p = np.empty(n_params, dtype=object)
     for q in range(n_params):
         p[q] = g.var(f"p{q}")
+    y, p = y.view(TracedArray), p.view(TracedArray)
     try:
         out = f(t, y, p)
     except UnsupportedRHS:
         raise
     except TypeError as exc:
         raise UnsupportedRHS("trace", f"the right-hand side could not be traced: {exc}") from exc
-    arr = np.asarray(out, dtype=object).reshape(-1)
+    arr = np.asarray(_plain(out), dtype=object).reshape(-1)
     if arr.shape[0] != dim:
         raise UnsupportedRHS("shape", f"f returns {arr.shape[0]} components, expected "
                                       f"dim={dim}")

@@ -351,3 +351,41 @@ def test_piecewise_subset_field_and_vjp():
 def test_python_max_on_traced_values_names_the_alternative():
     with pytest.raises(UnsupportedRHS, match="maximum"):
         FG.trace(lambda t, y, p: np.array([max(y[0], y[1]), y[1]]), 2, 1)
+
+
+def numpy_piecewise(t, y, p):
+    return np.where(y > 0.25, np.clip(y, -1.5, 1.5) * p[0], -y) + \
+        np.maximum(y[::-1], 0.1) - np.minimum(y * y[0], p[1]) + np.sign(y) * 0.01 + \
+        np.fmax(np.sin(t), 0.0) * (y <= p[0])
+
+
+def test_numpy_select_max_min_clip_sign_trace():
+    """The same subset written with NumPy's own np.where / np.maximum / np.minimum / np.fmax /
+    np.clip / np.sign and array comparisons (what a NumPy port of a jnp right-hand side uses):
+    f bitwise equal to the Python function on floats, the VJP against central differences away
+    from the kinks (rel 1e-7)."""
+    hf = HostField(numpy_piecewise, 2, 2)
+    assert hf.time_dependent
+    rng = np.random.default_rng(6)
+    checked = 0
+    while checked < 40:
+        t = rng.uniform(0.0, 6.0)
+        y = rng.uniform(-2.0, 2.0, 2)
+        p = rng.uniform(0.5, 2.0, 2)
+        kinks = [*(y - 0.25), *(np.abs(y) - 1.5), *(y[::-1] - 0.1), *(y * y[0] - p[1]), *y,
+                 np.sin(t), *(y - p[0])]
+        if min(abs(k) for k in kinks) < 1e-3:
+            continue
+        checked += 1
+        want = np.asarray(numpy_piecewise(t, y, p), np.float64)
+        assert np.array_equal(hf.f(t, y, p), want)
+        v = rng.standard_normal(2)
+        yb, pb = hf.vjp(t, y, p, v)
+        fd_y, fd_p = np.zeros(2), np.zeros(2)
+        for k in range(2):
+            e = np.zeros(2)
+            e[k] = 1e-7
+            fd_y[k] = v @ (numpy_piecewise(t, y + e, p) - numpy_piecewise(t, y - e, p)) / 2e-7
+            fd_p[k] = v @ (numpy_piecewise(t, y, p + e) - numpy_piecewise(t, y, p - e)) / 2e-7
+        np.testing.assert_allclose(yb, fd_y, rtol=1e-7, atol=1e-7)
+        np.testing.assert_allclose(pb, fd_p, rtol=1e-7, atol=1e-7)
