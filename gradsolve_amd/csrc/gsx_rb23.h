@@ -144,7 +144,12 @@ GSX_HD double rb23_trial(double t, const double* y, double h, const double* p,
     r[i] = F2[i] - e32 * (k2[i] - F1[i]) - 2.0 * (k1[i] - F0[i]);
   lu_solve<D>(W, piv, r, k3);
   double e[D];
-  GSX_UNROLL for (int i = 0; i < D; ++i) e[i] = (h / 6.0) * (k1[i] - 2.0 * k2[i] + k3[i]);
+#if defined(GSX_STRICT_ARITH)
+  const double h6 = h / 6.0;
+#else
+  const double h6 = h * (1.0 / 6.0);  // within an ulp of h / 6, without a division
+#endif
+  GSX_UNROLL for (int i = 0; i < D; ++i) e[i] = h6 * (k1[i] - 2.0 * k2[i] + k3[i]);
   return wrms<D>(e, y, ynew, rtol, atol);
 }
 
