@@ -51,8 +51,8 @@ def main():
     args = _parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build runs the ensemble on a single MI355X "
-              "(trajectories shard by contiguous ranges with no exchange; see DESIGN.md). "
+        print("bench.py measures one GPU: the ensemble runs on a single MI355X (trajectories "
+              "would shard by contiguous ranges with no exchange between GPUs). "
               f"--gpus {args.gpus} / WORLD_SIZE {world} is not supported.", file=sys.stderr)
         sys.exit(2)
 
