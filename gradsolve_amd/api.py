@@ -118,9 +118,10 @@ def _solve_tsit5(problem, y0, params, *, rtol, atol, device, strict=False,
                        rejected_steps=nr.cpu().numpy().astype(np.int64), solver=D.HIP_TSIT5)
 
 
-def _solve_rb23(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult:
+def _solve_rb23(problem, y0, params, *, rtol, atol, device, strict=False,
+                **kw) -> SolveResult:
     from . import hip_rosenbrock23 as R
-    return R.solve(problem, y0, params, rtol=rtol, atol=atol, device=device)
+    return R.solve(problem, y0, params, rtol=rtol, atol=atol, device=device, strict=strict)
 
 
 def _supports_rodas5p(problem) -> bool:
@@ -370,7 +371,7 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
     else:
         from . import hip_rosenbrock23 as R
         closure = R.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
-                                 device=dev, ckpt_interval=ckpt_interval)
+                                 device=dev, ckpt_interval=ckpt_interval, strict=strict)
     closure.route = Route(requested=engine, actual=name,
                           reason="; ".join(reasons) if reasons else None)
     return closure
