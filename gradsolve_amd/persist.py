@@ -78,9 +78,14 @@ class PersistentTsit5:
             2 * (est.max_steps // C + 1) + 8
         self.traj_cap = int(math.ceil(4.0 * per_lane)) + 64
         self.queue_order: Optional[torch.Tensor] = None  # see plan_longest_first
+        self.frozen = False  # set while a captured HIP graph holds the buffers
         self._alloc()
 
     def _alloc(self):
+        if self.frozen:
+            raise RuntimeError(
+                "persistent engine: its buffers are held by a captured HIP graph and cannot be "
+                "reallocated (capacity growth); build a new engine and capture again")
         L, n, dev = self.lanes, self.n, self.device
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
@@ -153,15 +158,23 @@ class PersistentTsit5:
             self._alloc()
         raise RuntimeError("persistent record: capacity still exceeded after growth")
 
-    def padded_tape(self) -> torch.Tensor:
-        """The reference's zero-padded [n, S] tape, gathered from the lane records."""
-        S = int(self.n_acc.max().item()) if self.n else 0
+    def padded_tape(self, rows: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """The reference's zero-padded [n, S] tape, gathered from the lane records; with
+        ``rows`` (trajectory indices), only those trajectories, padded to their own longest."""
+        n_acc = self.n_acc.to(torch.int64)
+        lane_of, rec_start = self.lane_of, self.rec_start
+        if rows is not None:
+            rows = rows.to(device=self.rec.device, dtype=torch.int64)
+            n_acc, lane_of, rec_start = n_acc[rows], lane_of[rows], rec_start[rows]
+        m = n_acc.numel()
+        S = int(n_acc.max().item()) if m else 0
         s = torch.arange(S, device=self.rec.device, dtype=torch.int64)[None, :]
-        valid = s < self.n_acc.to(torch.int64)[:, None]
-        r = (self.rec_start.to(torch.int64)[:, None] + s // self.C).clamp(max=self.rec_cap - 1)
-        lane = self.lane_of.to(torch.int64)[:, None]
+        valid = s < n_acc[:, None]
+        r = (rec_start.to(torch.int64)[:, None] + s // self.C).clamp(max=self.rec_cap - 1)
+        lane = lane_of.to(torch.int64)[:, None]
         flat = (r * self.lanes + lane) * self.R + self.rows + s % self.C
-        out = self.rec.reshape(-1)[flat.reshape(-1)].reshape(self.n, S)
+        flat = torch.where(valid, flat, torch.zeros((), dtype=flat.dtype, device=flat.device))
+        out = self.rec.reshape(-1)[flat.reshape(-1)].reshape(m, S)
         return torch.where(valid, out, torch.zeros((), dtype=out.dtype, device=out.device))
 
 
@@ -170,6 +183,13 @@ class CapturedStep:
 
     ``cotangent(y_final_soa, out_soa)`` writes dL/dy_final; it runs inside the graph, so it
     must be a capturable device computation (no host synchronisation, no allocation).
+
+    Capacities are grown (``run_forward``) on the capture inputs before capture; from then on
+    the engine's buffers are frozen, since the graph holds their addresses. A replay cannot
+    grow them: a trajectory that no longer fits (inputs changed in place between replays) ends
+    with status 3 and no gradient, and one that exhausts ``max_steps`` with status 1, so
+    ``replay`` checks the statuses (one synchronisation) and raises on either unless
+    ``check=False``; ``check()`` does the same later.
     """
 
     def __init__(self, eng: PersistentTsit5, y0_soa, p_soa, cotangent, *, t0, t1, rtol, atol,
@@ -178,17 +198,34 @@ class CapturedStep:
         kw = dict(t0=t0, t1=t1, rtol=rtol, atol=atol, max_steps=max_steps, t1_per=t1_per)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):  # warm the allocator and the kernels outside the graph
-            eng.forward(y0_soa, p_soa, **kw)
+        with torch.cuda.stream(s):  # size the buffers, warm the kernels outside the graph
+            eng.run_forward(y0_soa, p_soa, **kw)
             cotangent(eng.y_final, eng.ybar)
             eng.adjoint(p_soa, eng.ybar, t0=t0)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
+        self.check()
+        eng.frozen = True
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             eng.forward(y0_soa, p_soa, **kw)
             cotangent(eng.y_final, eng.ybar)
             eng.adjoint(p_soa, eng.ybar, t0=t0)
 
-    def replay(self):
+    def check(self) -> None:
+        """Raise if a trajectory of the last run did not finish (status 1 or 3)."""
+        st = self.eng.status
+        full = int((st == 3).sum().item())
+        exhausted = int((st == 1).sum().item())
+        if full:
+            raise RuntimeError(
+                f"persistent record: {full} trajectories did not fit the captured capacities "
+                "(status 3); their gradients were not computed. Capture again on the new "
+                "inputs (capacities are grown before capture)")
+        if exhausted:
+            raise RuntimeError(f"{exhausted} trajectories exhausted max_steps before t1")
+
+    def replay(self, check: bool = True) -> None:
         self.graph.replay()
+        if check:
+            self.check()

@@ -471,6 +471,19 @@ struct Tsit5Work {
   T z[7][kMaxD];
 };
 
+// Stage time t + c h. Only a field that reads t needs it: every other field ignores its time
+// argument, so it gets the step's own t and the sum, which is no part of an autonomous
+// field's algorithmic work, is neither formed nor counted (values are unchanged).
+template <class T>
+T stage_time(const FieldSpec& fs, T t, double c, T h) {
+  return field_time_dependent(fs) ? t + T(c) * h : t;
+}
+
+template <class T>
+T stage_time_end(const FieldSpec& fs, T t, T h) {
+  return field_time_dependent(fs) ? t + h : t;
+}
+
 // Computes k2..k6 (k1 given in w.k[0]) and their stage inputs z2..z6, then y5 (no B7 term).
 template <class T>
 void tsit5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, Tsit5Work<T>& w,
@@ -480,20 +493,20 @@ void tsit5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, Ts
   T(&z)[7][kMaxD] = w.z;
   for (int i = 0; i < D; ++i) z[0][i] = y[i];
   for (int i = 0; i < D; ++i) z[1][i] = y[i] + h * (T(A21) * k[0][i]);
-  field_eval(fs, t + T(C2) * h, z[1], p, k[1]);
+  field_eval(fs, stage_time(fs, t, C2, h), z[1], p, k[1]);
   for (int i = 0; i < D; ++i) z[2][i] = y[i] + h * (T(A31) * k[0][i] + T(A32) * k[1][i]);
-  field_eval(fs, t + T(C3) * h, z[2], p, k[2]);
+  field_eval(fs, stage_time(fs, t, C3, h), z[2], p, k[2]);
   for (int i = 0; i < D; ++i)
     z[3][i] = y[i] + h * (T(A41) * k[0][i] + T(A42) * k[1][i] + T(A43) * k[2][i]);
-  field_eval(fs, t + T(C4) * h, z[3], p, k[3]);
+  field_eval(fs, stage_time(fs, t, C4, h), z[3], p, k[3]);
   for (int i = 0; i < D; ++i)
     z[4][i] = y[i] + h * (T(A51) * k[0][i] + T(A52) * k[1][i] + T(A53) * k[2][i] +
                           T(A54) * k[3][i]);
-  field_eval(fs, t + T(C5) * h, z[4], p, k[4]);
+  field_eval(fs, stage_time(fs, t, C5, h), z[4], p, k[4]);
   for (int i = 0; i < D; ++i)
     z[5][i] = y[i] + h * (T(A61) * k[0][i] + T(A62) * k[1][i] + T(A63) * k[2][i] +
                           T(A64) * k[3][i] + T(A65) * k[4][i]);
-  field_eval(fs, t + T(C6) * h, z[5], p, k[5]);
+  field_eval(fs, stage_time(fs, t, C6, h), z[5], p, k[5]);
   for (int i = 0; i < D; ++i)
     y5[i] = y[i] + h * (T(B1) * k[0][i] + T(B2) * k[1][i] + T(B3) * k[2][i] +
                         T(B4) * k[3][i] + T(B5) * k[4][i] + T(B6) * k[5][i]);
@@ -505,7 +518,7 @@ T tsit5_trial(const FieldSpec& fs, T t, const T* y, T h, const double* p, Tsit5W
               T* y5, T rtol, T atol) {
   const int D = fs.D;
   tsit5_stages(fs, t, y, h, p, w, y5);
-  field_eval(fs, t + h, y5, p, w.k[6]);
+  field_eval(fs, stage_time_end(fs, t, h), y5, p, w.k[6]);
   T e[kMaxD];
   const auto& k = w.k;
   for (int i = 0; i < D; ++i)
@@ -524,7 +537,7 @@ void tsit5_step(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* y
   field_eval(fs, t, y, p, w.k[0]);
   tsit5_stages(fs, t, y, h, p, w, y5);
   // k7 input: y + dt*(B1 k1 + ... + B6 k6) -- the same expression as y5.
-  field_eval(fs, t + T(C6) * h, y5, p, w.k[6]);
+  field_eval(fs, stage_time(fs, t, C6, h), y5, p, w.k[6]);
   const auto& k = w.k;
   for (int i = 0; i < D; ++i)
     ynext[i] = y[i] + h * (T(B1) * k[0][i] + T(B2) * k[1][i] + T(B3) * k[2][i] +
@@ -559,7 +572,7 @@ void tsit5_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, 
   for (int s = 5; s >= 0; --s) {
     T zbar[kMaxD];
     for (int i = 0; i < D; ++i) zbar[i] = 0.0;
-    field_vjp(fs, t + T(Cn[s]) * h, w.z[s], p, kbar[s], zbar, pbar);
+    field_vjp(fs, stage_time(fs, t, Cn[s], h), w.z[s], p, kbar[s], zbar, pbar);
     for (int i = 0; i < D; ++i) ynew[i] = ynew[i] + zbar[i];
     for (int r = 0; r < s; ++r)
       for (int i = 0; i < D; ++i) kbar[r][i] = kbar[r][i] + h * (T(A[s][r]) * zbar[i]);
@@ -785,7 +798,7 @@ void r5_stages(const FieldSpec& fs, T t, const T* y, T h, const double* p, R5Sta
   for (int i = 0; i < 8; ++i) {
     T u[kMaxD], acc[kMaxD], F[kMaxD], rhs[kMaxD];
     r5_stage_input(D, y, w.k, i, u, acc);
-    field_eval(fs, t + T(r5::kC[i]) * h, u, p, F);
+    field_eval(fs, stage_time(fs, t, r5::kC[i], h), u, p, F);
     for (int r = 0; r < D; ++r) {
       T ja = T(0.0);
       for (int c = 0; c < D; ++c) ja = ja + w.J[r * D + c] * acc[c];
@@ -874,7 +887,7 @@ void r5_step_vjp(const FieldSpec& fs, T t, const T* y, T h, const double* p, T* 
     for (int a = 0; a < D; ++a)
       for (int b = 0; b < D; ++b) Wb[a * D + b] = Wb[a * D + b] - rb[a] * w.k[i][b];
     for (int d = 0; d < D; ++d) { Fb[d] = h * rb[d]; ub[d] = T(0.0); }
-    field_vjp(fs, t + T(r5::kC[i]) * h, u, p, Fb, ub, pbar);
+    field_vjp(fs, stage_time(fs, t, r5::kC[i], h), u, p, Fb, ub, pbar);
     for (int a = 0; a < D; ++a)
       for (int b = 0; b < D; ++b) Jb[a * D + b] = Jb[a * D + b] + h * rb[a] * acc[b];
     for (int c = 0; c < D; ++c) {

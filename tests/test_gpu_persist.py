@@ -250,3 +250,27 @@ def test_graph_replay_matches_eager(gpu):
         for a, b in zip(got, ref):
             assert torch.equal(a, b), f"replay {r} differs from the eager result"
         _ = torch.ones(4, device=gpu).sum().item()  # eager work between replays
+        assert int((eng.status != 0).sum().item()) == 0
+    with pytest.raises(RuntimeError, match="captured HIP graph"):
+        eng._alloc()
+
+
+def test_graph_replay_reports_overflow(gpu):
+    """Inputs changed in place between replays so that the trajectories need more records
+    than were captured: the kernels end those trajectories with status 3 (no gradient), and
+    the replay raises instead of returning stale gradients."""
+    n = 4096
+    e, y0, p, t1 = _inputs(n, 5, gpu)
+    short = torch.full_like(t1, 0.5)
+    eng = _engine(n, y0, p, short, gpu, margin=1.0)
+    kw = dict(t0=0.0, t1=10.0, rtol=1e-6, atol=1e-6, max_steps=100000, t1_per=short)
+
+    def cot(yf, out):
+        torch.mul(yf, 2.0, out=out)
+
+    cap = PS.CapturedStep(eng, y0, p, cot, **kw)
+    cap.replay()
+    short.fill_(10.0)  # 20x longer trajectories, same captured capacities
+    with pytest.raises(RuntimeError, match="status 3"):
+        cap.replay()
+    assert int((eng.status == 3).sum().item()) > 0
