@@ -150,8 +150,13 @@ int gsx_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux, const d
   if (rc) return rc;
   if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
   if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
-  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+  // n_acc, n_rej and status may be null for a forward without a tape (the reference's
+  // forward handler returns y_final and, optionally, the accepted-step count); a recording
+  // forward needs n_acc and status (the tape's length and its "tape full" code).
+  if (n > 0 && (!y0 || !p || !y_final))
     return set_error(GSX_E_INVALID, "tsit5_fwd: null input/output pointer");
+  if (n > 0 && tape && (!n_acc || !status))
+    return set_error(GSX_E_INVALID, "tsit5_fwd: a recording forward needs n_acc and status");
   if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
   if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
@@ -382,8 +387,13 @@ int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux, const do
   int rc = check_field(field, D, P, &fi);
   if (rc) return rc;
   if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
-  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+  // n_acc, n_rej and status may be null for a forward without a tape (the reference's
+  // forward handler returns y_final and, optionally, the accepted-step count); a recording
+  // forward needs n_acc and status (the tape's length and its "tape full" code).
+  if (n > 0 && (!y0 || !p || !y_final))
     return set_error(GSX_E_INVALID, "rb23_fwd: null input/output pointer");
+  if (n > 0 && tape && (!n_acc || !status))
+    return set_error(GSX_E_INVALID, "rb23_fwd: a recording forward needs n_acc and status");
   if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
   if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
@@ -461,8 +471,13 @@ int gsx_rodas5p_fwd(int field, int64_t n, int D, int P, const double* aux, const
   int rc = r5_check(field, D, P, "rodas5p_fwd");
   if (rc) return rc;
   if (n < 0 || max_steps < 0) return set_error(GSX_E_INVALID, "negative n or max_steps");
-  if (n > 0 && (!y0 || !p || !y_final || !n_acc || !n_rej || !status))
+  // n_acc, n_rej and status may be null for a forward without a tape (the reference's
+  // forward handler returns y_final and, optionally, the accepted-step count); a recording
+  // forward needs n_acc and status (the tape's length and its "tape full" code).
+  if (n > 0 && (!y0 || !p || !y_final))
     return set_error(GSX_E_INVALID, "rodas5p_fwd: null input/output pointer");
+  if (n > 0 && tape && (!n_acc || !status))
+    return set_error(GSX_E_INVALID, "rodas5p_fwd: a recording forward needs n_acc and status");
   if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
   if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,

@@ -730,9 +730,10 @@ GSX_HD void tsit5_fwd_lane(const FwdArgs& a, int64_t j) {
   }
   if (status == kReached && !(t >= t1)) status = kExhausted;
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y_final[d * n + j] = y[d];
-  a.n_acc[j] = acc;
-  a.n_rej[j] = rej;
-  a.status[j] = status;
+  // the counts and the status are optional outputs of a forward without a tape
+  if (a.n_acc) a.n_acc[j] = acc;
+  if (a.n_rej) a.n_rej[j] = rej;
+  if (a.status) a.status[j] = status;
 }
 
 // ---------------------------------------------------------------------------------------

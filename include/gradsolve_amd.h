@@ -122,7 +122,10 @@ int64_t gsx_ckpt_slots(int64_t tape_cap, int32_t ckpt_interval);
  * forward+record kernel _tsit5_adaptive_general
  * (gradsolve/warp/_warp_kernel.py:325-399, driven by warp_ode._launch, warp_ode.py:215-327).
  * tape may be NULL (forward only); ckpt may be NULL (no adjoint will follow).
- * t1_per may be NULL (every trajectory ends at t1). dt0 = (t1 - t0)/100 per trajectory. */
+ * t1_per may be NULL (every trajectory ends at t1). dt0 = (t1 - t0)/100 per trajectory.
+ * Without a tape, n_acc, n_rej and status may each be NULL, so Tsit5Fwd binds 1:1 as
+ * (y0, p) -> (y_final, n_acc = nsteps); with a tape, n_acc and status are required (the
+ * tape's lengths and its "tape full" status). */
 int gsx_tsit5_fwd(int field, int64_t n, int D, int P, const double* aux,
                   const double* y0, const double* p, double t0, double t1,
                   const double* t1_per, double rtol, double atol, int64_t max_steps,
@@ -189,7 +192,8 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
  * I-controller with exponent -1/3. As in the reference, every stage evaluates the field at the
  * step's start time t and no df/dt term is added (a field that reads t gets the reference's
  * autonomous formula; Rodas5P carries the df/dt term). Same argument contract as
- * gsx_tsit5_fwd. Replaces the XLA FFI handler
+ * gsx_tsit5_fwd (optional counts and status without a tape, so Rosenbrock23Fwd binds as
+ * (y0, p) -> (y_final, nsteps)). Replaces the XLA FFI handler
  * Rosenbrock23Fwd (gradsolve/cuda/kernels/rosenbrock23_template.cu.in:151-223) and the Warp
  * kernel _rosenbrock23_adaptive (gradsolve/warp/_warp_rosenbrock.py:383-452, driven by
  * warp_rosenbrock._launch, gradsolve/warp/warp_rosenbrock.py:166-218). */

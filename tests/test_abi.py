@@ -50,3 +50,20 @@ def test_invalid_arguments_raise_before_launch():
     with pytest.raises(ValueError, match="flags"):
         N.call("gsx_tsit5_fwd", N.FIELD_LORENZ, 0, 3, 1, None, None, None, 0.0, 1.0, None,
                1e-8, 1e-8, 10, None, None, None, None, None, 0, None, 1, 8, None)
+
+
+@pytest.mark.parametrize("entry,field,P", [("gsx_tsit5_fwd", N.FIELD_LORENZ, 1),
+                                           ("gsx_rb23_fwd", N.FIELD_ROBERTSON, 3),
+                                           ("gsx_rodas5p_fwd", N.FIELD_ROBERTSON, 3)])
+def test_forward_counts_are_optional_without_a_tape(entry, field, P):
+    """The reference's forward handler binds (y0, p) -> (y_final, nsteps) with nothing else
+    (gradsolve/cuda/kernels/tsit5_template.cu.in:127-178): without a tape the counts and the
+    status may be null. A recording forward needs n_acc and status. Checked on the host's
+    argument validation (the call stops at the invalid flags, after the pointer checks)."""
+    fake = 4096  # never dereferenced: validation ends before any device work
+    with pytest.raises(ValueError, match="flags"):
+        N.call(entry, field, 4, 3, P, None, fake, fake, 0.0, 1.0, None, 1e-8, 1e-8, 10,
+               fake, None, None, None, None, 0, None, 1, 64, None)
+    with pytest.raises(ValueError, match="needs n_acc and status"):
+        N.call(entry, field, 4, 3, P, None, fake, fake, 0.0, 1.0, None, 1e-8, 1e-8, 10,
+               fake, fake, None, None, fake, 16, None, 1, 0, None)
