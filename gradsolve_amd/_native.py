@@ -86,6 +86,9 @@ SIGNATURES = {
     "gsx_tsit5_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                       _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _i, _vp]),
+    "gsx_tsit5_adjoint_saveat_noalloc": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64,
+                                              _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
+                                              _vp, _vp, _vp, _i, _vp]),
     "gsx_rb23_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _i64,
                              _vp, _vp, _vp, _vp, _i, _vp]),
     "gsx_rb23_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,

@@ -5,8 +5,17 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "gsx_internal.h"
+
+// Scratch of gsx_tsit5_adjoint_saveat_noalloc: the re-steps' cotangents mu [nsave][D][n],
+// the one buffer the saveat gradient needs beyond its inputs and outputs. 2^23 doubles
+// (64 MiB), held by every device the library runs on until the process ends; every call on a
+// device shares that device's copy. In host code `scratch` is only a stand-in for the device
+// array: the device address comes from hipGetSymbolAddress (gsx::noalloc_scratch).
+constexpr int64_t kNoallocScratchDoubles = int64_t(1) << 23;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
 
 namespace gsx {
 
@@ -25,6 +34,34 @@ int set_error(int code, const char* fmt, ...) {
 int check_launch(const char* what) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(GSX_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  return GSX_OK;
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The current device's copy of `scratch`: looked up once per device, with that device
+// current, and kept (each device holds its own copy at its own address).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(GSX_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(GSX_E_UNSUPPORTED, "device %d: the scratch table holds %d devices", dev,
+                     kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(GSX_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
   return GSX_OK;
 }
 
@@ -353,6 +390,39 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                                aux, y0, p, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ts,
                                nsave, save_step, save_t, states, ysbar, ybar, mu, y0bar, pbar,
                                flags, stream);
+}
+
+int gsx_tsit5_adjoint_saveat_noalloc(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t tape_cap, const int32_t* n_acc,
+                                     const double* ckpt, int32_t ckpt_interval,
+                                     const double* ts, int64_t nsave,
+                                     const int32_t* save_step, const double* save_t,
+                                     const double* states, const double* ysbar,
+                                     const double* ybar, double* y0bar, double* pbar,
+                                     int flags, void* stream) {
+  const char* what = "tsit5_adjoint_saveat_noalloc";
+  if (rt::is_runtime(field))
+    return set_error(GSX_E_UNSUPPORTED, "%s: fields compiled at run time load their kernels on "
+                     "first use (an allocation); use gsx_tsit5_adjoint_saveat", what);
+  if (n < 0 || nsave < 0 || D < 1) return set_error(GSX_E_INVALID, "%s: n %lld, nsave %lld, D %d",
+                                                    what, static_cast<long long>(n),
+                                                    static_cast<long long>(nsave), D);
+  double* mu = nullptr;
+  if (n > 0 && nsave > 0) {
+    // nsave * D * n doubles must fit the scratch (each factor checked before multiplying)
+    const int64_t cap = kNoallocScratchDoubles;
+    if (n > cap || static_cast<int64_t>(D) > cap / n || nsave > cap / (static_cast<int64_t>(D) * n))
+      return set_error(GSX_E_INVALID, "%s: nsave*D*n = %lld*%d*%lld doubles exceeds the "
+                       "library's %lld-double (64 MiB) scratch; use gsx_tsit5_adjoint_saveat "
+                       "with a caller buffer", what, static_cast<long long>(nsave), D,
+                       static_cast<long long>(n), static_cast<long long>(cap));
+    const int rc = noalloc_scratch(&mu);
+    if (rc) return rc;
+  }
+  return adjoint_saveat_common(DenseMethod::kTsit5, what, field, n, D, P, aux, y0, p, t0, tape,
+                               tape_cap, n_acc, ckpt, ckpt_interval, ts, nsave, save_step,
+                               save_t, states, ysbar, ybar, mu, y0bar, pbar, flags, stream);
 }
 
 int gsx_rb23_saveat(int field, int64_t n, int D, int P, const double* aux, const double* y0,

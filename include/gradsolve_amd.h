@@ -186,6 +186,25 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                              const double* ybar, double* mu, double* y0bar, double* pbar,
                              int flags, void* stream);
 
+/* gsx_tsit5_adjoint_saveat without the caller's mu buffer: the re-steps' cotangents go to a
+ * library-owned device array of 2^23 doubles (64 MiB) per device, so the call allocates
+ * nothing. No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). nsave * D * n above 2^23 is rejected (GSX_E_INVALID), as are fields compiled at
+ * run time (their kernels load on first use). Every call on a device shares that device's
+ * array, so calls that overlap on one device (other streams, other threads) race; the array
+ * needs no clearing (each call writes every entry it reads). Arguments and results as
+ * gsx_tsit5_adjoint_saveat. */
+int gsx_tsit5_adjoint_saveat_noalloc(int field, int64_t n, int D, int P, const double* aux,
+                                     const double* y0, const double* p, double t0,
+                                     const double* tape, int64_t tape_cap,
+                                     const int32_t* n_acc, const double* ckpt,
+                                     int32_t ckpt_interval, const double* ts, int64_t nsave,
+                                     const int32_t* save_step, const double* save_t,
+                                     const double* states, const double* ysbar,
+                                     const double* ybar, double* y0bar, double* pbar,
+                                     int flags, void* stream);
+
 /* Fused adaptive Rosenbrock23 (ode23s, autonomous specialisation) forward + accepted-h
  * record for stiff fields with an analytic Jacobian: W = I - d*h*J(t, y, p) factorised once
  * per step (LU with partial pivoting in registers) and reused by the three stage solves;
