@@ -213,9 +213,11 @@ def make_tsit5_f32_closure(binding, problem, y0, params, *, wrt, rtol, atol, dev
 
 
 def make_rb23_closure(binding, problem, y0, params, *, wrt, rtol, atol, device,
-                      ckpt_interval=None, strict=False, aux=None,
-                      max_steps=50000) -> ReplayClosure:
-    lane = _Rb23Lane(binding, aux, strict)
+                      ckpt_interval=None, strict=False, aux=None, max_steps=50000,
+                      fp32=False) -> ReplayClosure:
+    """The Rosenbrock23 lane (``fp32``: FP32 stages on a compensated state, FP64 W, LU,
+    solves, error norm, controller and tape)."""
+    lane = _Rb23Lane(binding, aux, strict, fp32=fp32)
     return ReplayClosure(lane, binding, problem, y0, params, wrt=wrt, rtol=rtol, atol=atol,
                          device=device, ckpt_interval=ckpt_interval, max_steps=max_steps)
 

@@ -448,6 +448,22 @@ int gsx_rb23_adjoint_saveat(int field, int64_t n, int D, int P, const double* au
                                stream);
 }
 
+namespace {
+// Rosenbrock23 flags: 0, GSX_STRICT, or GSX_FP32 (autonomous fields).
+int rb23_flags(int field, int D, int flags) {
+  if (flags & ~(GSX_STRICT | GSX_FP32)) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  if ((flags & GSX_STRICT) && (flags & GSX_FP32))
+    return set_error(GSX_E_INVALID, "GSX_STRICT and GSX_FP32 are exclusive");
+  if (flags & GSX_FP32) {
+    FieldInfo fi;
+    if (field_info(field, D, &fi) && !fi.autonomous)
+      return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 FP32: field %d reads t; the FP32 "
+                       "variant serves autonomous fields", field);
+  }
+  return GSX_OK;
+}
+}  // namespace
+
 int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
                  const double* p, double t0, double t1, const double* t1_per, double rtol,
                  double atol, int64_t max_steps, double* y_final, int32_t* n_acc,
@@ -470,10 +486,12 @@ int gsx_rb23_fwd(int field, int64_t n, int D, int P, const double* aux, const do
                      kMaxCkptInterval);
   if (ckpt && !tape) return set_error(GSX_E_INVALID, "checkpoints need a tape");
   if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  rc = rb23_flags(field, D, flags);
+  if (rc) return rc;
   FwdArgs a{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
             status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1};
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (flags & GSX_FP32) return rb23_fwd_f32_dispatch(field, D, a, st);
   return (flags & GSX_STRICT) ? gsx_strict::rb23_fwd_dispatch(field, D, a, st)
                               : rb23_fwd_dispatch(field, D, a, st);
 }
@@ -491,9 +509,11 @@ int gsx_rb23_replay(int field, int64_t n, int D, int P, const double* aux, const
   if (ckpt && (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval))
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  rc = rb23_flags(field, D, flags);
+  if (rc) return rc;
   ReplayArgs a{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1};
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (flags & GSX_FP32) return rb23_replay_f32_dispatch(field, D, a, st);
   return (flags & GSX_STRICT) ? gsx_strict::rb23_replay_dispatch(field, D, a, st)
                               : rb23_replay_dispatch(field, D, a, st);
 }
@@ -512,9 +532,11 @@ int gsx_rb23_adjoint(int field, int64_t n, int D, int P, const double* aux, cons
   if (ckpt_interval < 1 || ckpt_interval > kMaxCkptInterval)
     return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", ckpt_interval,
                      kMaxCkptInterval);
-  if (flags & ~GSX_STRICT) return set_error(GSX_E_INVALID, "unknown flags 0x%x", flags);
+  rc = rb23_flags(field, D, flags);
+  if (rc) return rc;
   AdjArgs a{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar, pbar};
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (flags & GSX_FP32) return rb23_adj_f32_dispatch(field, D, a, st);
   return (flags & GSX_STRICT) ? gsx_strict::rb23_adj_dispatch(field, D, a, st)
                               : rb23_adj_dispatch(field, D, a, st);
 }

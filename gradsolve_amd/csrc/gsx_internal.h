@@ -42,6 +42,9 @@ int r5_saveat_vjp_dispatch(int field, int D, const SaveVjpArgs& a, hipStream_t s
 int tsit5_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
 int tsit5_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
 int tsit5_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
+int rb23_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s);
+int rb23_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s);
+int rb23_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s);
 int persist_fwd_dispatch(int field, int D, bool fp32, const PersistFwdArgs& pa, hipStream_t s);
 int persist_adj_dispatch(int field, int D, bool fp32, const PersistAdjArgs& pa, hipStream_t s);
 int kura_fwd_launch(const KuraFwdArgs& a, bool fp32, hipStream_t s);
@@ -54,7 +57,7 @@ int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 // kernels compiled with hiprtc per family and launched with the built-in argument blocks.
 namespace rt {
 enum Family { kTsit5, kTsit5Strict, kTsit5F32, kRb23, kRb23Strict, kRodas5p, kRodas5pStrict,
-              kNumFamilies };
+              kRb23F32, kNumFamilies };
 bool is_runtime(int field);
 bool info(int field, FieldInfo* out);
 int tsit5_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);

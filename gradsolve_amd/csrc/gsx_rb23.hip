@@ -64,6 +64,42 @@ int launch_saveat_vjp(const SaveVjpArgs& a, hipStream_t s) {
   return check_launch("rb23_saveat_vjp_kernel");
 }
 
+template <class F>
+int launch_fwd_f32(const FwdArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_fwd_f32_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_fwd_f32_kernel");
+}
+
+template <class F>
+int launch_replay_f32(const ReplayArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+  rb23_replay_f32_kernel<F><<<grid, kFwdBlock, 0, s>>>(a);
+  return check_launch("rb23_replay_f32_kernel");
+}
+
+template <class F, int CMAX>
+int launch_adj_f32_c(const AdjArgs& a, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((a.n + kAdjBlock - 1) / kAdjBlock);
+  rb23_adj_f32_kernel<F, CMAX><<<grid, kAdjBlock, 0, s>>>(a);
+  return check_launch("rb23_adj_f32_kernel");
+}
+
+template <class F>
+int launch_adj_f32(const AdjArgs& a, hipStream_t s) {
+  if (a.n == 0) return GSX_OK;
+  const int C = a.ckpt_interval;
+  if (C <= 1) return launch_adj_f32_c<F, 1>(a, s);
+  if (C <= 4) return launch_adj_f32_c<F, 4>(a, s);
+  if (C <= 8) return launch_adj_f32_c<F, 8>(a, s);
+  if (C <= 16) return launch_adj_f32_c<F, 16>(a, s);
+  if (C <= 32) return launch_adj_f32_c<F, 32>(a, s);
+  return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d", C,
+                   kMaxCkptInterval);
+}
+
 #if defined(GSX_STRICT_ARITH)
 constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23Strict;
 #else
@@ -71,6 +107,60 @@ constexpr ::gsx::rt::Family kRtFamily = ::gsx::rt::kRb23;
 #endif
 
 }  // namespace
+
+#if !defined(GSX_STRICT_ARITH)
+// The FP32 variant: autonomous stiff fields (built in, or compiled at run time).
+int rb23_fwd_f32_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_fwd_f32<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_fwd_f32<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_fwd_f32<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_fwd_f32<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field))
+        return ::gsx::rt::rb23_fwd(field, ::gsx::rt::kRb23F32, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 FP32 forward: field %d with D=%d is not "
+                   "built (the FP32 variant serves autonomous stiff fields)", field, D);
+}
+
+int rb23_replay_f32_dispatch(int field, int D, const ReplayArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_replay_f32<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_replay_f32<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_replay_f32<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_replay_f32<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field))
+        return ::gsx::rt::rb23_replay(field, ::gsx::rt::kRb23F32, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 FP32 replay: field %d with D=%d is not "
+                   "built (the FP32 variant serves autonomous stiff fields)", field, D);
+}
+
+int rb23_adj_f32_dispatch(int field, int D, const AdjArgs& a, hipStream_t s) {
+  switch (field) {
+    case GSX_FIELD_ROBERTSON: return launch_adj_f32<FRobertson>(a, s);
+    case GSX_FIELD_LINEAR_DECAY:
+      if (D == 1) return launch_adj_f32<FLinearDecay<1>>(a, s);
+      if (D == 2) return launch_adj_f32<FLinearDecay<2>>(a, s);
+      if (D == 3) return launch_adj_f32<FLinearDecay<3>>(a, s);
+      break;
+    default:
+      if (::gsx::rt::is_runtime(field))
+        return ::gsx::rt::rb23_adj(field, ::gsx::rt::kRb23F32, a, s);
+      break;
+  }
+  return set_error(GSX_E_UNSUPPORTED, "rosenbrock23 FP32 adjoint: field %d with D=%d is not "
+                   "built (the FP32 variant serves autonomous stiff fields)", field, D);
+}
+#endif
 
 int rb23_fwd_dispatch(int field, int D, const FwdArgs& a, hipStream_t s) {
   switch (field) {

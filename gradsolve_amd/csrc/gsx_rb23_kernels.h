@@ -4,6 +4,7 @@
 
 #include "gsx_core.h"
 #include "gsx_rb23.h"
+#include "gsx_rb23_f32.h"
 
 namespace GSX_NS {
 
@@ -42,6 +43,29 @@ __global__ __launch_bounds__(kFwdBlock) void rb23_saveat_vjp_kernel(SaveVjpArgs 
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
   if (j >= a.n) return;
   rb23_saveat_vjp_lane<F>(a, j);
+}
+
+// The FP32 variant (gsx_rb23_f32.h): FP32 stages on a compensated state, FP64 LU and solves.
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_fwd_f32_kernel(FwdArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_fwd_lane_f32<F>(a, j);
+}
+
+template <class F>
+__global__ __launch_bounds__(kFwdBlock) void rb23_replay_f32_kernel(ReplayArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_replay_lane_f32<F>(a, j);
+}
+
+template <class F, int CMAX>
+__global__ __launch_bounds__(kAdjBlock) void rb23_adj_f32_kernel(AdjArgs a) {
+  __shared__ double seg[CMAX * F::D * kAdjBlock];
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  if (j >= a.n) return;
+  rb23_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock);
 }
 
 }  // namespace GSX_NS

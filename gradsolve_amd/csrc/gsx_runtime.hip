@@ -14,6 +14,7 @@
 //   kTsit5F32      gsx          the FP32 forward, replay and adjoint (autonomous fields)
 //   kRb23          gsx          Rosenbrock23 forward, replay, adjoint (stiff fields)
 //   kRb23Strict    gsx_strict   the same, strict arithmetic
+//   kRb23F32       gsx          the FP32 Rosenbrock23 forward, replay and adjoint (autonomous)
 //
 // The recording forward of the primary family (kTsit5, or kRb23 for a stiff field) is
 // compiled at registration so that errors in the source surface there; every other kernel on
@@ -97,6 +98,11 @@ const FamilySpec& spec(Family f) {
         {"adj32", "r5_adj_kernel<%s, 32>"}, {"saveat", "r5_saveat_kernel<%s>"},
         {"saveat_vjp", "r5_saveat_vjp_kernel<%s>"},
         {"saveat_dense", "r5_saveat_dense_kernel<%s>"}}},
+      {"gsx", "gsx_rb23_kernels.h", {},
+       {{"fwd", "rb23_fwd_f32_kernel<%s>"}, {"replay", "rb23_replay_f32_kernel<%s>"},
+        {"adj1", "rb23_adj_f32_kernel<%s, 1>"}, {"adj4", "rb23_adj_f32_kernel<%s, 4>"},
+        {"adj8", "rb23_adj_f32_kernel<%s, 8>"}, {"adj16", "rb23_adj_f32_kernel<%s, 16>"},
+        {"adj32", "rb23_adj_f32_kernel<%s, 32>"}}},
   };
   return specs[f];
 }
@@ -104,7 +110,8 @@ const FamilySpec& spec(Family f) {
 const char* family_name(Family f) {
   static const char* names[kNumFamilies] = {"tsit5", "tsit5 (strict)", "tsit5 (FP32)",
                                             "rosenbrock23", "rosenbrock23 (strict)",
-                                            "rodas5p", "rodas5p (strict)"};
+                                            "rodas5p", "rodas5p (strict)",
+                                            "rosenbrock23 (FP32)"};
   return names[f];
 }
 
@@ -160,7 +167,7 @@ int seg_cmax(const char* key) {
 }
 
 int ckpt_rows_of(const RtField& f, Family fam) {
-  return f.D + ((f.autonomous || fam == kTsit5F32) ? 0 : 1);
+  return f.D + ((f.autonomous || fam == kTsit5F32 || fam == kRb23F32) ? 0 : 1);
 }
 
 std::string program_source(const RtField& f, const FamilySpec& sp) {

@@ -157,7 +157,10 @@ def test_float32_closure_against_float64_replay_of_its_tape(gpu):
     assert np.max(np.abs(res.y_final - oyf) / np.abs(oyf).max(axis=1, keepdims=True)) < 1e-4
 
 
-def test_float32_rejected_on_stiff_lane(gpu):
+def test_float32_rejected_on_the_rodas5p_lane(gpu):
+    """Rodas5P is float64 throughout; float32 raises naming the precision, as the reference
+    does for its f64-only lanes (gradsolve/api.py:262-274)."""
     e = W.robertson(4, seed=0)
     with pytest.raises(ValueError, match="float32"):
-        gs.grad_closure(P.Robertson(), e.y0, e.params, precision="float32", device="cuda")
+        gs.grad_closure(P.Robertson(), e.y0, e.params, engine="rodas5p_replay",
+                        precision="float32", device="cuda")
