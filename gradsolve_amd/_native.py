@@ -86,6 +86,13 @@ SIGNATURES = {
     "gsx_tsit5_adjoint_saveat": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
                                       _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _i, _vp]),
+    "gsx_tsit5_wide_ws_rows": (_i64, [_i, _i32]),
+    "gsx_tsit5_wide_fwd": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _d, _vp, _d, _d, _i64,
+                                _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]),
+    "gsx_tsit5_wide_replay": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp,
+                                   _i32, _vp, _vp]),
+    "gsx_tsit5_wide_adjoint": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64, _vp, _vp,
+                                    _i32, _vp, _vp, _vp, _vp, _vp]),
     "gsx_tsit5_adjoint_saveat_noalloc": (_i, [_i, _i64, _i, _i, _vp, _vp, _vp, _d, _vp, _i64,
                                               _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
                                               _vp, _vp, _vp, _i, _vp]),

@@ -23,7 +23,7 @@ from . import fields as F
 from .base import Route, SolveResult
 
 _VALID_WRT = "'params', 'y0', or ('y0', 'params')"
-_NOT_BUILT = {"tsit5_replay", "vern7_replay", "fixed_step_tsit5", "fixed_step_imex", "diffrax",
+_NOT_BUILT = {"vern7_replay", "fixed_step_tsit5", "fixed_step_imex", "diffrax",
               "fused_rosenbrock_backward"}
 _RODAS5P_NAMES = ("rodas5p_replay", "hip_rodas5p")
 
@@ -86,6 +86,18 @@ def _native_ok(binding, problem, params) -> bool:
 def _supports_tsit5(problem) -> bool:
     b = F.binding_for(problem)
     return b is not None and not b.stiff and not problem.is_stiff
+
+
+def _supports_wide(problem) -> bool:
+    """The wide lane serves non-stiff fields compiled at run time (any D up to 4096)."""
+    b = F.binding_for(problem)
+    return (b is not None and b.source is not None and not b.stiff
+            and not problem.is_stiff)
+
+
+def _solve_wide(problem, y0, params, *, rtol, atol, device, **kw) -> SolveResult:
+    from . import hip_wide as HW
+    return HW.solve(problem, y0, params, rtol=rtol, atol=atol, device=device)
 
 
 def _supports_rb23(problem) -> bool:
@@ -182,6 +194,7 @@ ENGINE_REGISTRY = {
     D.HIP_KURAMOTO: EngineSpec(D.HIP_KURAMOTO, _supports_kuramoto, _solve_kuramoto,
                                "kuramoto"),
     D.HIP_RODAS5P: EngineSpec(D.HIP_RODAS5P, _supports_rodas5p, _solve_rodas5p, "rodas5p"),
+    D.HIP_TSIT5_WIDE: EngineSpec(D.HIP_TSIT5_WIDE, _supports_wide, _solve_wide, "wide"),
 }
 
 
@@ -197,6 +210,19 @@ def _resolve(engine, problem, need_grad, batch_n, accuracy_target, pre=None):
             # the reference substitutes its Rodas5P lane when diffrax is not installed
             name = D.HIP_RODAS5P
             reasons.append("diffrax-not-installed")
+        elif name == D.DIFFRAX:
+            # ... and its general Tsit5 record-and-replay lane for a non-stiff problem
+            name = D.HIP_TSIT5_WIDE
+            reasons.append("diffrax-not-installed")
+        elif name == D.TSIT5_REPLAY:
+            name = D.HIP_TSIT5_WIDE
+            reasons.append("alias:tsit5_replay")
+    elif engine == D.TSIT5_REPLAY:
+        # the reference's general lane: the wide lane for fields compiled at run time, the
+        # register lane for a built-in field
+        b = F.binding_for(problem)
+        name = D.HIP_TSIT5_WIDE if (b is None or b.source is not None) else D.HIP_REPLAY
+        reasons.append(f"alias:{engine}")
     elif engine in ENGINE_REGISTRY:
         name = engine
     elif engine in D.ALIASES:
@@ -363,6 +389,10 @@ def grad_closure(problem, y0, params, *, wrt="params", saveat=None, saveat_dense
         closure = make_tsit5_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
                                      device=dev, ckpt_interval=ckpt_interval, strict=strict,
                                      aux=aux)
+    elif name == D.HIP_TSIT5_WIDE:
+        from . import hip_wide as HW
+        closure = HW.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,
+                                  device=dev, ckpt_interval=ckpt_interval, aux=aux)
     elif name == D.HIP_RODAS5P:
         from . import hip_rodas5p as R5
         closure = R5.make_closure(b, problem, y0, params, wrt=wrt_t, rtol=rtol, atol=atol,

@@ -59,7 +59,7 @@ def sources() -> list[tuple[str, str, list[str]]]:
 
 # Headers embedded into the library for fields compiled at run time (gsx_runtime.hip hands
 # them to hiprtc by name, so the library needs no header files where it runs).
-RT_HEADERS = ["gsx_platform.h", "gsx_args.h", "gsx_core.h", "gsx_f32.h", "gsx_rb23.h",
+RT_HEADERS = ["gsx_platform.h", "gsx_args.h", "gsx_core.h", "gsx_f32.h", "gsx_rb23.h", "gsx_wide.h",
               "gsx_rb23_f32.h",
               "gsx_tsit5_kernels.h", "gsx_rb23_kernels.h", "gsx_rodas5p_tab.h",
               "gsx_rodas5p.h", "gsx_rodas5p_kernels.h"]

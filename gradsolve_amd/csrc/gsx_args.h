@@ -70,6 +70,28 @@ struct AdjArgs {
   const double* ysbar = nullptr;
 };
 
+// The wide Tsit5 lanes (gsx_wide.h): the register lanes' argument blocks plus a workspace.
+// Workspace rows, D doubles each per trajectory, [row][D][n]:
+//   forward / replay: y, y5 (exchanged on accept), z (stage input), k1..k7 (k1 and k7
+//   exchanged on accept: FSAL without a copy)
+//   adjoint: for each of the segment's C steps its start state and stage inputs z2..z6
+//   (6 rows), then k1..k6, kbar1..kbar6, lam, zbar
+constexpr int kWideFwdRows = 10;
+GSX_HD constexpr int wide_adj_rows(int C) { return 6 * C + 14; }
+
+struct WideFwdArgs {
+  FwdArgs f;
+  double* ws;  // [kWideFwdRows][D][n]
+};
+struct WideReplayArgs {
+  ReplayArgs r;
+  double* ws;  // [kWideFwdRows][D][n]
+};
+struct WideAdjArgs {
+  AdjArgs a;
+  double* ws;  // [wide_adj_rows(C)][D][n]
+};
+
 // Dense output on a recorded mesh (gradsolve/solvers/dense.py:106-163): for every save time
 // ts[q] (ascending, shared by the ensemble), the accepted step s that brackets it
 // (t_s < ts[q] <= t_s + h_s), its start time and state, and the saved state, one Tsit5 step of

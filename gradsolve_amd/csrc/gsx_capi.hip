@@ -266,6 +266,82 @@ int gsx_tsit5_adjoint(int field, int64_t n, int D, int P, const double* aux, con
   return tsit5_adj_dispatch(field, D, a, static_cast<hipStream_t>(stream));
 }
 
+// ---- The wide Tsit5 lane (fields compiled at run time, any D up to 4096) ----------------
+namespace {
+int wide_check(const char* what, int field, int64_t n, int D, int P, int32_t C, bool need_c,
+               const double* ws) {
+  FieldInfo fi;
+  int rc = check_field(field, D, P, &fi);
+  if (rc) return rc;
+  if (!rt::is_runtime(field))
+    return set_error(GSX_E_UNSUPPORTED, "%s: the wide lane serves fields compiled at run time "
+                     "(gsx_field_register); field %d is built in", what, field);
+  if (fi.stiff) return set_error(GSX_E_INVALID, "field %d is stiff; use Rosenbrock23", field);
+  if (n < 0) return set_error(GSX_E_INVALID, "%s: negative n", what);
+  if (need_c && (C < 1 || C > kMaxCkptInterval))
+    return set_error(GSX_E_INVALID, "ckpt_interval %d outside [1, %d]", C, kMaxCkptInterval);
+  if (n > 0 && !ws) return set_error(GSX_E_INVALID, "%s: null workspace", what);
+  return GSX_OK;
+}
+}  // namespace
+
+int64_t gsx_tsit5_wide_ws_rows(int which, int32_t ckpt_interval) {
+  if (which == 0) return kWideFwdRows;
+  if (which == 1 && ckpt_interval >= 1 && ckpt_interval <= kMaxCkptInterval)
+    return wide_adj_rows(ckpt_interval);
+  return set_error(GSX_E_INVALID, "tsit5_wide_ws_rows: which %d, ckpt_interval %d", which,
+                   ckpt_interval);
+}
+
+int gsx_tsit5_wide_fwd(int field, int64_t n, int D, int P, const double* aux, const double* y0,
+                       const double* p, double t0, double t1, const double* t1_per,
+                       double rtol, double atol, int64_t max_steps, double* y_final,
+                       int32_t* n_acc, int32_t* n_rej, int32_t* status, double* tape,
+                       int64_t tape_cap, double* ckpt, int32_t ckpt_interval, double* ws,
+                       void* stream) {
+  int rc = wide_check("tsit5_wide_fwd", field, n, D, P, ckpt_interval, ckpt != nullptr, ws);
+  if (rc) return rc;
+  if (max_steps < 0) return set_error(GSX_E_INVALID, "negative max_steps");
+  if (n > 0 && (!y0 || !p || !y_final))
+    return set_error(GSX_E_INVALID, "tsit5_wide_fwd: null input/output pointer");
+  if (n > 0 && tape && (!n_acc || !status))
+    return set_error(GSX_E_INVALID, "tsit5_wide_fwd: a recording forward needs n_acc and status");
+  if (tape && tape_cap < 0) return set_error(GSX_E_INVALID, "negative tape_cap");
+  if (ckpt && !tape) return set_error(GSX_E_INVALID, "checkpoints need a tape");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return set_error(GSX_E_INVALID, "bad rtol/atol");
+  WideFwdArgs w{{n, y0, p, aux, t0, t1, t1_per, rtol, atol, max_steps, y_final, n_acc, n_rej,
+                 status, tape, tape ? tape_cap : 0, ckpt, ckpt ? ckpt_interval : 1}, ws};
+  return rt::tsit5w_fwd(field, w, static_cast<hipStream_t>(stream));
+}
+
+int gsx_tsit5_wide_replay(int field, int64_t n, int D, int P, const double* aux,
+                          const double* y0, const double* p, double t0, const double* tape,
+                          const int32_t* n_acc, double* y_final, double* ckpt,
+                          int32_t ckpt_interval, double* ws, void* stream) {
+  int rc = wide_check("tsit5_wide_replay", field, n, D, P, ckpt_interval, ckpt != nullptr, ws);
+  if (rc) return rc;
+  if (n > 0 && (!y0 || !p || !tape || !n_acc || !y_final))
+    return set_error(GSX_E_INVALID, "tsit5_wide_replay: null pointer");
+  WideReplayArgs w{{n, y0, p, aux, t0, tape, n_acc, y_final, ckpt, ckpt ? ckpt_interval : 1},
+                   ws};
+  return rt::tsit5w_replay(field, w, static_cast<hipStream_t>(stream));
+}
+
+int gsx_tsit5_wide_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                           const double* y0, const double* p, double t0, const double* tape,
+                           int64_t tape_cap, const int32_t* n_acc, const double* ckpt,
+                           int32_t ckpt_interval, const double* ybar, double* y0bar,
+                           double* pbar, double* ws, void* stream) {
+  int rc = wide_check("tsit5_wide_adjoint", field, n, D, P, ckpt_interval, true, ws);
+  if (rc) return rc;
+  if (n > 0 && (!p || !tape || !n_acc || !ckpt || !ybar || !y0bar || !pbar))
+    return set_error(GSX_E_INVALID, "tsit5_wide_adjoint: null pointer");
+  WideAdjArgs w{{n, y0, p, aux, t0, tape, tape_cap, n_acc, ckpt, ckpt_interval, ybar, y0bar,
+                 pbar},
+                ws};
+  return rt::tsit5w_adj(field, w, static_cast<hipStream_t>(stream));
+}
+
 namespace {
 
 // Dense output and its gradient for one of the two one-step methods; the Tsit5 and

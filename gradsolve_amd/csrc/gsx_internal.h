@@ -57,7 +57,7 @@ int plain_dispatch(int field, bool adjoint, const PlainArgs& a, hipStream_t s);
 // kernels compiled with hiprtc per family and launched with the built-in argument blocks.
 namespace rt {
 enum Family { kTsit5, kTsit5Strict, kTsit5F32, kRb23, kRb23Strict, kRodas5p, kRodas5pStrict,
-              kRb23F32, kNumFamilies };
+              kRb23F32, kTsit5Wide, kNumFamilies };
 bool is_runtime(int field);
 bool info(int field, FieldInfo* out);
 int tsit5_fwd(int field, Family fam, const FwdArgs& a, hipStream_t s);
@@ -79,6 +79,14 @@ int r5_launch(int field, Family fam, const char* key, int64_t n, const SaveVjpAr
               hipStream_t s);
 int r5_adj(int field, Family fam, const AdjArgs& a, hipStream_t s);
 int rb23_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s);
+int tsit5w_fwd(int field, const WideFwdArgs& w, hipStream_t s);
+int tsit5w_replay(int field, const WideReplayArgs& w, hipStream_t s);
+int tsit5w_adj(int field, const WideAdjArgs& w, hipStream_t s);
+// Widest state a registered field may have (the wide Tsit5 lane), the widest the register
+// lanes serve, and the widest stiff field (each lane holds a D x D W and its LU).
+constexpr int kMaxWideD = 4096;
+constexpr int kMaxRegisterD = 64;
+constexpr int kMaxStiffD = 12;
 }  // namespace rt
 
 }  // namespace gsx

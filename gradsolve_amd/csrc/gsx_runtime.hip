@@ -15,6 +15,8 @@
 //   kRb23          gsx          Rosenbrock23 forward, replay, adjoint (stiff fields)
 //   kRb23Strict    gsx_strict   the same, strict arithmetic
 //   kRb23F32       gsx          the FP32 Rosenbrock23 forward, replay and adjoint (autonomous)
+//   kTsit5Wide     gsx          the wide Tsit5 forward, replay and adjoint (gsx_wide.h): the
+//                               only family of a field with D above 64
 //
 // The recording forward of the primary family (kTsit5, or kRb23 for a stiff field) is
 // compiled at registration so that errors in the source surface there; every other kernel on
@@ -103,6 +105,9 @@ const FamilySpec& spec(Family f) {
         {"adj1", "rb23_adj_f32_kernel<%s, 1>"}, {"adj4", "rb23_adj_f32_kernel<%s, 4>"},
         {"adj8", "rb23_adj_f32_kernel<%s, 8>"}, {"adj16", "rb23_adj_f32_kernel<%s, 16>"},
         {"adj32", "rb23_adj_f32_kernel<%s, 32>"}}},
+      {"gsx", "gsx_wide.h", {},
+       {{"fwd", "tsit5w_fwd_kernel<%s>"}, {"replay", "tsit5w_replay_kernel<%s>"},
+        {"adj", "tsit5w_adj_kernel<%s>"}}},
   };
   return specs[f];
 }
@@ -111,7 +116,7 @@ const char* family_name(Family f) {
   static const char* names[kNumFamilies] = {"tsit5", "tsit5 (strict)", "tsit5 (FP32)",
                                             "rosenbrock23", "rosenbrock23 (strict)",
                                             "rodas5p", "rodas5p (strict)",
-                                            "rosenbrock23 (FP32)"};
+                                            "rosenbrock23 (FP32)", "tsit5 (wide)"};
   return names[f];
 }
 
@@ -194,6 +199,14 @@ int compile(RtField& f, Family fam, const char* key, Module** out) {
   *out = &m;
   if (m.tried) return m.rc;
   m.tried = true;
+  if (f.D > kMaxRegisterD && fam != kTsit5Wide) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "D=%d is above the %d components a lane's registers hold: "
+                  "a field this wide runs on the wide Tsit5 lane only (gsx_tsit5_wide_*)", f.D,
+                  kMaxRegisterD);
+    m.error = buf;
+    return m.rc = GSX_E_UNSUPPORTED;
+  }
   const FamilySpec& sp = spec(fam);
   const char* tmpl = kernel_template(sp, key);
   if (!tmpl) {
@@ -404,6 +417,18 @@ int rb23_saveat_vjp(int field, Family fam, const SaveVjpArgs& a, hipStream_t s) 
   return launch(field, fam, "saveat_vjp", a.n, kFwdBlock, a, s);
 }
 
+int tsit5w_fwd(int field, const WideFwdArgs& w, hipStream_t s) {
+  return launch(field, kTsit5Wide, "fwd", w.f.n, kFwdBlock, w, s);
+}
+
+int tsit5w_replay(int field, const WideReplayArgs& w, hipStream_t s) {
+  return launch(field, kTsit5Wide, "replay", w.r.n, kFwdBlock, w, s);
+}
+
+int tsit5w_adj(int field, const WideAdjArgs& w, hipStream_t s) {
+  return launch(field, kTsit5Wide, "adj", w.a.n, kFwdBlock, w, s);
+}
+
 }  // namespace rt
 }  // namespace gsx
 
@@ -413,12 +438,19 @@ extern "C" {
 
 int gsx_field_register(const char* source, int D, int P, int flags, int* field_id) {
   if (!source || !field_id) return set_error(GSX_E_INVALID, "field_register: null argument");
-  if (D < 1 || D > 64) return set_error(GSX_E_INVALID, "field_register: D=%d outside 1..64", D);
+  if (D < 1 || D > rt::kMaxWideD)
+    return set_error(GSX_E_INVALID, "field_register: D=%d outside 1..%d", D, rt::kMaxWideD);
   if (P < 0 || P > 64) return set_error(GSX_E_INVALID, "field_register: P=%d outside 0..64", P);
   if (flags & ~(GSX_FIELD_STIFF | GSX_FIELD_TIME_DEPENDENT))
     return set_error(GSX_E_INVALID, "field_register: unknown flags 0x%x", flags);
   const bool stiff = (flags & GSX_FIELD_STIFF) != 0;
   const bool autonomous = (flags & GSX_FIELD_TIME_DEPENDENT) == 0;
+  // The stiff lanes hold a D x D W and its LU per lane (gradsolve/warp/warp_rosenbrock.py:67,
+  // 324-340 refuses its fused stiff kernel above the same 12).
+  if (stiff && D > rt::kMaxStiffD)
+    return set_error(GSX_E_INVALID, "field_register: a stiff field of D=%d is above the register "
+                     "limit of the fused stiff lanes (D <= %d: each lane factorises a D x D "
+                     "W in registers)", D, rt::kMaxStiffD);
   std::lock_guard<std::mutex> lk(rt::g_mu);
   auto f = std::make_unique<rt::RtField>();
   f->D = D;
@@ -432,7 +464,8 @@ int gsx_field_register(const char* source, int D, int P, int flags, int* field_i
   const int id = GSX_FIELD_RUNTIME_BASE + static_cast<int>(rt::g_fields.size());
   rt::Module* m = nullptr;
   int rc = stiff ? rt::compile(*f, rt::kRb23, "fwd", &m)
-                 : rt::compile(*f, rt::kTsit5, "fwd_rec", &m);
+                 : D > rt::kMaxRegisterD ? rt::compile(*f, rt::kTsit5Wide, "fwd", &m)
+                                         : rt::compile(*f, rt::kTsit5, "fwd_rec", &m);
   if (rc != GSX_OK) return rt::fail(*m, id);
   if (stiff && !autonomous) {
     rc = rt::compile(*f, rt::kRodas5p, "fwd", &m);

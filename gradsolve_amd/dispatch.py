@@ -16,8 +16,11 @@ The codomain names this build's native engines:
                          gradients above the low-NVAR band and which explicit opt-in reaches).
 
 High-dimensional non-stiff problems route to the reference's general engines
-(``tsit5_replay``, ``diffrax``), which are not part of this build: the API reports them
-instead of silently substituting.
+(``tsit5_replay`` for gradients, ``diffrax`` forward); the API runs both cells on
+
+* ``hip_tsit5_wide``   -- the wide-state Tsit5 lane for fields compiled at run time (any D up
+                         to 4096; state and stages in an HBM workspace), recording the reason
+                         on ``route.reason`` (``alias:tsit5_replay`` / ``diffrax-not-installed``).
 """
 from __future__ import annotations
 
@@ -40,6 +43,9 @@ HIP_ROSENBROCK23 = "hip_rosenbrock23"
 #: field is the Kuramoto kernel, and records why on ``route.reason``.
 HIP_KURAMOTO = "hip_kuramoto"
 HIP_RODAS5P = "hip_rodas5p"
+#: The wide-state Tsit5 lane; not a router output: the API substitutes it for the router's
+#: general non-stiff engines (tsit5_replay, diffrax) above NVAR_CEILING.
+HIP_TSIT5_WIDE = "hip_tsit5_wide"
 TSIT5_REPLAY = "tsit5_replay"
 RODAS5P_REPLAY = "rodas5p_replay"
 DIFFRAX = "diffrax"
@@ -115,11 +121,13 @@ DECISION_MAP: List[DecisionRow] = [
     {"nvar": "high", "stiff": False, "need_grad": False, "dim": 1000, "engine": DIFFRAX,
      "gated_engine": TSIT5_REPLAY,
      "evidence": "Above NVAR_CEILING a per-lane state no longer fits the register file; the "
-                 "reference's general engines serve this cell and are not part of this "
-                 "build."},
+                 "API runs this cell on hip_tsit5_wide (state and stages in an HBM "
+                 "workspace, one trajectory per lane) for fields compiled at run time."},
     {"nvar": "high", "stiff": False, "need_grad": True, "dim": 1000, "engine": TSIT5_REPLAY,
      "gated_engine": None,
-     "evidence": "General-RHS record-and-replay of the reference; not part of this build."},
+     "evidence": "General-RHS record-and-replay of the reference; the API runs it on "
+                 "hip_tsit5_wide: the wide forward with the accepted-h record and "
+                 "checkpoints, then its hand-written adjoint."},
     {"nvar": "low", "stiff": True, "need_grad": False, "dim": 3, "engine": HIP_ROSENBROCK23,
      "gated_engine": DIFFRAX,
      "evidence": "Fused Rosenbrock23: per-lane 3x3 W = I - d h J factored once per step in "

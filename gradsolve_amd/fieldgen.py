@@ -690,8 +690,10 @@ def emit_field(tr: Traced, stiff: bool = False) -> str:
     # f
     em = _Emitter(_inputs(tr, {}))
     em.emit(tr.out)
-    lines += ["  template <class T>",
-              "  GSX_HD static void f(T t, const T* y, const T* p, T* dy, const double*) {",
+    # y / dy (and v / yb below) are any indexable views: plain arrays in the register lanes,
+    # strided workspace rows in the wide lanes
+    lines += ["  template <class T, class YP, class DP>",
+              "  GSX_HD static void f(T t, YP y, const T* p, DP dy, const double*) {",
               "    (void)t; (void)y; (void)p;"]
     lines += em.lines
     lines += [f"    dy[{i}] = {em.ref(o)};" for i, o in enumerate(tr.out)]
@@ -702,8 +704,8 @@ def emit_field(tr: Traced, stiff: bool = False) -> str:
     grads = vjp_nodes(g, tr.out, v, wrt)
     em = _Emitter(_inputs(tr, {f"v{i}": f"v[{i}]" for i in range(D)}))
     em.emit([x for x in grads if x is not None])
-    lines += ["  template <class T>",
-              "  GSX_HD static void vjp(T t, const T* y, const T* p, const T* v, T* yb, T* pb,",
+    lines += ["  template <class T, class YP, class VP, class YBP>",
+              "  GSX_HD static void vjp(T t, YP y, const T* p, VP v, YBP yb, T* pb,",
               "                         const double*) {",
               "    (void)t; (void)y; (void)p; (void)v; (void)yb; (void)pb;"]
     lines += em.lines

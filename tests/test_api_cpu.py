@@ -87,3 +87,39 @@ def test_validation_errors():
         gs.solve(P.LinearDecay(7), np.zeros((4, 7)), np.ones((4, 1)), fused=False)
     with pytest.raises(RuntimeError, match="no CPU engine"):
         gs.solve(P.DiffEqGPULorenz(), e_y0, np.ones((4, 1)), device="cpu")
+
+
+class _AllenCahn100:
+    """A 100-wide lattice (above NVAR_CEILING): the reference's general-engine cells."""
+    name = "allen_cahn_100"
+    dim = 100
+    t0 = 0.0
+    t1 = 1.0
+    is_stiff = False
+
+    def f_jax(self, t, y, p):
+        lap = np.roll(y, -1) - 2.0 * y + np.roll(y, 1)
+        return p[0] * lap + y - y * y * y + p[1]
+
+
+def test_high_nvar_cells_route_to_the_wide_lane():
+    """gradsolve/dispatch.py:256-258 names tsit5_replay (gradient) and diffrax (forward) above
+    NVAR_CEILING; both run on the wide lane once the field is compiled (fused='auto'), with the
+    substitution on route.reason. A built-in field asked for the general lane keeps the
+    register lane."""
+    import gradsolve_amd.api as A
+    prob = _AllenCahn100()
+    params = np.ones((4, 2))
+    pre = []
+    A._translate(prob, params, "auto", pre)
+    assert F.binding_for(prob).source is not None
+    name, reasons = A._resolve("auto", prob, True, None, None, pre)
+    assert name == D.HIP_TSIT5_WIDE and "alias:tsit5_replay" in reasons
+    name, reasons = A._resolve("auto", prob, False, None, None, pre)
+    assert name == D.HIP_TSIT5_WIDE and "diffrax-not-installed" in reasons
+    name, reasons = A._resolve("tsit5_replay", prob, True, None, None)
+    assert name == D.HIP_TSIT5_WIDE
+    name, _ = A._resolve("tsit5_replay", P.DiffEqGPULorenz(), True, None, None)
+    assert name == D.HIP_REPLAY
+    with pytest.raises(ValueError, match="diffrax"):
+        A._resolve("diffrax", prob, False, None, None)

@@ -389,3 +389,41 @@ def test_numpy_select_max_min_clip_sign_trace():
             fd_p[k] = v @ (numpy_piecewise(t, y, p + e) - numpy_piecewise(t, y, p - e)) / 2e-7
         np.testing.assert_allclose(yb, fd_y, rtol=1e-7, atol=1e-7)
         np.testing.assert_allclose(pb, fd_p, rtol=1e-7, atol=1e-7)
+
+
+def allen_cahn(t, y, p):
+    """A wide lattice field: a * (discrete Laplacian) + y - y^3 + b."""
+    a, b = p[0], p[1]
+    lap = np.roll(y, -1) - 2.0 * y + np.roll(y, 1)
+    return a * lap + y - y * y * y + b
+
+
+def test_register_limits_and_the_wide_lane():
+    """Stiff fields stop at D = 12 (each lane of the stiff lanes factorises a D x D W in
+    registers; gradsolve/warp/warp_rosenbrock.py:67,324-340 refuses its fused stiff kernel
+    above the same 12). A non-stiff field may be up to 4096 wide: above 64 it compiles the wide
+    Tsit5 lane at registration, and the register lanes refuse it before any device work."""
+    src = FG.emit_field(FG.trace(lambda t, y, p: -p[0] * y, 13, 1), stiff=True)
+    with pytest.raises(ValueError, match="register limit"):
+        N.field_register(src, 13, 1, N.FIELD_STIFF)
+    src12 = FG.emit_field(FG.trace(lambda t, y, p: -p[0] * y, 12, 1), stiff=True)
+    fid12 = N.field_register(src12, 12, 1, N.FIELD_STIFF)
+    assert N.call("gsx_field_release", fid12) == 0
+    with pytest.raises(ValueError, match="outside 1..4096"):
+        N.field_register(src, 4097, 1, 0)
+    wide = FG.emit_field(FG.trace(allen_cahn, 100, 2))
+    fid = N.field_register(wide, 100, 2, 0)
+    assert N.call("gsx_ckpt_rows", fid, 100) == 100
+    assert N.call("gsx_tsit5_wide_ws_rows", 0, 1) == 10
+    assert N.call("gsx_tsit5_wide_ws_rows", 1, 8) == 6 * 8 + 14
+    fake = 4096  # never dereferenced: the call stops before any device work
+    with pytest.raises(N.NativeError, match="wide Tsit5 lane only"):
+        N.call("gsx_tsit5_fwd", fid, 4, 100, 2, None, fake, fake, 0.0, 1.0, None, 1e-8, 1e-8,
+               10, fake, fake, fake, fake, None, 0, None, 1, 0, None)
+    with pytest.raises(ValueError, match="workspace"):
+        N.call("gsx_tsit5_wide_fwd", fid, 4, 100, 2, None, fake, fake, 0.0, 1.0, None, 1e-8,
+               1e-8, 10, fake, fake, fake, fake, None, 0, None, 1, None, None)
+    with pytest.raises(N.NativeError, match="built in"):
+        N.call("gsx_tsit5_wide_fwd", N.FIELD_LORENZ, 4, 3, 1, None, fake, fake, 0.0, 1.0, None,
+               1e-8, 1e-8, 10, fake, fake, fake, fake, None, 0, None, 1, fake, None)
+    assert N.call("gsx_field_release", fid) == 0
