@@ -63,10 +63,11 @@ enum gsx_field_flags { GSX_FIELD_STIFF = 1, GSX_FIELD_TIME_DEPENDENT = 2 };
  * GSX_STRICT: contraction off, so every operation rounds once in the reference loop's
  * textual order (gradsolve/solvers/tsit5_replay.py:100-124); the recorded tape then matches
  * an IEEE-sequential host loop up to the last-ulp differences of the device pow.
- * GSX_FP32 (Tsit5 only): stages, the field and its adjoint in FP32 with a Kahan-compensated
- * state and compensated adjoint accumulators; time, step size, error norm, controller and tape
- * stay FP64; checkpoints hold the compensated pair in the bytes of an FP64 checkpoint.
- * The two flags are exclusive. */
+ * GSX_FP32 (Tsit5 and Rosenbrock23, autonomous fields): stages, the field and its adjoint
+ * in FP32 with a Kahan-compensated state and compensated adjoint accumulators; time, step
+ * size, error norm, controller and tape stay FP64, and on Rosenbrock23 so do W = I - d h J, its
+ * LU and the triangular solves; checkpoints hold the compensated pair in the bytes of an FP64
+ * checkpoint. The two flags are exclusive. */
 enum gsx_flags { GSX_STRICT = 1, GSX_FP32 = 2 };
 
 /* Registers a field given as HIP C++ source and returns its id in *field_id. The id is
@@ -75,18 +76,22 @@ enum gsx_flags { GSX_STRICT = 1, GSX_FP32 = 2 };
  * layout queries. The source defines `struct Field` (the library places it in a namespace of
  * its own, where the gsx_core.h helpers are visible) with the members of the built-in fields:
  *   static constexpr int D, P;  static constexpr bool autonomous;
- *   template <class T> GSX_HD static void f(T t, const T* y, const T* p, T* dy,
- *                                           const double* aux);
- *   template <class T> GSX_HD static void vjp(T t, const T* y, const T* p, const T* v,
- *                                             T* yb, T* pb, const double* aux);
- * (vjp adds v^T df/dy to yb and v^T df/dp to pb), and for stiff fields
+ *   template <class T, class YP, class DP> GSX_HD static void f(T t, YP y, const T* p,
+ *                                                             DP dy, const double* aux);
+ *   template <class T, class YP, class VP, class YBP> GSX_HD static void vjp(
+ *       T t, YP y, const T* p, VP v, YBP yb, T* pb, const double* aux);
+ * (vjp adds v^T df/dy to yb and v^T df/dp to pb; y, dy, v and yb are indexed with [i]: arrays
+ * in the register lanes, strided workspace views in the wide lane), and for stiff fields
  *   template <class T> GSX_HD static void jac(T t, const T* y, const T* p, T* J,
  *                                             const double* aux);       (J row-major D x D)
  *   template <class T> GSX_HD static void jac_vjp(const T* y, const T* p, const T* Jb,
  *                                                 T* yb, T* pb, const double* aux);
- * D, P and autonomy must agree with the arguments and flags. The kernels are compiled for
- * gfx950 with hiprtc, one per program: the recording forward (Tsit5, or Rosenbrock23 for a
- * stiff field) now, so that a compile error returns GSX_E_INVALID with the compiler's log in
+ * D, P and autonomy must agree with the arguments and flags. D is at most 4096; a field wider
+ * than 64 runs on the wide Tsit5 lane only (gsx_tsit5_wide_*), and a stiff field is at most
+ * 12 wide (each lane of the stiff lanes factorises a D x D W in registers; the reference
+ * refuses its fused stiff kernel above the same 12, gradsolve/warp/warp_rosenbrock.py:67,
+ * 324-340). The kernels are compiled for gfx950 with hiprtc, one per program: the recording
+ * forward (Tsit5, the wide Tsit5 lane above D = 64, or Rosenbrock23 for a stiff field) now, so that a compile error returns GSX_E_INVALID with the compiler's log in
  * gsx_last_error(); every other kernel (replay, adjoints, strict and FP32 variants) on its
  * first launch. A segment adjoint holds C checkpoint rows per lane in LDS, so for wide fields
  * checkpoint intervals whose kernel would need more than 64 KiB per wavefront are rejected
@@ -185,6 +190,34 @@ int gsx_tsit5_adjoint_saveat(int field, int64_t n, int D, int P, const double* a
                              const double* save_t, const double* states, const double* ysbar,
                              const double* ybar, double* mu, double* y0bar, double* pbar,
                              int flags, void* stream);
+
+/* ---- Wide-state Tsit5 lane (fields registered at run time, D up to 4096) ------------------
+ * The reference routes non-stiff problems above NVAR_CEILING = 64 to its general
+ * record-and-replay engine (gradsolve/dispatch.py:256-258: record_tsit5_jax with the
+ * vmapped while-loop recorder, gradsolve/solvers/record_jax.py:171-213, and jax.grad of the
+ * replay scan, gradsolve/solvers/tsit5_replay.py:127-232). These entry points replace it for
+ * fields compiled with gsx_field_register: one trajectory per lane, the state, the stage
+ * derivatives and the stage inputs in a caller-owned workspace ws [rows][D][n] f64 (rows from
+ * gsx_tsit5_wide_ws_rows: which 0 for the forward and the replay, which 1 with the checkpoint
+ * interval for the adjoint). Arguments, layouts, statuses, tape and checkpoints otherwise as
+ * gsx_tsit5_fwd / gsx_tsit5_replay / gsx_tsit5_adjoint (default arithmetic only). A field
+ * wider than 64 runs on this lane only; built-in fields are rejected (GSX_E_UNSUPPORTED). */
+int64_t gsx_tsit5_wide_ws_rows(int which, int32_t ckpt_interval);
+int gsx_tsit5_wide_fwd(int field, int64_t n, int D, int P, const double* aux,
+                       const double* y0, const double* p, double t0, double t1,
+                       const double* t1_per, double rtol, double atol, int64_t max_steps,
+                       double* y_final, int32_t* n_acc, int32_t* n_rej, int32_t* status,
+                       double* tape, int64_t tape_cap, double* ckpt, int32_t ckpt_interval,
+                       double* ws, void* stream);
+int gsx_tsit5_wide_replay(int field, int64_t n, int D, int P, const double* aux,
+                          const double* y0, const double* p, double t0, const double* tape,
+                          const int32_t* n_acc, double* y_final, double* ckpt,
+                          int32_t ckpt_interval, double* ws, void* stream);
+int gsx_tsit5_wide_adjoint(int field, int64_t n, int D, int P, const double* aux,
+                           const double* y0, const double* p, double t0, const double* tape,
+                           int64_t tape_cap, const int32_t* n_acc, const double* ckpt,
+                           int32_t ckpt_interval, const double* ybar, double* y0bar,
+                           double* pbar, double* ws, void* stream);
 
 /* gsx_tsit5_adjoint_saveat without the caller's mu buffer: the re-steps' cotangents go to a
  * library-owned device array of 2^23 doubles (64 MiB) per device, so the call allocates
