@@ -96,10 +96,10 @@ constexpr double kTsit5Exp = -0.2;
 // The strict variant (GSX_STRICT_ARITH, contraction off) calls the correctly rounded IEEE
 // division and the math library's pow, as the reference loop does. The default variant
 // computes x^e for the controller exponents e = -0.2 (Tsit5) and e = -1/3 (Rosenbrock23)
-// from a single-precision hardware seed (2^(e log2 x)) refined by two Newton steps on
-// g(y) = x y^m - 1 (m = 5 or 3) in double precision, then applies the factor x^(e + 1/m)
+// from a single-precision hardware seed (2^(e log2 x)) refined in double precision against
+// g(y) = x y^m - 1 (m = 5 or 3; 10 for the squared norm), with the factor x^(e + 1/m)
 // = exp((e + 1/m) ln x) that separates the binary64 exponent from -1/m; the result is
-// within a few ulp of the correctly rounded power. Inputs are first clamped to
+// within an ulp of the correctly rounded power. Inputs are first clamped to
 // [1e-5, 1e4]: outside that range the controller factor 0.9 x^e saturates at the
 // [0.2, 5] clamp for both exponents, so the step size is unchanged.
 // ---------------------------------------------------------------------------------------
@@ -131,21 +131,27 @@ GSX_HD float fast_exp2f(float x) { return std::exp2(x); }
 
 // x^(-1/m) * exp(-c ln x) for positive finite x, m in {3, 5, 10}: see above. For m = 10 the
 // input is a squared norm and the clamp is squared with it.
+//
+// The refinement is one step of the binomial series rather than two Newton steps: with the
+// seed y (relative error e below 1e-6) and u = x y^m - 1 (about m e), the root is
+// y (1 + u)^(-1/m) = y (1 - u/m + (m+1)/(2 m^2) u^2 - O(u^3)), and the cubic term (below
+// 1e-17 relative) is dropped. The exponent factor exp(-c ln x) ~ 1 - c ln x enters the same
+// final fused multiply-add, so the step costs m's powers, three fused multiply-adds and the
+// final one.
 template <int M>
 GSX_HD double ctrl_pow_newton(double x, double c) {
   constexpr double lo = M == 10 ? 1e-10 : 1e-5, hi = M == 10 ? 1e8 : 1e4;
+  constexpr double c1 = -1.0 / M, c2 = (M + 1.0) / (2.0 * M * M);
   const double xc = nn_min(nn_max(x, lo), hi);
   const float lf = fast_log2f(static_cast<float>(xc));
-  double y = static_cast<double>(fast_exp2f(lf * (-1.0f / M)));
-  GSX_UNROLL for (int it = 0; it < 2; ++it) {
-    const double y2 = y * y;
-    const double y5 = y2 * y2 * y;
-    const double ym = (M == 10) ? y5 * y5 : (M == 5) ? y5 : y2 * y;
-    const double r = std::fma(xc, ym, -1.0);
-    y = std::fma(y * (-1.0 / M), r, y);
-  }
-  const double lnx = static_cast<double>(lf) * 0.6931471805599453;
-  return std::fma(y, -c * lnx, y);
+  const double y = static_cast<double>(fast_exp2f(lf * (-1.0f / M)));
+  const double y2 = y * y;
+  const double y5 = y2 * y2 * y;
+  const double ym = (M == 10) ? y5 * y5 : (M == 5) ? y5 : y2 * y;
+  const double u = std::fma(xc, ym, -1.0);
+  const double d = static_cast<double>(lf) * (-c * 0.6931471805599453);  // -c ln x
+  const double q = std::fma(u, std::fma(u, c2, c1), d);
+  return std::fma(y, q, y);
 }
 
 // x^(-0.2) with -0.2 in binary64: the double nearest 0.2 exceeds 1/5 by

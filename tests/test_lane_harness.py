@@ -99,8 +99,9 @@ def test_lane_tape_full_and_exhausted(lh):
 
 
 def test_controller_power_fast_variant_accuracy():
-    """The default variant's controller power (hardware f32 seed + two f64 Newton steps) is
-    within 1 ulp of the C library's pow, and the clamped step factor within 2 ulp."""
+    """The default variant's controller power (hardware f32 seed + one second-order step of
+    the binomial series in f64) is within 1 ulp of the C library's pow, and the clamped step
+    factor within 2 ulp."""
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
     F = ctypes.CDLL(os.path.join(HERE, "native_cpu", "liblane_harness_fast.so"))
     F.lh_ctrl_pow.restype = ctypes.c_double
