@@ -256,7 +256,7 @@ def _plain_hip_arm(args, y0, p, kw, dev, S_max):
                    tape.data_ptr(), stages.data_ptr(), na.data_ptr(), ybar.data_ptr(),
                    g0.data_ptr(), gp.data_ptr(), s)
 
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):  # at least one call before the status check
             step()
         torch.cuda.synchronize()
         if int((st != 0).sum().item()):

@@ -4,6 +4,9 @@
 // inputs z_1..z_6 of every accepted step, and the reverse reads them back and applies the
 // stage VJPs (the VJP of gradsolve/solvers/tsit5_step.py:80-102 without re-evaluating the
 // field). Layout: stages [cap][6][D][n], element (s, i, d, j) at ((s*6 + i)*D + d)*n + j.
+// The reverse sweep walks a wavefront's lanes in step, from the wavefront's largest count
+// down (a lane joins once the walk reaches its own last step), so that every stage-tape read
+// of the wavefront is one contiguous row segment, as in the fused adjoints.
 #include <hip/hip_runtime.h>
 
 #include "gsx_internal.h"
@@ -88,7 +91,7 @@ __device__ void plain_fwd_lane(const PlainArgs& a, int64_t j) {
 }
 
 template <class F>
-__device__ void plain_adj_lane(const PlainArgs& a, int64_t j) {
+__device__ void plain_adj_lane(const PlainArgs& a, int64_t j, int32_t smax) {
   constexpr int D = F::D, P = F::P;
   using namespace tab;
   constexpr double A[6][6] = {{0, 0, 0, 0, 0, 0},
@@ -106,7 +109,8 @@ __device__ void plain_adj_lane(const PlainArgs& a, int64_t j) {
   }
   GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
   const int32_t na = a.n_acc[j];
-  for (int32_t s = na - 1; s >= 0; --s) {
+  for (int32_t s = smax - 1; s >= 0; --s) {
+    if (s >= na) continue;
     const double h = a.tape[static_cast<int64_t>(s) * n + j];
     const double* row = a.stages + static_cast<int64_t>(s) * 6 * D * n + j;
     double hl[D], lam_new[D], kb[6][D];
@@ -145,7 +149,13 @@ __global__ __launch_bounds__(kPlainBlock) void plain_fwd_kernel(PlainArgs a) {
 template <class F>
 __global__ __launch_bounds__(kPlainBlock) void plain_adj_kernel(PlainArgs a) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kPlainBlock + threadIdx.x;
-  if (j < a.n) plain_adj_lane<F>(a, j);
+  const bool valid = j < a.n;
+  int32_t smax = valid ? a.n_acc[j] : 0;  // the wavefront's largest count
+  GSX_UNROLL for (int off = 32; off >= 1; off >>= 1) {
+    const int32_t o = __shfl_xor(smax, off);
+    smax = o > smax ? o : smax;
+  }
+  if (valid) plain_adj_lane<F>(a, j, smax);
 }
 
 unsigned plain_grid(int64_t n) {
