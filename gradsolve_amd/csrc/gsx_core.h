@@ -83,6 +83,20 @@ constexpr double e7 = -1.0 / 66.0;
 constexpr int kFwdBlock = 256;
 constexpr int kAdjBlock = 64;
 
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+// The largest v over the wavefront's 64 lanes; every lane of the wavefront must call it. The
+// adjoint kernels pass the wavefront's largest segment count to their lanes, so that the lanes
+// walk the segments in step and each checkpoint and tape read of the wavefront is one
+// contiguous row segment.
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+  GSX_UNROLL for (int off = 32; off >= 1; off >>= 1) {
+    const int32_t o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+#endif
+
 // Step-size controller constants (Hairer-Norsett-Wanner II.4; gradsolve/solvers/tsit5_step.py:26).
 constexpr double kSafety = 0.9;
 constexpr double kErrFloor = 1e-10;
@@ -995,8 +1009,17 @@ GSX_HD void inject_saves(const AdjArgs& a, int64_t j, int32_t s, int32_t& q, dou
   }
 }
 
+// Segment count of lane j: checkpoint m starts accepted step m * C.
+GSX_HD int32_t segment_count(const AdjArgs& a, int64_t j) {
+  return (a.n_acc[j] + a.ckpt_interval - 1) / a.ckpt_interval;
+}
+
+// The segment adjoints walk from segment max(mtop, own count) - 1 down and skip the segments
+// above their own count: with mtop the wavefront's largest count (wave_max) the lanes run in
+// step; mtop = 0 walks the lane's own segments only (the CPU harness).
 template <class F>
-GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
+GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride,
+                           int32_t mtop = 0) {
   constexpr int D = F::D, P = F::P;
   constexpr int DCK = ckpt_rows<F>();
   const int64_t n = a.n;
@@ -1010,7 +1033,8 @@ GSX_HD void tsit5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sst
   const int32_t na = a.n_acc[j];
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
-  for (int32_t m = nseg - 1; m >= 0; --m) {
+  for (int32_t m = (mtop > nseg ? mtop : nseg) - 1; m >= 0; --m) {
+    if (m >= nseg) continue;
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     double y[D];

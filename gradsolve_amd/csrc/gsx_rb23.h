@@ -327,7 +327,8 @@ GSX_HD void rb23_replay_lane(const ReplayArgs& a, int64_t j) {
 }
 
 template <class F>
-GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
+GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride,
+                          int32_t mtop = 0) {
   constexpr int D = F::D, P = F::P;
   constexpr int DCK = ckpt_rows<F>();
   const int64_t n = a.n;
@@ -341,7 +342,8 @@ GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstr
   const int32_t na = a.n_acc[j];
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
-  for (int32_t m = nseg - 1; m >= 0; --m) {
+  for (int32_t m = (mtop > nseg ? mtop : nseg) - 1; m >= 0; --m) {
+    if (m >= nseg) continue;  // the wavefront's walk has not reached this lane
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     double y[D];

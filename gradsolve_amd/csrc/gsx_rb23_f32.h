@@ -273,7 +273,8 @@ GSX_HD void rb23_step_vjp_f32(const float* y, const float* lo, double h, const f
 // The segment buffer is the FP64 lane's: each double cell holds one component's compensated
 // pair (s, lo) as two floats, so a slot takes D doubles per lane.
 template <class F>
-GSX_HD void rb23_adj_lane_f32(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
+GSX_HD void rb23_adj_lane_f32(const AdjArgs& a, int64_t j, double* seg, int64_t sstride,
+                              int32_t mtop = 0) {
   constexpr int D = F::D, P = F::P;
   const int64_t n = a.n;
   float pf[P], pb[P], pb_lo[P], lam[D], lam_lo[D];
@@ -292,7 +293,8 @@ GSX_HD void rb23_adj_lane_f32(const AdjArgs& a, int64_t j, double* seg, int64_t 
   const int32_t na = a.n_acc[j];
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
-  for (int32_t m = nseg - 1; m >= 0; --m) {
+  for (int32_t m = (mtop > nseg ? mtop : nseg) - 1; m >= 0; --m) {
+    if (m >= nseg) continue;  // the wavefront's walk has not reached this lane
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     float y[D], lo[D];

@@ -26,8 +26,9 @@ template <class F, int CMAX>
 __global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
   __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < a.n ? segment_count(a, j) : 0);
   if (j >= a.n) return;
-  rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+  rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
 // Dense output on a recorded mesh and the VJPs of its re-steps, one trajectory per lane.
@@ -64,8 +65,9 @@ template <class F, int CMAX>
 __global__ __launch_bounds__(kAdjBlock) void rb23_adj_f32_kernel(AdjArgs a) {
   __shared__ double seg[CMAX * F::D * kAdjBlock];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < a.n ? segment_count(a, j) : 0);
   if (j >= a.n) return;
-  rb23_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock);
+  rb23_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
 }  // namespace GSX_NS

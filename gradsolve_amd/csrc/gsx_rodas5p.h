@@ -350,7 +350,8 @@ GSX_HD bool r5_dense_cotangents(const AdjArgs& a, int64_t j, int32_t s, double h
 // the Tsit5 lane). With a.ysbar: dense-output cotangents (save_step / save_t / ts as the dense
 // saveat lane wrote them; saves at the final time and at or before t0 are handled here).
 template <class F>
-GSX_HD void r5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride) {
+GSX_HD void r5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstride,
+                        int32_t mtop = 0) {
   constexpr int D = F::D, P = F::P;
   constexpr int DCK = ckpt_rows<F>();
   const int64_t n = a.n;
@@ -373,7 +374,8 @@ GSX_HD void r5_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstrid
   }
   const int32_t C = a.ckpt_interval;
   const int32_t nseg = (na + C - 1) / C;
-  for (int32_t m = nseg - 1; m >= 0; --m) {
+  for (int32_t m = (mtop > nseg ? mtop : nseg) - 1; m >= 0; --m) {
+    if (m >= nseg) continue;  // the wavefront's walk has not reached this lane
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     double y[D];

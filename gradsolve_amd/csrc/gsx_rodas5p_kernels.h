@@ -25,8 +25,9 @@ template <class F, int CMAX>
 __global__ __launch_bounds__(kAdjBlock) void r5_adj_kernel(AdjArgs a) {
   __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < a.n ? segment_count(a, j) : 0);
   if (j >= a.n) return;
-  r5_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+  r5_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
 template <class F>

@@ -23,8 +23,9 @@ template <class F, int CMAX>
 __global__ __launch_bounds__(kAdjBlock) void tsit5_adj_kernel(AdjArgs a) {
   __shared__ double seg[CMAX * ckpt_rows<F>() * kAdjBlock];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < a.n ? segment_count(a, j) : 0);
   if (j >= a.n) return;
-  tsit5_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock);
+  tsit5_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
 template <class F>
@@ -45,8 +46,9 @@ template <class F, int CMAX>
 __global__ __launch_bounds__(kAdjBlock) void tsit5_adj_f32_kernel(AdjArgs a) {
   __shared__ double seg[CMAX * F::D * kAdjBlock];
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kAdjBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < a.n ? segment_count(a, j) : 0);
   if (j >= a.n) return;
-  tsit5_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock);
+  tsit5_adj_lane_f32<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
 template <class F>
@@ -74,11 +76,7 @@ void tsit5_adj_pair_kernel(AdjArgs a) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
   const bool valid = j < a.n;
   const int32_t na = valid ? a.n_acc[j] : 0;
-  int32_t mmax = (na + 1) / 2;
-  GSX_UNROLL for (int off = 32; off >= 1; off >>= 1) {
-    const int32_t o = __shfl_xor(mmax, off);
-    mmax = o > mmax ? o : mmax;
-  }
+  const int32_t mmax = wave_max((na + 1) / 2);
   if (!valid) return;
   tsit5_adj_pair_lane<F, kSaves>(a, j, na, mmax);
 }

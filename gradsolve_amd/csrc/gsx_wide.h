@@ -280,9 +280,10 @@ __device__ void tsit5w_replay_lane(const WideReplayArgs& w, int64_t j) {
 // Adjoint (the VJP of the frozen-h replay, tsit5_step_vjp): each segment of C accepted steps
 // is re-integrated from its checkpoint, keeping every step's start state and stage inputs
 // z2..z6, then walked in reverse: kbar_s = h b_s lam, and for s = 6..1
-// zbar = (df/dz_s)^T kbar_s, lam += zbar, kbar_r += h a_sr zbar (r < s).
+// zbar = (df/dz_s)^T kbar_s, lam += zbar, kbar_r += h a_sr zbar (r < s). The segments are
+// walked in step across the wavefront from mtop, its largest segment count (tsit5_adj_lane).
 template <class F>
-__device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j) {
+__device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j, int32_t mtop) {
   constexpr int D = F::D, P = F::P;
   constexpr int DCK = ckpt_rows<F>();
   const AdjArgs& a = w.a;
@@ -306,7 +307,8 @@ __device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j) {
   for (int d = 0; d < D; ++d) lam[d] = a.ybar[static_cast<int64_t>(d) * n + j];
   const int32_t na = a.n_acc[j];
   const int32_t nseg = (na + C - 1) / C;
-  for (int32_t m = nseg - 1; m >= 0; --m) {
+  for (int32_t m = (mtop > nseg ? mtop : nseg) - 1; m >= 0; --m) {
+    if (m >= nseg) continue;  // the wavefront's walk has not reached this lane
     const int32_t s0 = m * C;
     const int32_t s1 = (s0 + C < na) ? s0 + C : na;
     const double* base = a.ckpt + static_cast<int64_t>(m) * DCK * n + j;
@@ -385,8 +387,9 @@ __global__ __launch_bounds__(kFwdBlock) void tsit5w_replay_kernel(WideReplayArgs
 template <class F>
 __global__ __launch_bounds__(kFwdBlock) void tsit5w_adj_kernel(WideAdjArgs w) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  const int32_t mtop = wave_max(j < w.a.n ? segment_count(w.a, j) : 0);
   if (j >= w.a.n) return;
-  tsit5w_adj_lane<F>(w, j);
+  tsit5w_adj_lane<F>(w, j, mtop);
 }
 
 }  // namespace GSX_NS
