@@ -158,7 +158,8 @@ def _stiff_lane(args, dev, method, name):
            "mean_rejected": round(rej / n, 2), "max_accepted": S}
     wvec = torch.tensor([1.0, 10.0, 1.0], dtype=torch.float64, device=dev)[:, None]
     best = None
-    for C in (4, 8, 16):
+    # C = 2 is the Rosenbrock23 pair adjoint; the Rodas5P lane has segment adjoints only
+    for C in ((2, 4, 8, 16) if method == "rb23" else (4, 8, 16)):
         ws = H.Workspace(N.FIELD_ROBERTSON, 3, 3, n, S + 16, C, dev)
         ybar = (wvec * torch.ones((3, n), dtype=torch.float64, device=dev)).contiguous()
 

@@ -197,20 +197,28 @@ GSX_HD double ctrl_pow_rb23(double x) {
 #endif
 }
 
+// The seed of a refined reciprocal: the hardware's double-precision reciprocal on the device.
+// Host builds seed with the single-precision reciprocal, of similar accuracy, so that the
+// refinement can be tested on the CPU; where 1/b lies outside the single-precision normal
+// range (which the device's seed covers) they take the double quotient instead.
+GSX_HD double recip_seed(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(b);
+#else
+  const double q = 1.0 / b;
+  const double m = std::fabs(q);
+  return (m > 1.2e-38 && m < 3.4e38) ? static_cast<double>(static_cast<float>(q)) : q;
+#endif
+}
+
 // a / b for the error norm (b >= atol > 0): the IEEE quotient in the strict variant; in the
 // default variant the hardware reciprocal refined by one Newton step, the product a r, and
 // one Markstein correction q + r (a - b q) of the quotient.
-// (Host builds seed with the single-precision reciprocal, of similar accuracy, so that the
-// refinement can be tested on the CPU.)
 GSX_HD double norm_div(double a, double b) {
 #if defined(GSX_STRICT_ARITH)
   return a / b;
 #else
-#if defined(__HIP_DEVICE_COMPILE__)
-  double r = __builtin_amdgcn_rcp(b);
-#else
-  double r = static_cast<double>(1.0f / static_cast<float>(b));
-#endif
+  double r = recip_seed(b);
   r = std::fma(r, std::fma(-b, r, 1.0), r);
   const double q = a * r;
   return std::fma(std::fma(-b, q, a), r, q);

@@ -20,14 +20,24 @@ GSX_HD double rb_e32() { return 6.0 + std::sqrt(2.0); }
 // and the pivot search use compile-time indices with data-dependent selects, so nothing is
 // indexed dynamically (no scratch memory). piv[c] is the row swapped with row c at step c.
 // The default variant keeps the reciprocal of U's diagonal in place of the diagonal after
-// factoring: one division per pivot, and the multipliers and every triangular solve multiply
-// by it (within an ulp or so of the quotients). The strict variant keeps U's diagonal and
+// factoring: one refined reciprocal per pivot (pivot_recip), and the multipliers and every
+// triangular solve multiply by it (within an ulp or so of the quotients). The strict variant keeps U's diagonal and
 // divides, as LAPACK's factorisation and triangular solves do.
 #if defined(GSX_STRICT_ARITH)
 constexpr bool kLuRecipDiag = false;
 #else
 constexpr bool kLuRecipDiag = true;
 #endif
+
+// 1/b for a pivot of the default variant: the hardware reciprocal refined by two Newton steps
+// (each squares the relative error, so the result is within an ulp or so of 1/b) in place of
+// the correctly rounded division's scale-and-fixup sequence. A zero pivot gives a non-finite
+// result, as the division does.
+GSX_HD double pivot_recip(double b) {
+  double r = recip_seed(b);
+  r = std::fma(r, std::fma(-b, r, 1.0), r);
+  return std::fma(r, std::fma(-b, r, 1.0), r);
+}
 
 template <int D>
 GSX_HD void lu_factor(double (&a)[D * D], int (&piv)[D]) {
@@ -50,7 +60,7 @@ GSX_HD void lu_factor(double (&a)[D * D], int (&piv)[D]) {
         a[r * D + k] = sw ? x : z;
       }
     }
-    const double inv = kLuRecipDiag ? 1.0 / a[c * D + c] : 0.0;
+    const double inv = kLuRecipDiag ? pivot_recip(a[c * D + c]) : 0.0;
     GSX_UNROLL for (int r = c + 1; r < D; ++r) {
       const double l = kLuRecipDiag ? a[r * D + c] * inv : a[r * D + c] / a[c * D + c];
       a[r * D + c] = l;
@@ -170,23 +180,47 @@ GSX_HD void rb23_advance(double t, double* y, double h, const double* p, const d
   GSX_UNROLL for (int i = 0; i < D; ++i) y[i] = y[i] + h * (k2[i] + k1[i]);
 }
 
-// Adjoint of rb23_advance at (t, y, h): lam <- (dy_next/dy)^T lam; pb += (dy_next/dp)^T lam.
-// Differentiates through the solves (x = W^-1 r: rbar = W^-T xbar, Wbar -= rbar x^T) and
-// through W = I - d h J(y, p) (Jbar = -d h Wbar, contracted with dJ/dy and dJ/dp).
+// What the reverse of one step needs from its forward: the LU of W, k1 = W^-1 f(y) and
+// x2 = W^-1 (f(u) - k1) (so that k2 = x2 + k1 and y_next = y + h (x2 + k1)).
 template <class F>
-GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, double* lam,
-                          double* pb, const double* aux) {
-  constexpr int D = F::D;
+struct Rb23Step {
+  static constexpr int D = F::D;
   double W[D * D];
   int piv[D];
-  rb23_build_w<F>(t, y, h, p, aux, W, piv);
-  double F0[D], F1[D], k1[D], x2[D], u[D], r[D];
+  double k1[D], x2[D];
+};
+
+// The forward of one replay step at (t, y, h), keeping what its VJP needs; y_next (when
+// given) is rb23_advance's result bit for bit.
+template <class F>
+GSX_HD void rb23_step_state(double t, const double* y, double h, const double* p,
+                            const double* aux, Rb23Step<F>& w, double* y_next) {
+  constexpr int D = F::D;
+  rb23_build_w<F>(t, y, h, p, aux, w.W, w.piv);
+  double F0[D], F1[D], u[D], r[D];
   F::f(t, y, p, F0, aux);
-  lu_solve<D>(W, piv, F0, k1);
-  GSX_UNROLL for (int i = 0; i < D; ++i) u[i] = y[i] + 0.5 * h * k1[i];
+  lu_solve<D>(w.W, w.piv, F0, w.k1);
+  GSX_UNROLL for (int i = 0; i < D; ++i) u[i] = y[i] + 0.5 * h * w.k1[i];
   F::f(t, u, p, F1, aux);
-  GSX_UNROLL for (int i = 0; i < D; ++i) r[i] = F1[i] - k1[i];
-  lu_solve<D>(W, piv, r, x2);
+  GSX_UNROLL for (int i = 0; i < D; ++i) r[i] = F1[i] - w.k1[i];
+  lu_solve<D>(w.W, w.piv, r, w.x2);
+  if (y_next) {
+    GSX_UNROLL for (int i = 0; i < D; ++i) y_next[i] = y[i] + h * (w.x2[i] + w.k1[i]);
+  }
+}
+
+// Adjoint of rb23_advance at (t, y, h) from its forward state: lam <- (dy_next/dy)^T lam;
+// pb += (dy_next/dp)^T lam. Differentiates through the solves (x = W^-1 r: rbar = W^-T xbar,
+// Wbar -= rbar x^T) and through W = I - d h J(y, p) (Jbar = -d h Wbar, contracted with dJ/dy
+// and dJ/dp). With h = 0 it leaves lam and pb unchanged (every cotangent it adds is a zero;
+// only the sign of a zero may change).
+template <class F>
+GSX_HD void rb23_step_vjp_state(double t, const double* y, double h, const double* p,
+                                const Rb23Step<F>& w, double* lam, double* pb,
+                                const double* aux) {
+  constexpr int D = F::D;
+  double u[D];
+  GSX_UNROLL for (int i = 0; i < D; ++i) u[i] = y[i] + 0.5 * h * w.k1[i];
   // ynew = y + h (x2 + k1)
   double yb[D], k1b[D], x2b[D], r2b[D], ub[D], F0b[D], Wb[D * D];
   GSX_UNROLL for (int i = 0; i < D; ++i) {
@@ -194,9 +228,9 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
     x2b[i] = h * lam[i];
     k1b[i] = h * lam[i];
   }
-  lu_solve_t<D>(W, piv, x2b, r2b);
+  lu_solve_t<D>(w.W, w.piv, x2b, r2b);
   GSX_UNROLL for (int i = 0; i < D; ++i) {
-    GSX_UNROLL for (int j = 0; j < D; ++j) Wb[i * D + j] = -r2b[i] * x2[j];
+    GSX_UNROLL for (int j = 0; j < D; ++j) Wb[i * D + j] = -r2b[i] * w.x2[j];
   }
   GSX_UNROLL for (int i = 0; i < D; ++i) {
     k1b[i] -= r2b[i];
@@ -207,9 +241,9 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
     yb[i] += ub[i];
     k1b[i] += 0.5 * h * ub[i];
   }
-  lu_solve_t<D>(W, piv, k1b, F0b);
+  lu_solve_t<D>(w.W, w.piv, k1b, F0b);
   GSX_UNROLL for (int i = 0; i < D; ++i) {
-    GSX_UNROLL for (int j = 0; j < D; ++j) Wb[i * D + j] -= F0b[i] * k1[j];
+    GSX_UNROLL for (int j = 0; j < D; ++j) Wb[i * D + j] -= F0b[i] * w.k1[j];
   }
   F::vjp(t, y, p, F0b, yb, pb, aux);
   const double dh = h * rb_d();
@@ -217,6 +251,15 @@ GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, 
   GSX_UNROLL for (int i = 0; i < D * D; ++i) Jb[i] = -dh * Wb[i];
   F::jac_vjp(t, y, p, Jb, yb, pb, aux);
   GSX_UNROLL for (int i = 0; i < D; ++i) lam[i] = yb[i];
+}
+
+// Adjoint of rb23_advance at (t, y, h): the step's forward state, then its reverse.
+template <class F>
+GSX_HD void rb23_step_vjp(double t, const double* y, double h, const double* p, double* lam,
+                          double* pb, const double* aux) {
+  Rb23Step<F> w;
+  rb23_step_state<F>(t, y, h, p, aux, w, nullptr);
+  rb23_step_vjp_state<F>(t, y, h, p, w, lam, pb, aux);
 }
 
 // Rosenbrock23 as the step policy of the dense-output lanes (saveat_lane / saveat_vjp_lane):
@@ -371,6 +414,64 @@ GSX_HD void rb23_adj_lane(const AdjArgs& a, int64_t j, double* seg, int64_t sstr
     }
   }
   inject_saves<D>(a, j, -1, qs, lam);
+  GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
+  GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
+}
+
+// The adjoint with checkpoints every 2 accepted steps (the Tsit5 pair adjoint's schedule,
+// tsit5_adj_pair_segment): for segment m one forward state of step 2m (W's LU, k1, x2) gives
+// y_{2m+1}; step 2m+1's forward state is formed for its own VJP, then step 2m's VJP reuses the
+// state kept from the advance, so each step's W is built and factored once and nothing is
+// re-integrated. A missing second step (odd n_acc) runs with h = 0, which leaves lam and pb
+// unchanged, so every lane follows the same control flow; `m` is wavefront-uniform in the
+// kernel. kSaves as in the Tsit5 pair adjoint.
+template <class F, bool kSaves = true>
+GSX_HD void rb23_adj_pair_segment(const AdjArgs& a, int64_t j, int32_t m, int32_t na,
+                                  const double* p, double* lam, double* pb, int32_t& qs) {
+  constexpr int D = F::D;
+  constexpr int DCK = ckpt_rows<F>();
+  const int64_t n = a.n;
+  const int32_t s0 = 2 * m;
+  const double* base = a.ckpt + static_cast<int64_t>(m) * DCK * n + j;
+  double y0[D], y1[D];
+  GSX_UNROLL for (int d = 0; d < D; ++d) y0[d] = base[d * n];
+  double t0 = 0.0;
+  if constexpr (!F::autonomous) t0 = base[D * n];
+  const double h0 = a.tape[static_cast<int64_t>(s0) * n + j];
+  double h1 = 0.0;
+  if (s0 + 1 < na) h1 = a.tape[static_cast<int64_t>(s0 + 1) * n + j];
+  Rb23Step<F> w0;
+  rb23_step_state<F>(t0, y0, h0, p, a.aux, w0, y1);
+  {
+    const double t1 = t0 + h0;
+    Rb23Step<F> w1;
+    rb23_step_state<F>(t1, y1, h1, p, a.aux, w1, nullptr);
+    rb23_step_vjp_state<F>(t1, y1, h1, p, w1, lam, pb, a.aux);
+  }
+  if constexpr (kSaves) {
+    if (s0 + 1 < na) inject_saves<D>(a, j, s0 + 1, qs, lam);
+  }
+  rb23_step_vjp_state<F>(t0, y0, h0, p, w0, lam, pb, a.aux);
+  if constexpr (kSaves) inject_saves<D>(a, j, s0, qs, lam);
+}
+
+// The pair adjoint of one lane; `mmax` as in tsit5_adj_pair_lane.
+template <class F, bool kSaves = true>
+GSX_HD void rb23_adj_pair_lane(const AdjArgs& a, int64_t j, int32_t na, int32_t mmax) {
+  constexpr int D = F::D, P = F::P;
+  const int64_t n = a.n;
+  double p[P], pb[P], lam[D];
+  GSX_UNROLL for (int q = 0; q < P; ++q) {
+    p[q] = a.p[q * n + j];
+    pb[q] = (kSaves && a.pbar_in) ? a.pbar_in[q * n + j] : 0.0;
+  }
+  GSX_UNROLL for (int d = 0; d < D; ++d) lam[d] = a.ybar[d * n + j];
+  int32_t qs = kSaves ? static_cast<int32_t>(a.nsave) - 1 : -1;
+  const int32_t nseg = (na + 1) / 2;
+  for (int32_t m = mmax - 1; m >= 0; --m) {
+    if (m < nseg) rb23_adj_pair_segment<F, kSaves>(a, j, m, na, p, lam, pb, qs);
+  }
+  if constexpr (kSaves) inject_saves<D>(a, j, -1, qs, lam);
   GSX_UNROLL for (int d = 0; d < D; ++d) a.y0bar[d * n + j] = lam[d];
   GSX_UNROLL for (int q = 0; q < P; ++q) a.pbar[q * n + j] = pb[q];
 }

@@ -42,6 +42,14 @@ int launch_adj(const AdjArgs& a, hipStream_t s) {
   if (a.n == 0) return GSX_OK;
   const int C = a.ckpt_interval;
   if (C <= 1) return launch_adj_c<F, 1>(a, s);
+  if (C == 2) {
+    const unsigned grid = static_cast<unsigned>((a.n + kFwdBlock - 1) / kFwdBlock);
+    if (a.nsave > 0 || a.pbar_in)
+      rb23_adj_pair_kernel<F, true><<<grid, kFwdBlock, 0, s>>>(a);
+    else
+      rb23_adj_pair_kernel<F, false><<<grid, kFwdBlock, 0, s>>>(a);
+    return check_launch("rb23_adj_pair_kernel");
+  }
   if (C <= 4) return launch_adj_c<F, 4>(a, s);
   if (C <= 8) return launch_adj_c<F, 8>(a, s);
   if (C <= 16) return launch_adj_c<F, 16>(a, s);

@@ -31,6 +31,18 @@ __global__ __launch_bounds__(kAdjBlock) void rb23_adj_kernel(AdjArgs a) {
   rb23_adj_lane<F>(a, j, seg + threadIdx.x, kAdjBlock, mtop);
 }
 
+// Checkpoint interval 2: the pair adjoint (rb23_adj_pair_segment; no LDS), walked in step
+// across the wavefront from its largest segment count down, as tsit5_adj_pair_kernel.
+template <class F, bool kSaves>
+__global__ __launch_bounds__(kFwdBlock) void rb23_adj_pair_kernel(AdjArgs a) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * kFwdBlock + threadIdx.x;
+  const bool valid = j < a.n;
+  const int32_t na = valid ? a.n_acc[j] : 0;
+  const int32_t mmax = wave_max((na + 1) / 2);
+  if (!valid) return;
+  rb23_adj_pair_lane<F, kSaves>(a, j, na, mmax);
+}
+
 // Dense output on a recorded mesh and the VJPs of its re-steps, one trajectory per lane.
 template <class F>
 __global__ __launch_bounds__(kFwdBlock) void rb23_saveat_kernel(SaveArgs a) {

@@ -12,7 +12,8 @@
 //                               (pair kernel for C = 2, segment kernels for C <= 1/4/8/16/32)
 //   kTsit5Strict   gsx_strict   the same with contraction off and the library's pow/division
 //   kTsit5F32      gsx          the FP32 forward, replay and adjoint (autonomous fields)
-//   kRb23          gsx          Rosenbrock23 forward, replay, adjoint (stiff fields)
+//   kRb23          gsx          Rosenbrock23 forward, replay, adjoint (stiff fields; pair kernel
+//                               for C = 2)
 //   kRb23Strict    gsx_strict   the same, strict arithmetic
 //   kRb23F32       gsx          the FP32 Rosenbrock23 forward, replay and adjoint (autonomous)
 //   kTsit5Wide     gsx          the wide Tsit5 forward, replay and adjoint (gsx_wide.h): the
@@ -76,13 +77,15 @@ const FamilySpec& spec(Family f) {
         {"adj32", "tsit5_adj_f32_kernel<%s, 32>"}}},
       {"gsx", "gsx_rb23_kernels.h", {},
        {{"fwd", "rb23_fwd_kernel<%s>"}, {"replay", "rb23_replay_kernel<%s>"},
-        {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
+        {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj_pair", "rb23_adj_pair_kernel<%s, true>"},
+        {"adj_pair_plain", "rb23_adj_pair_kernel<%s, false>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
         {"adj8", "rb23_adj_kernel<%s, 8>"}, {"adj16", "rb23_adj_kernel<%s, 16>"},
         {"adj32", "rb23_adj_kernel<%s, 32>"}, {"saveat", "rb23_saveat_kernel<%s>"},
         {"saveat_vjp", "rb23_saveat_vjp_kernel<%s>"}}},
       {"gsx_strict", "gsx_rb23_kernels.h", kStrict,
        {{"fwd", "rb23_fwd_kernel<%s>"}, {"replay", "rb23_replay_kernel<%s>"},
-        {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
+        {"adj1", "rb23_adj_kernel<%s, 1>"}, {"adj_pair", "rb23_adj_pair_kernel<%s, true>"},
+        {"adj_pair_plain", "rb23_adj_pair_kernel<%s, false>"}, {"adj4", "rb23_adj_kernel<%s, 4>"},
         {"adj8", "rb23_adj_kernel<%s, 8>"}, {"adj16", "rb23_adj_kernel<%s, 16>"},
         {"adj32", "rb23_adj_kernel<%s, 32>"}, {"saveat", "rb23_saveat_kernel<%s>"},
         {"saveat_vjp", "rb23_saveat_vjp_kernel<%s>"}}},
@@ -374,11 +377,13 @@ int rb23_replay(int field, Family fam, const ReplayArgs& a, hipStream_t s) {
 }
 
 int rb23_adj(int field, Family fam, const AdjArgs& a, hipStream_t s) {
-  const char* key = adj_key(a.ckpt_interval, false);
+  const char* key = adj_key(a.ckpt_interval, fam != kRb23F32);
   if (!key)
     return set_error(GSX_E_INVALID, "ckpt_interval %d exceeds the adjoint's maximum of %d",
                      a.ckpt_interval, kMaxCkptInterval);
-  return launch(field, fam, key, a.n, kAdjBlock, a, s);
+  const bool is_pair = std::strcmp(key, "adj_pair") == 0;
+  if (is_pair && a.nsave == 0 && !a.pbar_in) key = "adj_pair_plain";
+  return launch(field, fam, key, a.n, is_pair ? kFwdBlock : kAdjBlock, a, s);
 }
 
 int r5_launch(int field, Family fam, const char* key, int64_t n, const FwdArgs& a,

@@ -65,8 +65,21 @@ int run_rb_replay(const ReplayArgs& a) {
   for (int64_t j = 0; j < a.n; ++j) rb23_replay_lane<F>(a, j);
   return 0;
 }
+// As gsx_rb23_adjoint dispatches: the pair adjoint at C = 2 (with the save injection when save
+// cotangents or an incoming pbar are given), the segment adjoint otherwise.
 template <class F>
 int run_rb_adj(const AdjArgs& a) {
+  if (a.ckpt_interval == 2) {
+    const bool saves = a.nsave > 0 || a.pbar_in;
+    for (int64_t j = 0; j < a.n; ++j) {
+      const int32_t na = a.n_acc[j];
+      if (saves)
+        rb23_adj_pair_lane<F, true>(a, j, na, (na + 1) / 2);
+      else
+        rb23_adj_pair_lane<F, false>(a, j, na, (na + 1) / 2);
+    }
+    return 0;
+  }
   std::vector<double> seg(static_cast<size_t>(a.ckpt_interval) * ckpt_rows<F>());
   for (int64_t j = 0; j < a.n; ++j) rb23_adj_lane<F>(a, j, seg.data(), 1);
   return 0;

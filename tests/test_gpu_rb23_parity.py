@@ -65,8 +65,9 @@ def test_rb23_forward_matches_oracle(gpu, strict):
     assert np.max(np.abs(tape.sum(axis=1) - 100.0)) < 1e-9
 
 
-@pytest.mark.parametrize("C", [1, 8, 32])
+@pytest.mark.parametrize("C", [1, 2, 8, 32])
 def test_rb23_adjoint_matches_oracle_on_same_tape(gpu, C):
+    """C = 2 runs the pair adjoint (rb23_adj_pair_kernel), the others the segment adjoint."""
     e = W.robertson(512, seed=1)
     print(f"case robertson adjoint C={C}", flush=True)
     rec = _record(e, gpu, ckpt_interval=C, tape_cap=512)

@@ -71,7 +71,7 @@ def test_rb23_saveat_matches_oracle(gpu, strict):
     assert np.all(step[0] == -1) and np.all(step[-1] == rec.n_acc.cpu().numpy())
 
 
-@pytest.mark.parametrize("C", [1, 8])
+@pytest.mark.parametrize("C", [1, 2, 8])
 def test_rb23_saveat_gradient_matches_oracle(gpu, C):
     e = W.robertson(1024, seed=62)
     print(f"case rb23 saveat adjoint robertson n=1024 C={C}", flush=True)

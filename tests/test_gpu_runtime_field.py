@@ -131,11 +131,13 @@ def test_generated_lorenz_matches_builtin_kernels(gpu, variant):
         assert np.max(np.abs(g - r) / np.abs(r).max(axis=0, keepdims=True)) < 1e-12
 
 
-def test_generated_robertson_matches_builtin_rosenbrock23(gpu):
+@pytest.mark.parametrize("C", [2, 8])
+def test_generated_robertson_matches_builtin_rosenbrock23(gpu, C):
+    """C = 2 compiles and runs the pair adjoint for the generated field."""
     b = register_field("rt_robertson", robertson, 3, 3, stiff=True)
     e = W.robertson(2048, seed=22)
-    print("case runtime robertson n=2048", flush=True)
-    kw = dict(entry="gsx_rb23_fwd", ckpt_interval=8)
+    print(f"case runtime robertson n=2048 C={C}", flush=True)
+    kw = dict(entry="gsx_rb23_fwd", ckpt_interval=C)
     rec = {}
     for key, fid in (("gen", b.field_id), ("ref", N.FIELD_ROBERTSON)):
         rec[key] = H.record(fid, _soa(e.y0, gpu), _soa(e.params, gpu), t0=e.t0, t1=e.t1,

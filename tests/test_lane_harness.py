@@ -309,12 +309,13 @@ def _stiff_case(case, n, seed, t1_forced=5.0):
 
 
 @pytest.mark.parametrize("case", ["robertson", "forced"])
-@pytest.mark.parametrize("C", [1, 4])
+@pytest.mark.parametrize("C", [1, 2, 4])
 def test_rb23_lanes_and_saveat_match_oracle(lh, C, case):
     """The Rosenbrock23 lane bodies (strict build) against the oracle on Robertson: the record
-    (counts, tape and final state bit for bit), the replay, the adjoint (1e-12), the dense
-    output of gsx_rb23_saveat bit for bit, and the gradient of gsx_rb23_adjoint_saveat (save
-    cotangents plus a final-state cotangent) to 1e-12, with every buffer bounds-guarded."""
+    (counts, tape and final state bit for bit), the replay, the adjoint (1e-12; C = 2 runs the
+    pair adjoint, with and without the save injection), the dense output of gsx_rb23_saveat
+    bit for bit, and the gradient of gsx_rb23_adjoint_saveat (save cotangents plus a
+    final-state cotangent) to 1e-12, with every buffer bounds-guarded."""
     vp, i64, i32, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, \
         ctypes.c_int
     lh.lh_rb23_fwd.argtypes = [i, i64, vp, vp, d, d, vp, d, d, i64, vp, vp, vp, vp, vp, i64, vp,
