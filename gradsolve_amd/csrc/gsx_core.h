@@ -233,12 +233,12 @@ GSX_HD double norm_div(double a, double b) {
 // swaps and negates them. Within 2 ulp of the correctly rounded values. Larger |x| and
 // non-finite x go to the math library. Every rounding step is an explicit operation, so host
 // and device builds return the same bits.
-GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
-  if (!(std::fabs(x) < 1048576.0)) {
-    *s_out = std::sin(x);
-    *c_out = std::cos(x);
-    return;
-  }
+// The reduction and polynomials of gsx_sincos without its range branch, for callers that keep
+// a block of them branch-free (the Kuramoto field interleaves them with its MFMA products) and
+// redo the evaluation with gsx_sincos when some |x| >= 2^20. For such x (and non-finite x) the
+// results are meaningless but the evaluation is well defined: k is clamped before its
+// conversion to an integer.
+GSX_HD void gsx_sincos_core(double x, double* s_out, double* c_out) {
   constexpr double kTwoOverPi = 0.6366197723675814;
   constexpr double kPio2Hi = 1.5707963267948966;
   constexpr double kPio2Mid = 6.123233995736766e-17;
@@ -266,11 +266,21 @@ GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
   pc = std::fma(z, pc, 1.0 / 24.0);
   pc = std::fma(z, pc, -0.5);
   const double cr = std::fma(z, pc, 1.0);
-  const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
+  const double kc = std::fmin(std::fmax(k, -2097152.0), 2097152.0);  // NaN -> a bound
+  const int q = static_cast<int>(static_cast<int64_t>(kc) & 3);
   const double sa = (q & 1) ? cr : sr;
   const double ca = (q & 1) ? sr : cr;
   *s_out = (q & 2) ? -sa : sa;
   *c_out = ((q + 1) & 2) ? -ca : ca;
+}
+
+GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
+  if (!(std::fabs(x) < 1048576.0)) {
+    *s_out = std::sin(x);
+    *c_out = std::cos(x);
+    return;
+  }
+  gsx_sincos_core(x, s_out, c_out);
 }
 
 // Single-precision counterpart (the FP32 Kuramoto variant): for |x| < 2^14 the same three-part
@@ -278,13 +288,9 @@ GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
 // lo = -0x1.ee59dap-50; the first subtraction is exact because k*hi and x are multiples of
 // 2^-23 there), then the Taylor series of sin through r^11 and of cos through r^12 on
 // |r| <= pi/4 (the first omitted terms are below 2^-36 relative). Larger |x| and non-finite x
-// go to the math library.
-GSX_HD void gsx_sincosf(float x, float* s_out, float* c_out) {
-  if (!(std::fabs(x) < 16384.0f)) {
-    *s_out = std::sin(x);
-    *c_out = std::cos(x);
-    return;
-  }
+// go to the math library (gsx_sincosf); gsx_sincosf_core is the evaluation without that
+// branch, as gsx_sincos_core.
+GSX_HD void gsx_sincosf_core(float x, float* s_out, float* c_out) {
   constexpr float kTwoOverPi = 0.636619746685028f;
   constexpr float kPio2Hi = 0x1.921fb6p+0f;
   constexpr float kPio2Mid = -0x1.777a5cp-25f;
@@ -307,11 +313,21 @@ GSX_HD void gsx_sincosf(float x, float* s_out, float* c_out) {
   pc = std::fma(z, pc, 1.0f / 24.0f);
   pc = std::fma(z, pc, -0.5f);
   const float cr = std::fma(z, pc, 1.0f);
-  const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
+  const float kc = std::fmin(std::fmax(k, -32768.0f), 32768.0f);  // NaN -> a bound
+  const int q = static_cast<int>(static_cast<int64_t>(kc) & 3);
   const float sa = (q & 1) ? cr : sr;
   const float ca = (q & 1) ? sr : cr;
   *s_out = (q & 2) ? -sa : sa;
   *c_out = ((q + 1) & 2) ? -ca : ca;
+}
+
+GSX_HD void gsx_sincosf(float x, float* s_out, float* c_out) {
+  if (!(std::fabs(x) < 16384.0f)) {
+    *s_out = std::sin(x);
+    *c_out = std::cos(x);
+    return;
+  }
+  gsx_sincosf_core(x, s_out, c_out);
 }
 
 // Python's builtin min/max argument semantics (min(a, b) is a unless b < a), which the

@@ -74,6 +74,9 @@ struct Prec<double> {
   }
   __device__ static int krow(int T, int rho) { return 8 * (rho >> 2) + 2 * (rho & 3) + T; }
   __device__ static void sincos(double x, double* s, double* c) { gsx_sincos(x, s, c); }
+  // the branch-free evaluation and the range where it equals sincos
+  __device__ static void sincos_core(double x, double* s, double* c) { gsx_sincos_core(x, s, c); }
+  static constexpr double kCoreRange = 1048576.0;
 };
 template <>
 struct Prec<float> {
@@ -85,6 +88,8 @@ struct Prec<float> {
   }
   __device__ static int krow(int T, int rho) { return 8 * (rho & 3) + 2 * (rho >> 2) + T; }
   __device__ static void sincos(float x, float* s, float* c) { gsx_sincosf(x, s, c); }
+  __device__ static void sincos_core(float x, float* s, float* c) { gsx_sincosf_core(x, s, c); }
+  static constexpr float kCoreRange = 16384.0f;
 };
 
 // Fills the block's operand tables: op[e*64 + l] for e = T*8 + kk, forward then transposed.
@@ -100,26 +105,58 @@ __device__ void fill_operands(R* op, const double* K) {
   __syncthreads();
 }
 
-// a = (K s) and b = (K c) at the lane's 8 slots (tab: forward or transposed table).
-template <class R>
-__device__ __forceinline__ void coupling(const R* tab, const R* s, const R* c, R* a, R* b) {
+// a = (K b1) and b = (K b2) at the lane's 8 slots (tab: forward or transposed table), where
+// prod(kk, b1, b2) produces the B operands of k-step kk right before that step's MFMAs: its
+// VALU work (sin/cos of the slot) then overlaps the MFMAs of the k-steps before it, which run
+// asynchronously in the matrix pipe.
+template <class R, class Prod>
+__device__ __forceinline__ void coupling_by(const R* tab, Prod prod, R* a, R* b) {
   using P = Prec<R>;
   const int l = static_cast<int>(__lane_id());
   // volatile keeps the operand reads next to their MFMAs instead of 32 hoisted registers
   const volatile typename P::lds_t* tv = (const volatile typename P::lds_t*)tab;
   typename P::acc4 as0 = {R(0), R(0), R(0), R(0)}, as1 = as0, ac0 = as0, ac1 = as0;
   GSX_UNROLL for (int kk = 0; kk < 8; ++kk) {
+    R b1, b2;
+    prod(kk, b1, b2);
     const R k0 = tv[kk * 64 + l], k1 = tv[(8 + kk) * 64 + l];
-    as0 = P::mfma(k0, s[kk], as0);
-    ac0 = P::mfma(k0, c[kk], ac0);
-    as1 = P::mfma(k1, s[kk], as1);
-    ac1 = P::mfma(k1, c[kk], ac1);
+    as0 = P::mfma(k0, b1, as0);
+    ac0 = P::mfma(k0, b2, ac0);
+    as1 = P::mfma(k1, b1, as1);
+    ac1 = P::mfma(k1, b2, ac1);
   }
   GSX_UNROLL for (int r = 0; r < 4; ++r) {
     a[r] = as0[r];
     a[4 + r] = as1[r];
     b[r] = ac0[r];
     b[4 + r] = ac1[r];
+  }
+}
+
+// a = (K s) and b = (K c) from operands already formed.
+template <class R>
+__device__ __forceinline__ void coupling(const R* tab, const R* s, const R* c, R* a, R* b) {
+  coupling_by<R>(tab, [&](int kk, R& b1, R& b2) { b1 = s[kk]; b2 = c[kk]; }, a, b);
+}
+
+// a = (K s) and b = (K c) with s, c = sin y, cos y formed slot by slot inside the product
+// loop by the branch-free evaluation; when some lane of the wavefront holds a phase outside its
+// range (a wavefront-uniform test, so the MFMAs stay whole-wavefront) s, c and the products are
+// formed again with the range-checked sin/cos. Either way the results are those of sincos.
+template <class R>
+__device__ __forceinline__ void coupling_sincos(const R* tab, const R* y, R* s, R* c, R* a,
+                                                R* b) {
+  using P = Prec<R>;
+  bool out = false;
+  coupling_by<R>(tab, [&](int kk, R& b1, R& b2) {
+    out = out || !(std::fabs(y[kk]) < P::kCoreRange);
+    P::sincos_core(y[kk], &s[kk], &c[kk]);
+    b1 = s[kk];
+    b2 = c[kk];
+  }, a, b);
+  if (__ballot(out) != 0ull) {
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) P::sincos(y[m], &s[m], &c[m]);
+    coupling<R>(tab, s, c, a, b);
   }
 }
 
@@ -131,8 +168,7 @@ struct FKuramotoWave {
   // (sum_j K_mj cos(y_j - y_m)), which the VJP at the same point needs.
   __device__ static void f(const R* y, const R* om, R* dy, const R* op, R* dg = nullptr) {
     R s[kSlots], c[kSlots], a[kSlots], b[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) Prec<R>::sincos(y[m], &s[m], &c[m]);
-    coupling<R>(op, s, c, a, b);
+    coupling_sincos<R>(op, y, s, c, a, b);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m)
       dy[m] = om[m] + R(kInvD) * (a[m] * c[m] - b[m] * s[m]);
     if (dg) {
@@ -142,24 +178,41 @@ struct FKuramotoWave {
 
   // yb += J^T v, pb += v: (J^T v)_m = (1/32) [c_m (K^T (v c))_m + s_m (K^T (v s))_m
   //                                            - v_m (a_m s_m + b_m c_m)].
-  // dg_in: the diagonal term from f at the same y (skips the forward coupling products).
+  // dg_in: the diagonal term from f at the same y (skips the forward coupling products; sin/cos
+  // of each slot are then formed inside the transposed products, as coupling_sincos does).
   __device__ static void vjp(const R* y, const R* v, R* yb, R* pb, const R* op,
                              const R* dg_in = nullptr) {
-    R s[kSlots], c[kSlots], dg[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) Prec<R>::sincos(y[m], &s[m], &c[m]);
+    using P = Prec<R>;
+    R s[kSlots], c[kSlots], dg[kSlots], vc[kSlots], vs[kSlots], u[kSlots], w[kSlots];
     if (dg_in) {
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) dg[m] = dg_in[m];
+      bool out = false;
+      coupling_by<R>(op + kOpTable, [&](int kk, R& b1, R& b2) {
+        out = out || !(std::fabs(y[kk]) < P::kCoreRange);
+        P::sincos_core(y[kk], &s[kk], &c[kk]);
+        vc[kk] = v[kk] * c[kk];
+        vs[kk] = v[kk] * s[kk];
+        b1 = vc[kk];
+        b2 = vs[kk];
+      }, u, w);
+      if (__ballot(out) != 0ull) {
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+          P::sincos(y[m], &s[m], &c[m]);
+          vc[m] = v[m] * c[m];
+          vs[m] = v[m] * s[m];
+        }
+        coupling<R>(op + kOpTable, vc, vs, u, w);
+      }
     } else {
       R a[kSlots], b[kSlots];
-      coupling<R>(op, s, c, a, b);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) dg[m] = a[m] * s[m] + b[m] * c[m];
+      coupling_sincos<R>(op, y, s, c, a, b);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        dg[m] = a[m] * s[m] + b[m] * c[m];
+        vc[m] = v[m] * c[m];
+        vs[m] = v[m] * s[m];
+      }
+      coupling<R>(op + kOpTable, vc, vs, u, w);
     }
-    R vc[kSlots], vs[kSlots], u[kSlots], w[kSlots];
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-      vc[m] = v[m] * c[m];
-      vs[m] = v[m] * s[m];
-    }
-    coupling<R>(op + kOpTable, vc, vs, u, w);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       yb[m] += R(kInvD) * (c[m] * u[m] + s[m] * w[m] - v[m] * dg[m]);
       pb[m] += v[m];
