@@ -92,15 +92,16 @@ struct Prec<float> {
   static constexpr float kCoreRange = 16384.0f;
 };
 
-// Fills the block's operand tables: op[e*64 + l] for e = T*8 + kk, forward then transposed.
-template <class R>
+// Fills the block's operand tables: op[e*64 + l] for e = T*8 + kk, forward then (kTransposed)
+// transposed; the kernels that evaluate only the field fill the forward table.
+template <class R, bool kTransposed = true>
 __device__ void fill_operands(R* op, const double* K) {
   for (int x = threadIdx.x; x < kOpTable; x += blockDim.x) {
     const int e = x >> 6, l = x & 63;
     const int T = e >> 3, kk = e & 7;
     const int row = Prec<R>::krow(T, l & 15), col = kdim(l >> 4, kk);
     op[x] = static_cast<R>(K[row * kKD + col]);
-    op[kOpTable + x] = static_cast<R>(K[col * kKD + row]);
+    if (kTransposed) op[kOpTable + x] = static_cast<R>(K[col * kKD + row]);
   }
   __syncthreads();
 }
@@ -374,54 +375,113 @@ __device__ __forceinline__ void stage_input(const StageSlots<R>& ks, const R* y,
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
 }
 
-// The step's increment h * sum b_i k_i (FP32) or y5 = y + h * sum b_i k_i (FP64), from slots
-// 0..5.
-template <class R>
-__device__ __forceinline__ void step_update(const StageSlots<R>& ks, const R* y, R h, R* y5,
-                                            R* inc) {
-  R acc[kSlots];
-  stage_sum<R>(ks, 6, 6, acc);
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-    if constexpr (Prec<R>::kComp) {
-      inc[m] = h * acc[m];
-      y5[m] = y[m] + inc[m];
-    } else {
-      y5[m] = y[m] + h * acc[m];
-    }
-  }
-}
-
-// Stages first..6 of a trial step: k_st = f(stage input st) into slot st, with the stage-6
-// input y5 (and, in FP32, the increment).
-template <class R>
-__device__ __forceinline__ void trial_stages(const StageSlots<R>& ks, const R* y, R h,
-                                             const R* om, const R* op, int first, R* y5,
-                                             R* inc) {
-  _Pragma("unroll 1") for (int st = first; st <= 6; ++st) {
+// The stages of one step in four LDS slots per lane (the forward, replay and saveat kernels;
+// with a slot per stage a 256-lane block took 128 KiB of LDS: one block, so one wavefront, per
+// SIMD). S0 holds k1 for the whole step (the FSAL stage: a rejected trial still needs it); S1..S3
+// take k2..k4. Once k4 is formed, its stage input z5 = y + h P5 goes to registers and S1..S3
+// become the running sums P6 = sum a6r k_r, Y = sum b_r k_r and (kErr) E = sum e_r k_r over
+// r <= 4; k5, k6 and k7 are added to them as they are formed. Every sum keeps the
+// left-to-right order of stage_sum (the reference's order), so the results are those of a
+// slot per stage, bit for bit. first = 0 forms k1 = f(y) into S0; kErr also forms k7 = f(y5)
+// into S1 (free once z6 is formed; the caller copies it to S0 on acceptance) and leaves E in
+// S3. y5 and, in FP32, the increment inc = h Y are formed as the stage sums form them. The
+// stage loop is not unrolled (one copy of the field); its branches are on the
+// wavefront-uniform stage index.
+template <class R, bool kErr>
+__device__ __forceinline__ void compact_stages(const StageSlots<R>& ks, const R* y, R h,
+                                               const R* om, const R* op, int first, R* y5,
+                                               R* inc) {
+  constexpr int kLast = kErr ? 6 : 5;
+  _Pragma("unroll 1") for (int st = first; st <= kLast; ++st) {
     R z[kSlots], kk[kSlots];
-    if (st == 6) {
-      step_update<R>(ks, y, h, y5, inc);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y5[m];
+    if (st == 0) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
+    } else if (st <= 3) {
+      R acc[kSlots];
+      stage_sum<R>(ks, st, st, acc);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
+    } else if (st == 4) {
+      // k1..k4 -> z5 (registers), P6 -> S1, Y -> S2, E -> S3
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        const R k1 = ks.at(0, m), k2 = ks.at(1, m), k3 = ks.at(2, m), k4 = ks.at(3, m);
+        R p5 = stage_w<R>(4, 0) * k1;
+        p5 = p5 + stage_w<R>(4, 1) * k2;
+        p5 = p5 + stage_w<R>(4, 2) * k3;
+        p5 = p5 + stage_w<R>(4, 3) * k4;
+        R p6 = stage_w<R>(5, 0) * k1;
+        p6 = p6 + stage_w<R>(5, 1) * k2;
+        p6 = p6 + stage_w<R>(5, 2) * k3;
+        p6 = p6 + stage_w<R>(5, 3) * k4;
+        R yb = stage_w<R>(6, 0) * k1;
+        yb = yb + stage_w<R>(6, 1) * k2;
+        yb = yb + stage_w<R>(6, 2) * k3;
+        yb = yb + stage_w<R>(6, 3) * k4;
+        z[m] = y[m] + h * p5;
+        ks.at(1, m) = p6;
+        ks.at(2, m) = yb;
+        if (kErr) {
+          R eb = stage_w<R>(7, 0) * k1;
+          eb = eb + stage_w<R>(7, 1) * k2;
+          eb = eb + stage_w<R>(7, 2) * k3;
+          eb = eb + stage_w<R>(7, 3) * k4;
+          ks.at(3, m) = eb;
+        }
+      }
+    } else if (st == 5) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * ks.at(1, m);
     } else {
-      stage_input<R>(ks, y, h, st, z);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        const R acc = ks.at(2, m);
+        if constexpr (Prec<R>::kComp) {
+          inc[m] = h * acc;
+          y5[m] = y[m] + inc[m];
+        } else {
+          y5[m] = y[m] + h * acc;
+        }
+        z[m] = y5[m];
+      }
     }
     FKuramotoWave<R>::f(z, om, kk, op);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+    if (st <= 3) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+    } else if (st == 4) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        ks.at(1, m) = ks.at(1, m) + stage_w<R>(5, 4) * kk[m];
+        ks.at(2, m) = ks.at(2, m) + stage_w<R>(6, 4) * kk[m];
+        if (kErr) ks.at(3, m) = ks.at(3, m) + stage_w<R>(7, 4) * kk[m];
+      }
+    } else if (st == 5) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        ks.at(2, m) = ks.at(2, m) + stage_w<R>(6, 5) * kk[m];
+        if (kErr) ks.at(3, m) = ks.at(3, m) + stage_w<R>(7, 5) * kk[m];
+      }
+    } else {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        ks.at(3, m) = ks.at(3, m) + stage_w<R>(7, 6) * kk[m];
+        ks.at(1, m) = kk[m];
+      }
+    }
+  }
+  if constexpr (!kErr) {
+    // the step's result from Y (S2)
+    GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+      const R acc = ks.at(2, m);
+      if constexpr (Prec<R>::kComp) {
+        inc[m] = h * acc;
+        y5[m] = y[m] + inc[m];
+      } else {
+        y5[m] = y[m] + h * acc;
+      }
+    }
   }
 }
 
 // One replay step of the state (k7 is not formed: its weight is 0).
 template <class R>
 __device__ __forceinline__ void kura_advance(Vec<R>& y, R h, const R* om, const R* op,
-                                             const StageSlots<R>& ks) {
-  _Pragma("unroll 1") for (int st = 0; st <= 5; ++st) {
-    R z[kSlots], kk[kSlots];
-    stage_input<R>(ks, y.hi, h, st, z);
-    FKuramotoWave<R>::f(z, om, kk, op);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
-  }
+                                              const StageSlots<R>& ks) {
   R y5[kSlots], inc[kSlots];
-  step_update<R>(ks, y.hi, h, y5, inc);
+  compact_stages<R, false>(ks, y.hi, h, om, op, 0, y5, inc);
   if constexpr (Prec<R>::kComp) {
     y.add(inc);
   } else {
@@ -431,10 +491,11 @@ __device__ __forceinline__ void kura_advance(Vec<R>& y, R h, const R* om, const 
 
 // Forward + record (gradsolve/solvers/tsit5_replay.py:100-124 per trajectory).
 template <class R>
-__global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
-  __shared__ R op[2 * kOpTable];
-  __shared__ R ks_all[7 * kSlots * kKBlock];
-  fill_operands<R>(op, a.K);
+__global__ __launch_bounds__(kKBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void kura_fwd_kernel(KuraFwdArgs a) {
+  __shared__ R op[kOpTable];
+  __shared__ R ks_all[4 * kSlots * kKBlock];
+  fill_operands<R, false>(op, a.K);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
@@ -463,14 +524,12 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
   while (__ballot(active) != 0ull) {
     const double h = active ? py_min(dt, t1 - t) : 0.0;
     const R hr = static_cast<R>(h);
-    trial_stages<R>(ks, y.hi, hr, om, op, first, y5, inc);
+    compact_stages<R, true>(ks, y.hi, hr, om, op, first, y5, inc);
     first = 1;
     double e[kSlots], yd[kSlots], y5d[kSlots];
     {
-      R er[kSlots];
-      stage_sum<R>(ks, 7, 7, er);
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
-        e[m] = static_cast<double>(hr * er[m]);
+        e[m] = static_cast<double>(hr * ks.at(3, m));
         y5d[m] = static_cast<double>(y5[m]);
       }
       y.get(yd);
@@ -507,7 +566,7 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
       } else {
         GSX_UNROLL for (int m = 0; m < kSlots; ++m) y.hi[m] = y5[m];
       }
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(0, m) = ks.at(6, m);  // k1 <- k7
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(0, m) = ks.at(1, m);  // k1 <- k7
       ++acc;
     } else {
       ++rej;
@@ -540,10 +599,11 @@ __global__ __launch_bounds__(kKBlock) void kura_fwd_kernel(KuraFwdArgs a) {
 
 // Frozen-mesh replay from (y0, omega): the states at the start of each step and y_final.
 template <class R>
-__global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) {
-  __shared__ R op[2 * kOpTable];
-  __shared__ R ks_all[6 * kSlots * kKBlock];
-  fill_operands<R>(op, a.K);
+__global__ __launch_bounds__(kKBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void kura_replay_kernel(KuraReplayArgs a) {
+  __shared__ R op[kOpTable];
+  __shared__ R ks_all[4 * kSlots * kKBlock];
+  fill_operands<R, false>(op, a.K);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
@@ -579,10 +639,11 @@ __global__ __launch_bounds__(kKBlock) void kura_replay_kernel(KuraReplayArgs a) 
 
 // ys[j][q] = one step of size ts[q] - t_s from the bracketing step's start state.
 template <class R>
-__global__ __launch_bounds__(kKBlock) void kura_saveat_kernel(KuraSaveatArgs a) {
-  __shared__ R op[2 * kOpTable];
-  __shared__ R ks_all[6 * kSlots * kKBlock];
-  fill_operands<R>(op, a.K);
+__global__ __launch_bounds__(kKBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void kura_saveat_kernel(KuraSaveatArgs a) {
+  __shared__ R op[kOpTable];
+  __shared__ R ks_all[4 * kSlots * kKBlock];
+  fill_operands<R, false>(op, a.K);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
