@@ -1,6 +1,6 @@
 """Throughput of the engines on each workload (one JSON line per workload).
 
-    python -m gradsolve_amd.benchmarks --workload uneven|robertson|lorenz1m|all
+    python -m gradsolve_amd.benchmarks --workload uneven|robertson|lorenz1m|kuramoto|wide|all
 
 Every measurement: inputs resident in HBM, buffers preallocated, one warm-up, then the best
 of ``--repeat`` timed calls with device synchronisation on both sides (the reference's
@@ -139,17 +139,19 @@ def robertson(args, dev):
     gradient of sum(y_final * [1, 10, 1]) w.r.t. k1..k3. The same workload on the Rodas5P
     lane is reported beside it (``rodas5p``)."""
     out = _stiff_lane(args, dev, "rb23", "robertson_rb23")
+    out["rb23_fp32"] = _stiff_lane(args, dev, "rb23", "robertson_rb23_fp32", flags=N.FP32)
     out["rodas5p"] = _stiff_lane(args, dev, "rodas5p", "robertson_rodas5p")
     return out
 
 
-def _stiff_lane(args, dev, method, name):
+def _stiff_lane(args, dev, method, name, flags=0):
     fwd_entry, adj_entry = f"gsx_{method}_fwd", f"gsx_{method}_adjoint"
     e = W.robertson(args.n or (1 << 18), seed=0)
     n = e.y0.shape[0]
     y0, p = _soa(e.y0, dev), _soa(e.params, dev)
     rec = H.record(N.FIELD_ROBERTSON, y0, p, t0=0.0, t1=100.0, rtol=1e-6, atol=1e-10,
-                   max_steps=50000, tape_cap=1024, ckpt_interval=8, entry=fwd_entry)
+                   max_steps=50000, tape_cap=1024, ckpt_interval=8, entry=fwd_entry,
+                   fp32=bool(flags & N.FP32))
     torch.cuda.synchronize()
     S = rec.max_steps_taken()
     acc = int(rec.n_acc.to(torch.int64).sum().item())
@@ -158,8 +160,9 @@ def _stiff_lane(args, dev, method, name):
            "mean_rejected": round(rej / n, 2), "max_accepted": S}
     wvec = torch.tensor([1.0, 10.0, 1.0], dtype=torch.float64, device=dev)[:, None]
     best = None
-    # C = 2 is the Rosenbrock23 pair adjoint; the Rodas5P lane has segment adjoints only
-    for C in ((2, 4, 8, 16) if method == "rb23" else (4, 8, 16)):
+    # C = 2 is the Rosenbrock23 pair adjoint (FP64); the Rodas5P lane and the FP32
+    # Rosenbrock23 lane have segment adjoints only
+    for C in ((2, 4, 8, 16) if method == "rb23" and not flags else (4, 8, 16)):
         ws = H.Workspace(N.FIELD_ROBERTSON, 3, 3, n, S + 16, C, dev)
         ybar = (wvec * torch.ones((3, n), dtype=torch.float64, device=dev)).contiguous()
 
@@ -167,14 +170,14 @@ def _stiff_lane(args, dev, method, name):
             N.call(fwd_entry, N.FIELD_ROBERTSON, n, 3, 3, None, y0.data_ptr(),
                    p.data_ptr(), 0.0, 100.0, None, 1e-6, 1e-10, 50000, ws.y_final.data_ptr(),
                    ws.n_acc.data_ptr(), ws.n_rej.data_ptr(), ws.status.data_ptr(),
-                   ws.tape.data_ptr(), ws.cap, ws.ckpt.data_ptr(), C, 0,
+                   ws.tape.data_ptr(), ws.cap, ws.ckpt.data_ptr(), C, flags,
                    H._stream_handle(None))
 
         def both():
             fwd()
             N.call(adj_entry, N.FIELD_ROBERTSON, n, 3, 3, None, None, p.data_ptr(),
                    0.0, ws.tape.data_ptr(), ws.cap, ws.n_acc.data_ptr(), ws.ckpt.data_ptr(), C,
-                   ybar.data_ptr(), ws.y0bar.data_ptr(), ws.pbar.data_ptr(), 0,
+                   ybar.data_ptr(), ws.y0bar.data_ptr(), ws.pbar.data_ptr(), flags,
                    H._stream_handle(None))
 
         tf = _best(fwd, args.repeat)
@@ -350,14 +353,87 @@ def copy(args, dev):
             "ms": round(t * 1e3, 3), "GB_per_s": round(2 * n * 8 / t / 1e9, 1)}
 
 
+def wide(args, dev):
+    """High-NVAR non-stiff: the periodic Allen-Cahn lattice (workloads.allen_cahn, D = 256 by
+    default, N = 16,384, t in [0, 2], rtol = atol = 1e-8) compiled at run time and integrated on
+    the wide Tsit5 lane (state and stages in an HBM workspace): the record, and record +
+    adjoint of sum(y_final^2), at checkpoint intervals 4, 8 and 16."""
+    from . import hip_wide as HW
+    from .fields import register_field
+    D = args.dim or 256
+    e = W.allen_cahn(args.n or (1 << 14), D=D, seed=0)
+    n = e.y0.shape[0]
+    b = register_field(f"bench_allen_cahn_{D}", W.allen_cahn_rhs, D, 2)
+    fid = b.field_id
+    y0, p = _soa(e.y0, dev), _soa(e.params, dev)
+    kw = dict(t0=e.t0, t1=e.t1, rtol=e.rtol, atol=e.atol)
+    rec = HW.record(fid, y0, p, max_steps=100000, ckpt_interval=8, **kw)
+    torch.cuda.synchronize()
+    assert int((rec.status != 0).sum().item()) == 0
+    S = rec.max_steps_taken()
+    acc = int(rec.n_acc.to(torch.int64).sum().item())
+    rej = int(rec.n_rej.to(torch.int64).sum().item())
+    cap = S + 16
+    out = {"workload": f"allen_cahn{D}_wide", "n": n, "D": D, "mean_accepted": round(acc / n, 2),
+           "mean_rejected": round(rej / n, 2), "max_accepted": S}
+    st = H._stream_handle(None)
+    ws0 = HW.workspace(0, D, n, dev)
+    y_final = torch.empty((D, n), dtype=torch.float64, device=dev)
+    na = torch.empty(n, dtype=torch.int32, device=dev)
+    nr = torch.empty(n, dtype=torch.int32, device=dev)
+    status = torch.empty(n, dtype=torch.int32, device=dev)
+    tape = torch.empty((cap, n), dtype=torch.float64, device=dev)
+    y0bar = torch.empty((D, n), dtype=torch.float64, device=dev)
+    pbar = torch.empty((2, n), dtype=torch.float64, device=dev)
+    ybar = torch.empty((D, n), dtype=torch.float64, device=dev)
+    best = None
+    for C in (4, 8, 16):
+        ckpt = H._alloc_ckpt(fid, D, n, cap, C, dev)
+        ws1 = HW.workspace(1, D, n, dev, C)
+
+        def fwd(record=True):
+            N.call("gsx_tsit5_wide_fwd", fid, n, D, 2, None, y0.data_ptr(), p.data_ptr(),
+                   kw["t0"], kw["t1"], None, kw["rtol"], kw["atol"], 100000,
+                   y_final.data_ptr(), na.data_ptr(), nr.data_ptr(), status.data_ptr(),
+                   tape.data_ptr() if record else None, cap if record else 0,
+                   ckpt.data_ptr() if record else None, C, ws0.data_ptr(), st)
+
+        def both():
+            fwd()
+            torch.mul(y_final, 2.0, out=ybar)
+            N.call("gsx_tsit5_wide_adjoint", fid, n, D, 2, None, None, p.data_ptr(), kw["t0"],
+                   tape.data_ptr(), cap, na.data_ptr(), ckpt.data_ptr(), C, ybar.data_ptr(),
+                   y0bar.data_ptr(), pbar.data_ptr(), ws1.data_ptr(), st)
+
+        tf0 = _best(lambda: fwd(False), args.repeat)
+        tf = _best(fwd, args.repeat)
+        tb = _best(both, args.repeat)
+        assert int((status != 0).sum().item()) == 0
+        out[f"C{C}"] = {"fwd_only_ms": round(tf0 * 1e3, 3), "record_ms": round(tf * 1e3, 3),
+                        "record_adj_ms": round(tb * 1e3, 3),
+                        "adj_workspace_GB": round(ws1.numel() * 8 / 1e9, 2)}
+        _log(f"wide D={D} n={n} C={C}: {out[f'C{C}']}")
+        if best is None or tb < best[0]:
+            best = (tb, tf, tf0, C)
+        del ckpt, ws1
+    tb, tf, tf0, C = best
+    out.update({"ckpt_interval": C, "fwd_traj_per_s": round(n / tf0, 1),
+                "fwd_adj_traj_per_s": round(n / tb, 1),
+                "fwd_acc_steps_per_s": round(acc / tf0, 1),
+                "fwd_adj_acc_steps_per_s": round(acc / tb, 1),
+                "fwd_workspace_GB": round(ws0.numel() * 8 / 1e9, 3)})
+    return out
+
+
 WORKLOADS = {"uneven": uneven, "robertson": robertson, "lorenz1m": lorenz1m,
-             "kuramoto": kuramoto, "copy": copy}
+             "kuramoto": kuramoto, "wide": wide, "copy": copy}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="all", choices=["all", *WORKLOADS])
     ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--dim", type=int, default=0, help="state dimension of the wide workload")
     ap.add_argument("--repeat", type=int, default=5)
     # the persistent autotune runs on the whole ensemble: its lanes' tails depend on how many
     # trajectories each lane gets, which a subsample would change

@@ -102,14 +102,17 @@ struct WideRows {
 };
 
 // out = y + h (w_s1 k1 + ... + w_sS k_S) for weight row s (stage input z_{s+1}, or the step's
-// result for s = 6): the sum runs left to right, as tsit5_step.py:88-102 writes it.
-template <int D>
-__device__ void wide_combine(const Strided<double>& y, const Strided<double>* k, int s, double h,
+// result for s = 6): the sum runs left to right, as tsit5_step.py:88-102 writes it. krow(r) is
+// the workspace row of k_{r+1}, computed from the row indices (an array of views indexed by
+// the dynamic r would live in per-lane scratch); four components per iteration keep that many
+// independent loads in flight.
+template <int D, class KRow>
+__device__ void wide_combine(const Strided<double>& y, const KRow& krow, int s, double h,
                              const Strided<double>& out) {
-#pragma nounroll
+#pragma unroll 4
   for (int i = 0; i < D; ++i) {
-    double acc = kWideW[s][0] * k[0][i];
-    for (int r = 1; r < s; ++r) acc = acc + kWideW[s][r] * k[r][i];
+    double acc = kWideW[s][0] * krow(0)[i];
+    for (int r = 1; r < s; ++r) acc = acc + kWideW[s][r] * krow(r)[i];
     out[i] = y[i] + h * acc;
   }
 }
@@ -117,9 +120,9 @@ __device__ void wide_combine(const Strided<double>& y, const Strided<double>* k,
 // Stages s0..s1 of the step (t, y, h): stage s >= 1 forms its input z_s (row zrow(s); the
 // step's result for s = 6) from y and k_1..k_s, and evaluates k_{s+1} = f(t + c_s h, z_s) into
 // k[s] (k7 at the step's result for s = 6); stage 0 evaluates k1 = f(t, y).
-template <class F, class ZRow>
+template <class F, class KRow, class ZRow>
 __device__ void wide_stage_loop(double t, const Strided<double>& y, double h, const double* p,
-                                const Strided<double>* k, const ZRow& zrow, int s0, int s1,
+                                const KRow& krow, const ZRow& zrow, int s0, int s1,
                                 const double* aux) {
   constexpr int D = F::D;
 #pragma nounroll
@@ -127,9 +130,9 @@ __device__ void wide_stage_loop(double t, const Strided<double>& y, double h, co
     Strided<double> in = y;
     if (s > 0) {
       in = zrow(s);
-      wide_combine<D>(y, k, s, h, in);
+      wide_combine<D>(y, krow, s, h, in);
     }
-    F::f(t + kWideC[s] * h, Strided<const double>{in.b, in.s}, p, k[s], aux);
+    F::f(t + kWideC[s] * h, Strided<const double>{in.b, in.s}, p, krow(s), aux);
   }
 }
 
@@ -175,15 +178,15 @@ __device__ void tsit5w_fwd_lane(const WideFwdArgs& w, int64_t j) {
     if (t >= t1) break;
     const double h = nn_min(dt, t1 - t);
     const Strided<double> y = R.row(ry), y5 = R.row(ry5);
-    const Strided<double> k[7] = {R.row(rk1), R.row(4), R.row(5), R.row(6),
-                                  R.row(7),   R.row(8), R.row(rk7)};
+    // k1 and k7 exchange rows on acceptance (FSAL); k2..k6 are rows 4..8
+    auto krow = [&](int r) { return R.row(r == 0 ? rk1 : (r == 6 ? rk7 : 3 + r)); };
     auto zrow = [&](int s) { return s == 6 ? y5 : R.row(2); };
-    wide_stage_loop<F>(t, y, h, p, k, zrow, have_k1 ? 1 : 0, 6, a.aux);
+    wide_stage_loop<F>(t, y, h, p, krow, zrow, have_k1 ? 1 : 0, 6, a.aux);
     have_k1 = true;
     const double rtol = a.rtol, atol = a.atol;
     auto q = [&](int i) {
-      double e = kWideE[0] * k[0][i];
-      for (int r = 1; r < 7; ++r) e = e + kWideE[r] * k[r][i];
+      double e = kWideE[0] * krow(0)[i];
+      for (int r = 1; r < 7; ++r) e = e + kWideE[r] * krow(r)[i];
       e = h * e;
 #if defined(GSX_STRICT_ARITH)
       const double sc = atol + rtol * py_max(std::fabs(y[i]), std::fabs(y5[i]));
@@ -255,13 +258,13 @@ __device__ void tsit5w_replay_lane(const WideReplayArgs& w, int64_t j) {
   double* cp = ck ? a.ckpt + j : nullptr;
   if (ck) wide_ckpt_put<F>(cp, n, R.row(ry), t);
   int32_t ck_left = a.ckpt_interval;
-  const Strided<double> k[6] = {R.row(3), R.row(4), R.row(5), R.row(6), R.row(7), R.row(8)};
+  auto krow = [&](int r) { return R.row(3 + r); };
   for (int32_t s = 0; s < na; ++s) {
     const double h = a.tape[static_cast<int64_t>(s) * n + j];
     const Strided<double> y = R.row(ry);
     auto zrow = [&](int) { return R.row(2); };
-    wide_stage_loop<F>(t, y, h, p, k, zrow, 0, 5, a.aux);
-    wide_combine<D>(y, k, 6, h, R.row(rn));
+    wide_stage_loop<F>(t, y, h, p, krow, zrow, 0, 5, a.aux);
+    wide_combine<D>(y, krow, 6, h, R.row(rn));
     const int sw = ry;
     ry = rn;
     rn = sw;
@@ -293,10 +296,8 @@ __device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j, int32_t mtop) {
   // rows: step i of the segment at 6i (its start state) .. 6i+5 (z6) | k1..k6 | kbar1..6 |
   // lam | zbar
   const int rk = 6 * C, rkb = rk + 6, rlam = rkb + 6, rzb = rlam + 1;
-  const Strided<double> k[6] = {R.row(rk),     R.row(rk + 1), R.row(rk + 2),
-                                R.row(rk + 3), R.row(rk + 4), R.row(rk + 5)};
-  const Strided<double> kb[6] = {R.row(rkb),     R.row(rkb + 1), R.row(rkb + 2),
-                                 R.row(rkb + 3), R.row(rkb + 4), R.row(rkb + 5)};
+  auto krow = [&](int r) { return R.row(rk + r); };
+  auto kbrow = [&](int r) { return R.row(rkb + r); };
   const Strided<double> lam = R.row(rlam), zb = R.row(rzb);
   double p[P], pb[P];
   GSX_UNROLL for (int q = 0; q < P; ++q) {
@@ -328,11 +329,11 @@ __device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j, int32_t mtop) {
       const double h = a.tape[static_cast<int64_t>(s) * n + j];
       const Strided<double> y = R.row(b);
       auto zrow = [&](int st) { return R.row(b + st); };
-      wide_stage_loop<F>(t, y, h, p, k, zrow, 0, s + 1 < s1 ? 5 : 4, a.aux);
+      wide_stage_loop<F>(t, y, h, p, krow, zrow, 0, s + 1 < s1 ? 5 : 4, a.aux);
       if (s + 1 < s1) {
-        wide_combine<D>(y, k, 6, h, R.row(b + 6));
+        wide_combine<D>(y, krow, 6, h, R.row(b + 6));
       } else {
-        wide_combine<D>(y, k, 5, h, R.row(b + 5));  // z6 of the segment's last step
+        wide_combine<D>(y, krow, 5, h, R.row(b + 5));  // z6 of the segment's last step
       }
       t = t + h;
     }
@@ -344,10 +345,10 @@ __device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j, int32_t mtop) {
         for (int32_t r = s0; r < s; ++r) ts = ts + a.tape[static_cast<int64_t>(r) * n + j];
       }
       const double h = a.tape[static_cast<int64_t>(s) * n + j];
-#pragma nounroll
+#pragma unroll 4
       for (int d = 0; d < D; ++d) {
         const double l = lam[d];
-        GSX_UNROLL for (int st = 0; st < 6; ++st) kb[st][d] = h * (kWideW[6][st] * l);
+        GSX_UNROLL for (int st = 0; st < 6; ++st) kbrow(st)[d] = h * (kWideW[6][st] * l);
       }
 #pragma nounroll
       for (int st = 5; st >= 0; --st) {
@@ -355,12 +356,12 @@ __device__ void tsit5w_adj_lane(const WideAdjArgs& w, int64_t j, int32_t mtop) {
         for (int d = 0; d < D; ++d) zb[d] = 0.0;
         const Strided<double> zs = R.row(b + st);
         F::vjp(ts + kWideC[st] * h, Strided<const double>{zs.b, n}, p,
-               Strided<const double>{kb[st].b, n}, zb, pb, a.aux);
-#pragma nounroll
+               Strided<const double>{kbrow(st).b, n}, zb, pb, a.aux);
+#pragma unroll 4
         for (int d = 0; d < D; ++d) {
           const double zbd = zb[d];
           lam[d] = lam[d] + zbd;
-          for (int r = 0; r < st; ++r) kb[r][d] = kb[r][d] + h * (kWideW[st][r] * zbd);
+          for (int r = 0; r < st; ++r) kbrow(r)[d] = kbrow(r)[d] + h * (kWideW[st][r] * zbd);
         }
       }
     }

@@ -74,3 +74,20 @@ def kuramoto32(n: int = 1 << 17, *, seed: int = 0, D: int = 32) -> Ensemble:
     omega = rng.standard_normal((n, D))
     theta0 = rng.uniform(0.0, 2.0 * np.pi, (n, D))
     return Ensemble(f"kuramoto{D}_n{n}", theta0, omega, 0.0, 10.0, 1e-8, 1e-8, aux=K)
+
+
+def allen_cahn_rhs(t, y, p):
+    """Periodic Allen-Cahn lattice: a * (discrete Laplacian) + y - y^3 + b, with p = [a, b]
+    (a NumPy right-hand side that the field tracer compiles; any D)."""
+    lap = np.roll(y, -1) - 2.0 * y + np.roll(y, 1)
+    return p[0] * lap + y - y * y * y + p[1]
+
+
+def allen_cahn(n: int = 1 << 14, *, D: int = 256, seed: int = 0, t1: float = 2.0,
+               tol: float = 1e-8) -> Ensemble:
+    """A wide non-stiff ensemble for the high-NVAR cells: a ~ U(0.3, 0.6), b ~ U(-0.1, 0.1),
+    y0 ~ U(-1, 1)^D, t in [0, t1], rtol = atol = tol."""
+    rng = np.random.default_rng(seed)
+    p = np.stack([rng.uniform(0.3, 0.6, n), rng.uniform(-0.1, 0.1, n)], axis=1)
+    y0 = rng.uniform(-1.0, 1.0, (n, D))
+    return Ensemble(f"allen_cahn{D}_n{n}", y0, p, 0.0, float(t1), tol, tol)
