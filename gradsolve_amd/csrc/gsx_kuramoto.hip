@@ -42,6 +42,8 @@
 // hi + lo is exact in FP64).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gsx_internal.h"
 
 namespace gsx {
@@ -106,21 +108,54 @@ __device__ void fill_operands(R* op, const double* K) {
   __syncthreads();
 }
 
-// a = (K b1) and b = (K b2) at the lane's 8 slots (tab: forward or transposed table), where
-// prod(kk, b1, b2) produces the B operands of k-step kk right before that step's MFMAs: its
-// VALU work (sin/cos of the slot) then overlaps the MFMAs of the k-steps before it, which run
-// asynchronously in the matrix pipe.
-template <class R, class Prod>
-__device__ __forceinline__ void coupling_by(const R* tab, Prod prod, R* a, R* b) {
+// Sources of the MFMA A operands. fwd(T, kk) is K's row krow(T, lane%16), column
+// kdim(lane/16, kk) (tile T of the product K x, k-step kk); tr(T, kk) is the same entry of K^T.
+// LdsOps reads the block's tables (fill_operands; volatile keeps each read next to its MFMA
+// instead of 32 hoisted registers). CachedOps reads K itself through the vector cache (8 KiB,
+// shared by every wavefront) at per-lane offsets fixed once, for the adjoint, whose LDS holds
+// only stage slots.
+template <class R>
+struct LdsOps {
+  const volatile typename Prec<R>::lds_t* tv;
+  int l;
+  __device__ explicit LdsOps(const R* op)
+      : tv((const volatile typename Prec<R>::lds_t*)op), l(static_cast<int>(__lane_id())) {}
+  __device__ R fwd(int T, int kk) const { return tv[(T * 8 + kk) * 64 + l]; }
+  __device__ R tr(int T, int kk) const { return tv[kOpTable + (T * 8 + kk) * 64 + l]; }
+};
+template <class R>
+struct CachedOps {
+  const double* K;
+  int f0, f1, t0, t1;  // offsets of the lane's k-step-0 entries for tiles 0 and 1
+  __device__ explicit CachedOps(const double* k) : K(k) {
+    const int l = static_cast<int>(__lane_id()), g = l >> 4;
+    const int r0 = Prec<R>::krow(0, l & 15), r1 = Prec<R>::krow(1, l & 15);
+    f0 = r0 * kKD + 2 * g;
+    f1 = r1 * kKD + 2 * g;
+    t0 = 2 * g * kKD + r0;
+    t1 = 2 * g * kKD + r1;
+  }
+  // kdim(g, kk) = 8 (kk % 4) + 2 g + kk / 4
+  __device__ R fwd(int T, int kk) const {
+    return static_cast<R>(K[(T ? f1 : f0) + 8 * (kk & 3) + (kk >> 2)]);
+  }
+  __device__ R tr(int T, int kk) const {
+    return static_cast<R>(K[(T ? t1 : t0) + (8 * (kk & 3) + (kk >> 2)) * kKD]);
+  }
+};
+
+// a = (K b1) and b = (K b2) at the lane's 8 slots, with aop(T, kk) the A operand (a forward or
+// transposed source above), where prod(kk, b1, b2) produces the B operands of k-step kk right
+// before that step's MFMAs: its VALU work (sin/cos of the slot) then overlaps the MFMAs of the
+// k-steps before it, which run asynchronously in the matrix pipe.
+template <class R, class AOp, class Prod>
+__device__ __forceinline__ void coupling_by(const AOp& aop, Prod prod, R* a, R* b) {
   using P = Prec<R>;
-  const int l = static_cast<int>(__lane_id());
-  // volatile keeps the operand reads next to their MFMAs instead of 32 hoisted registers
-  const volatile typename P::lds_t* tv = (const volatile typename P::lds_t*)tab;
   typename P::acc4 as0 = {R(0), R(0), R(0), R(0)}, as1 = as0, ac0 = as0, ac1 = as0;
   GSX_UNROLL for (int kk = 0; kk < 8; ++kk) {
     R b1, b2;
     prod(kk, b1, b2);
-    const R k0 = tv[kk * 64 + l], k1 = tv[(8 + kk) * 64 + l];
+    const R k0 = aop(0, kk), k1 = aop(1, kk);
     as0 = P::mfma(k0, b1, as0);
     ac0 = P::mfma(k0, b2, ac0);
     as1 = P::mfma(k1, b1, as1);
@@ -135,21 +170,21 @@ __device__ __forceinline__ void coupling_by(const R* tab, Prod prod, R* a, R* b)
 }
 
 // a = (K s) and b = (K c) from operands already formed.
-template <class R>
-__device__ __forceinline__ void coupling(const R* tab, const R* s, const R* c, R* a, R* b) {
-  coupling_by<R>(tab, [&](int kk, R& b1, R& b2) { b1 = s[kk]; b2 = c[kk]; }, a, b);
+template <class R, class AOp>
+__device__ __forceinline__ void coupling(const AOp& aop, const R* s, const R* c, R* a, R* b) {
+  coupling_by<R>(aop, [&](int kk, R& b1, R& b2) { b1 = s[kk]; b2 = c[kk]; }, a, b);
 }
 
 // a = (K s) and b = (K c) with s, c = sin y, cos y formed slot by slot inside the product
 // loop by the branch-free evaluation; when some lane of the wavefront holds a phase outside its
 // range (a wavefront-uniform test, so the MFMAs stay whole-wavefront) s, c and the products are
 // formed again with the range-checked sin/cos. Either way the results are those of sincos.
-template <class R>
-__device__ __forceinline__ void coupling_sincos(const R* tab, const R* y, R* s, R* c, R* a,
+template <class R, class AOp>
+__device__ __forceinline__ void coupling_sincos(const AOp& aop, const R* y, R* s, R* c, R* a,
                                                 R* b) {
   using P = Prec<R>;
   bool out = false;
-  coupling_by<R>(tab, [&](int kk, R& b1, R& b2) {
+  coupling_by<R>(aop, [&](int kk, R& b1, R& b2) {
     out = out || !(std::fabs(y[kk]) < P::kCoreRange);
     P::sincos_core(y[kk], &s[kk], &c[kk]);
     b1 = s[kk];
@@ -157,19 +192,21 @@ __device__ __forceinline__ void coupling_sincos(const R* tab, const R* y, R* s, 
   }, a, b);
   if (__ballot(out) != 0ull) {
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) P::sincos(y[m], &s[m], &c[m]);
-    coupling<R>(tab, s, c, a, b);
+    coupling<R>(aop, s, c, a, b);
   }
 }
 
 // The field on the lane's 8 slots of one trajectory; every lane of the wavefront must call it
-// together (the MFMA spans the wavefront). `op` is the block's operand tables.
+// together (the MFMA spans the wavefront). `ops` is an operand source (LdsOps or CachedOps).
 template <class R>
 struct FKuramotoWave {
   // dy = f(y); with dg, also the Jacobian's diagonal term dg_m = a_m s_m + b_m c_m
   // (sum_j K_mj cos(y_j - y_m)), which the VJP at the same point needs.
-  __device__ static void f(const R* y, const R* om, R* dy, const R* op, R* dg = nullptr) {
+  template <class Ops>
+  __device__ static void f(const R* y, const R* om, R* dy, const Ops& ops, R* dg = nullptr) {
     R s[kSlots], c[kSlots], a[kSlots], b[kSlots];
-    coupling_sincos<R>(op, y, s, c, a, b);
+    auto kf = [&](int T, int kk) { return ops.fwd(T, kk); };
+    coupling_sincos<R>(kf, y, s, c, a, b);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m)
       dy[m] = om[m] + R(kInvD) * (a[m] * c[m] - b[m] * s[m]);
     if (dg) {
@@ -181,14 +218,17 @@ struct FKuramotoWave {
   //                                            - v_m (a_m s_m + b_m c_m)].
   // dg_in: the diagonal term from f at the same y (skips the forward coupling products; sin/cos
   // of each slot are then formed inside the transposed products, as coupling_sincos does).
-  __device__ static void vjp(const R* y, const R* v, R* yb, R* pb, const R* op,
+  template <class Ops>
+  __device__ static void vjp(const R* y, const R* v, R* yb, R* pb, const Ops& ops,
                              const R* dg_in = nullptr) {
     using P = Prec<R>;
+    auto kf = [&](int T, int kk) { return ops.fwd(T, kk); };
+    auto kt = [&](int T, int kk) { return ops.tr(T, kk); };
     R s[kSlots], c[kSlots], dg[kSlots], vc[kSlots], vs[kSlots], u[kSlots], w[kSlots];
     if (dg_in) {
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) dg[m] = dg_in[m];
       bool out = false;
-      coupling_by<R>(op + kOpTable, [&](int kk, R& b1, R& b2) {
+      coupling_by<R>(kt, [&](int kk, R& b1, R& b2) {
         out = out || !(std::fabs(y[kk]) < P::kCoreRange);
         P::sincos_core(y[kk], &s[kk], &c[kk]);
         vc[kk] = v[kk] * c[kk];
@@ -202,17 +242,17 @@ struct FKuramotoWave {
           vc[m] = v[m] * c[m];
           vs[m] = v[m] * s[m];
         }
-        coupling<R>(op + kOpTable, vc, vs, u, w);
+        coupling<R>(kt, vc, vs, u, w);
       }
     } else {
       R a[kSlots], b[kSlots];
-      coupling_sincos<R>(op, y, s, c, a, b);
+      coupling_sincos<R>(kf, y, s, c, a, b);
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
         dg[m] = a[m] * s[m] + b[m] * c[m];
         vc[m] = v[m] * c[m];
         vs[m] = v[m] * s[m];
       }
-      coupling<R>(op + kOpTable, vc, vs, u, w);
+      coupling<R>(kt, vc, vs, u, w);
     }
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       yb[m] += R(kInvD) * (c[m] * u[m] + s[m] * w[m] - v[m] * dg[m]);
@@ -387,9 +427,9 @@ __device__ __forceinline__ void stage_input(const StageSlots<R>& ks, const R* y,
 // S3. y5 and, in FP32, the increment inc = h Y are formed as the stage sums form them. The
 // stage loop is not unrolled (one copy of the field); its branches are on the
 // wavefront-uniform stage index.
-template <class R, bool kErr>
+template <class R, bool kErr, class Ops>
 __device__ __forceinline__ void compact_stages(const StageSlots<R>& ks, const R* y, R h,
-                                               const R* om, const R* op, int first, R* y5,
+                                               const R* om, const Ops& op, int first, R* y5,
                                                R* inc) {
   constexpr int kLast = kErr ? 6 : 5;
   _Pragma("unroll 1") for (int st = first; st <= kLast; ++st) {
@@ -477,8 +517,8 @@ __device__ __forceinline__ void compact_stages(const StageSlots<R>& ks, const R*
 }
 
 // One replay step of the state (k7 is not formed: its weight is 0).
-template <class R>
-__device__ __forceinline__ void kura_advance(Vec<R>& y, R h, const R* om, const R* op,
+template <class R, class Ops>
+__device__ __forceinline__ void kura_advance(Vec<R>& y, R h, const R* om, const Ops& op,
                                               const StageSlots<R>& ks) {
   R y5[kSlots], inc[kSlots];
   compact_stages<R, false>(ks, y.hi, h, om, op, 0, y5, inc);
@@ -496,6 +536,7 @@ void kura_fwd_kernel(KuraFwdArgs a) {
   __shared__ R op[kOpTable];
   __shared__ R ks_all[4 * kSlots * kKBlock];
   fill_operands<R, false>(op, a.K);
+  const LdsOps<R> ops(op);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
@@ -524,7 +565,7 @@ void kura_fwd_kernel(KuraFwdArgs a) {
   while (__ballot(active) != 0ull) {
     const double h = active ? py_min(dt, t1 - t) : 0.0;
     const R hr = static_cast<R>(h);
-    compact_stages<R, true>(ks, y.hi, hr, om, op, first, y5, inc);
+    compact_stages<R, true>(ks, y.hi, hr, om, ops, first, y5, inc);
     first = 1;
     double e[kSlots], yd[kSlots], y5d[kSlots];
     {
@@ -604,6 +645,7 @@ void kura_replay_kernel(KuraReplayArgs a) {
   __shared__ R op[kOpTable];
   __shared__ R ks_all[4 * kSlots * kKBlock];
   fill_operands<R, false>(op, a.K);
+  const LdsOps<R> ops(op);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n;
   const LaneId id = lane_id(n);
@@ -627,7 +669,7 @@ void kura_replay_kernel(KuraReplayArgs a) {
       store_row(a.states + (static_cast<int64_t>(s) * n + j) * kKD, v);
     }
     Vec<R> yn = y;
-    kura_advance<R>(yn, static_cast<R>(h), om, op, ks);
+    kura_advance<R>(yn, static_cast<R>(h), om, ops, ks);
     if (act) y = yn;
   }
   if (id.valid) {
@@ -644,6 +686,7 @@ void kura_saveat_kernel(KuraSaveatArgs a) {
   __shared__ R op[kOpTable];
   __shared__ R ks_all[4 * kSlots * kKBlock];
   fill_operands<R, false>(op, a.K);
+  const LdsOps<R> ops(op);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
@@ -672,7 +715,7 @@ void kura_saveat_kernel(KuraSaveatArgs a) {
       }
       Vec<R> y;
       y.set(v);
-      kura_advance<R>(y, static_cast<R>(h), om, op, ks);
+      kura_advance<R>(y, static_cast<R>(h), om, ops, ks);
       y.get(v);
       if (need) store_row(row, v);
     }
@@ -682,15 +725,15 @@ void kura_saveat_kernel(KuraSaveatArgs a) {
 }
 
 // lam <- (d y_next / d y)^T lam, pb += (d y_next / d omega)^T lam at the step (y, h): the
-// reverse of tsit5_step_vjp in gsx_core.h with the stage derivatives in LDS slots 0..5. The
+// reverse of tsit5_step_vjp in gsx_core.h with the stage derivatives in five LDS slots. The
 // forward recompute stores k_1..k_5 in slots 0..4; the reverse rebuilds each stage input
-// z_s from y and slots 0..s-1 (still k's), then overwrites slot s with h * zbar_s, so
-// kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) reads the slots above it (s = 5 down to
-// r + 1, the order of the generic step VJP). In FP32 the step's contributions to lam and pb
-// are summed separately and added to the compensated pairs once.
-template <class R>
+// z_s from y and slots 0..s-1 (still k's: z_s needs k_1..k_s), after which k_s in slot s - 1
+// is dead and takes h * zbar_s, so kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) reads slots
+// r..4 (s = 5 down to r + 1, the order of the generic step VJP). In FP32 the step's
+// contributions to lam and pb are summed separately and added to the compensated pairs once.
+template <class R, class Ops>
 __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<R>& lam,
-                                              Vec<R>& pb, const R* op,
+                                              Vec<R>& pb, const Ops& op,
                                               const StageSlots<R>& ks) {
   // The recompute keeps each stage's diagonal term dg (per-thread memory: the stage index is
   // dynamic), so the VJPs of stages 0..4 run only the transposed products: 384 instead of 544
@@ -716,10 +759,10 @@ __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<
     if (st < 5) {
       R part[kSlots];
       const R w5 = stage_w<R>(5, st);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = w5 * ks.at(5, m);
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = w5 * ks.at(4, m);
       _Pragma("unroll 1") for (int r = 4; r > st; --r) {
         const R w = stage_w<R>(r, st);
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = part[m] + w * ks.at(r, m);
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = part[m] + w * ks.at(r - 1, m);
       }
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = kb[m] + part[m];
     }
@@ -728,7 +771,7 @@ __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<
     FKuramotoWave<R>::vjp(z, kb, zb, comp ? dp : pb.hi, op, st < 5 ? dgs + st * kSlots : nullptr);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       dl[m] += zb[m];
-      if (st > 0) ks.at(st, m) = h * zb[m];
+      if (st > 0) ks.at(st - 1, m) = h * zb[m];
     }
   }
   if constexpr (comp) {
@@ -743,11 +786,25 @@ __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<
 // iterations take, for every save time bracketed by step s, the VJP of the re-step
 // (mu = (d ys[q] / d y_s)^T ysbar[q] into the mu buffer; omega_bar accumulates); the remaining
 // iterations are the reverse sweep over the recorded states, adding mu at each bracketing step.
+// In FP64 its LDS holds only the five stage slots (80 KiB per 256-lane block) and the MFMA
+// operands come from K through the vector cache, so two blocks, two wavefronts per SIMD, fit; in
+// FP32 the slots take half that and the operand tables stay in LDS (reading K through the cache
+// measured slower there).
 template <class R>
-__global__ __launch_bounds__(kKBlock) void kura_adj_kernel(KuraAdjArgs a) {
-  __shared__ R op[2 * kOpTable];
-  __shared__ R ks_all[6 * kSlots * kKBlock];
-  fill_operands<R>(op, a.K);
+__global__ __launch_bounds__(kKBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void kura_adj_kernel(KuraAdjArgs a) {
+  constexpr bool kLdsTables = Prec<R>::kComp;
+  __shared__ R ks_all[5 * kSlots * kKBlock];
+  __shared__ R op_tab[kLdsTables ? 2 * kOpTable : 1];
+  if constexpr (kLdsTables) fill_operands<R>(op_tab, a.K);
+  using Ops = typename std::conditional<kLdsTables, LdsOps<R>, CachedOps<R>>::type;
+  const Ops op = [&]() {
+    if constexpr (kLdsTables) {
+      return LdsOps<R>(op_tab);
+    } else {
+      return CachedOps<R>(a.K);
+    }
+  }();
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
