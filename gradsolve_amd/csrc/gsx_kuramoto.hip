@@ -362,14 +362,15 @@ __device__ __forceinline__ LaneId lane_id(int64_t n) {
 // Stage derivatives live in LDS slots (this thread's column of a [slots][8][kKBlock] block
 // array) and the stage loops are not unrolled, so each kernel holds one copy of the field (its
 // 8 sincos and 32 MFMAs) instead of one per stage: unrolled, the kernels exceeded 110 KB of
-// code, far past the instruction cache. Accesses are volatile: each thread reads back only its
-// own column, and the compiler would otherwise forward the stores into registers.
+// code, far past the instruction cache. Each thread reads back only its own column; the
+// accesses are ordinary LDS loads and stores (address space 3), which the compiler batches
+// (ds_read2, one wait per group) -- volatile accesses each waited for their own result.
 template <class R>
 struct StageSlots {
   using lds_t = typename Prec<R>::lds_t;
-  volatile lds_t* base;
-  __device__ __forceinline__ explicit StageSlots(R* column) : base((volatile lds_t*)column) {}
-  __device__ __forceinline__ volatile lds_t& at(int s, int m) const {
+  lds_t* base;
+  __device__ __forceinline__ explicit StageSlots(R* column) : base((lds_t*)column) {}
+  __device__ __forceinline__ lds_t& at(int s, int m) const {
     return base[(s * kSlots + m) * kKBlock];
   }
 };
