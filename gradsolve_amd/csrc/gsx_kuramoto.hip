@@ -62,8 +62,9 @@ constexpr double kInvD = 1.0 / 32.0;
 __device__ __forceinline__ int kdim(int g, int m) { return 8 * (m & 3) + 2 * g + (m >> 2); }
 
 // Precision policies: the MFMA, its row permutation (see the header), sin/cos, and whether the
-// state is a compensated pair. LDS-qualified element types make the volatile accesses compile
-// to ds_read/ds_write (through a generic pointer they become FLAT instructions).
+// state is a compensated pair. LDS-qualified element types make the stage-slot and operand
+// accesses compile to ds_read/ds_write (through a generic pointer they could become FLAT
+// instructions).
 template <class R>
 struct Prec;
 template <>
