@@ -109,41 +109,23 @@ __device__ void fill_operands(R* op, const double* K) {
   __syncthreads();
 }
 
-// Sources of the MFMA A operands. fwd(T, kk) is K's row krow(T, lane%16), column
-// kdim(lane/16, kk) (tile T of the product K x, k-step kk); tr(T, kk) is the same entry of K^T.
-// LdsOps reads the block's tables (fill_operands; as ordinary loads, which the compiler may
-// hoist into registers where they fit: measured 2-3% faster than volatile reads kept next to
-// their MFMAs). CachedOps reads K itself through the vector cache (8 KiB,
-// shared by every wavefront) at per-lane offsets fixed once, for the adjoint, whose LDS holds
-// only stage slots.
-template <class R>
+// Source of the MFMA A operands: the block's tables (fill_operands). fwd(T, kk) is K's row
+// krow(T, lane%16), column kdim(lane/16, kk) (tile T of the product K x, k-step kk); tr(T, kk)
+// is the same entry of K^T. The forward kernels read them as ordinary loads, which the compiler
+// hoists into registers where they fit (measured 2-3% faster than reads kept next to their
+// MFMAs); the adjoint, whose registers are full, reads them at their use (kAtUse, volatile
+// loads): hoisted there, they were spilled to per-thread memory and reloaded inside the
+// products.
+template <class R, bool kAtUse = false>
 struct LdsOps {
-  const typename Prec<R>::lds_t* tv;
+  using elem_t = typename std::conditional<kAtUse, volatile typename Prec<R>::lds_t,
+                                           typename Prec<R>::lds_t>::type;
+  const elem_t* tv;
   int l;
   __device__ explicit LdsOps(const R* op)
-      : tv((const typename Prec<R>::lds_t*)op), l(static_cast<int>(__lane_id())) {}
+      : tv((const elem_t*)op), l(static_cast<int>(__lane_id())) {}
   __device__ R fwd(int T, int kk) const { return tv[(T * 8 + kk) * 64 + l]; }
   __device__ R tr(int T, int kk) const { return tv[kOpTable + (T * 8 + kk) * 64 + l]; }
-};
-template <class R>
-struct CachedOps {
-  const double* K;
-  int f0, f1, t0, t1;  // offsets of the lane's k-step-0 entries for tiles 0 and 1
-  __device__ explicit CachedOps(const double* k) : K(k) {
-    const int l = static_cast<int>(__lane_id()), g = l >> 4;
-    const int r0 = Prec<R>::krow(0, l & 15), r1 = Prec<R>::krow(1, l & 15);
-    f0 = r0 * kKD + 2 * g;
-    f1 = r1 * kKD + 2 * g;
-    t0 = 2 * g * kKD + r0;
-    t1 = 2 * g * kKD + r1;
-  }
-  // kdim(g, kk) = 8 (kk % 4) + 2 g + kk / 4
-  __device__ R fwd(int T, int kk) const {
-    return static_cast<R>(K[(T ? f1 : f0) + 8 * (kk & 3) + (kk >> 2)]);
-  }
-  __device__ R tr(int T, int kk) const {
-    return static_cast<R>(K[(T ? t1 : t0) + (8 * (kk & 3) + (kk >> 2)) * kKD]);
-  }
 };
 
 // a = (K b1) and b = (K b2) at the lane's 8 slots, with aop(T, kk) the A operand (a forward or
@@ -154,10 +136,17 @@ template <class R, class AOp, class Prod>
 __device__ __forceinline__ void coupling_by(const AOp& aop, Prod prod, R* a, R* b) {
   using P = Prec<R>;
   typename P::acc4 as0 = {R(0), R(0), R(0), R(0)}, as1 = as0, ac0 = as0, ac1 = as0;
+  // the A operands of k-step kk + 1 are read before k-step kk's B operands are formed, so a
+  // source read at its use (LdsOps<R, true>) has that VALU work to arrive in
+  R k0n = aop(0, 0), k1n = aop(1, 0);
   GSX_UNROLL for (int kk = 0; kk < 8; ++kk) {
+    const R k0 = k0n, k1 = k1n;
+    if (kk + 1 < 8) {
+      k0n = aop(0, kk + 1);
+      k1n = aop(1, kk + 1);
+    }
     R b1, b2;
     prod(kk, b1, b2);
-    const R k0 = aop(0, kk), k1 = aop(1, kk);
     as0 = P::mfma(k0, b1, as0);
     ac0 = P::mfma(k0, b2, ac0);
     as1 = P::mfma(k1, b1, as1);
@@ -199,7 +188,7 @@ __device__ __forceinline__ void coupling_sincos(const AOp& aop, const R* y, R* s
 }
 
 // The field on the lane's 8 slots of one trajectory; every lane of the wavefront must call it
-// together (the MFMA spans the wavefront). `ops` is an operand source (LdsOps or CachedOps).
+// together (the MFMA spans the wavefront). `ops` is an operand source (LdsOps).
 template <class R>
 struct FKuramotoWave {
   // dy = f(y); with dg, also the Jacobian's diagonal term dg_m = a_m s_m + b_m c_m
@@ -393,29 +382,31 @@ __device__ __forceinline__ R stage_w(int row, int r) {
   return W[row][r];
 }
 
-// sum_{r < nk} W[row][r] k_r, summed left to right (nk >= 1).
-template <class R>
-__device__ __forceinline__ void stage_sum(const StageSlots<R>& ks, int row, int nk, R* acc) {
-  const R w0 = stage_w<R>(row, 0);
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = w0 * ks.at(0, m);
-  _Pragma("unroll 1") for (int r = 1; r < nk; ++r) {
-    const R w = stage_w<R>(row, r);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) acc[m] = acc[m] + w * ks.at(r, m);
+// Input of stage S (1..5): y + h * sum_{r<S} a_{S+1,r+1} k_r over slots 0..S-1, summed left to
+// right (the reference's order). S is a compile-time constant, so the slot reads issue
+// together and the weights are immediates.
+template <class R, int S>
+__device__ __forceinline__ void stage_input_ct(const StageSlots<R>& ks, const R* y, R h, R* z) {
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+    R acc = stage_w<R>(S, 0) * ks.at(0, m);
+    GSX_UNROLL for (int r = 1; r < S; ++r) acc = acc + stage_w<R>(S, r) * ks.at(r, m);
+    z[m] = y[m] + h * acc;
   }
 }
 
-// Input of stage st (0: y itself; 1..5: y + h * sum_{r<st} a k_r). In FP32, y is the
-// compensated state's leading half.
+// Input of stage st (0: y itself; 1..5: stage_input_ct), dispatched on the wavefront-uniform
+// stage index. In FP32, y is the compensated state's leading half.
 template <class R>
 __device__ __forceinline__ void stage_input(const StageSlots<R>& ks, const R* y, R h, int st,
                                             R* z) {
-  if (st == 0) {
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
-    return;
+  switch (st) {
+    case 1: stage_input_ct<R, 1>(ks, y, h, z); break;
+    case 2: stage_input_ct<R, 2>(ks, y, h, z); break;
+    case 3: stage_input_ct<R, 3>(ks, y, h, z); break;
+    case 4: stage_input_ct<R, 4>(ks, y, h, z); break;
+    case 5: stage_input_ct<R, 5>(ks, y, h, z); break;
+    default: GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
   }
-  R acc[kSlots];
-  stage_sum<R>(ks, st, st, acc);
-  GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
 }
 
 // The stages of one step in four LDS slots per lane (the forward, replay and saveat kernels;
@@ -424,7 +415,7 @@ __device__ __forceinline__ void stage_input(const StageSlots<R>& ks, const R* y,
 // take k2..k4. Once k4 is formed, its stage input z5 = y + h P5 goes to registers and S1..S3
 // become the running sums P6 = sum a6r k_r, Y = sum b_r k_r and (kErr) E = sum e_r k_r over
 // r <= 4; k5, k6 and k7 are added to them as they are formed. Every sum keeps the
-// left-to-right order of stage_sum (the reference's order), so the results are those of a
+// left-to-right order of stage_input (the reference's order), so the results are those of a
 // slot per stage, bit for bit. first = 0 forms k1 = f(y) into S0; kErr also forms k7 = f(y5)
 // into S1 (free once z6 is formed; the caller copies it to S0 on acceptance) and leaves E in
 // S3. y5 and, in FP32, the increment inc = h Y are formed as the stage sums form them. The
@@ -437,12 +428,8 @@ __device__ __forceinline__ void compact_stages(const StageSlots<R>& ks, const R*
   constexpr int kLast = kErr ? 6 : 5;
   _Pragma("unroll 1") for (int st = first; st <= kLast; ++st) {
     R z[kSlots], kk[kSlots];
-    if (st == 0) {
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m];
-    } else if (st <= 3) {
-      R acc[kSlots];
-      stage_sum<R>(ks, st, st, acc);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = y[m] + h * acc[m];
+    if (st <= 3) {
+      stage_input<R>(ks, y, h, st, z);
     } else if (st == 4) {
       // k1..k4 -> z5 (registers), P6 -> S1, Y -> S2, E -> S3
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
@@ -727,54 +714,94 @@ void kura_saveat_kernel(KuraSaveatArgs a) {
   }
 }
 
+// The later stages' share of kbar_St (St <= 4) in the step VJP below:
+// sum_{s=6..St+2} a_{s,St+1} (h zbar_s) with h zbar_s in slot s - 2, then a_{St+2,St+1}
+// (h zbar_{St+1}) from the carried registers, summed in that order. St is a compile-time
+// constant (batched slot reads, immediate weights).
+template <class R, int St>
+__device__ __forceinline__ void stage_cot_ct(const StageSlots<R>& ks, const R* carry, R* part) {
+  GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+    R acc = stage_w<R>(5, St) * (St == 4 ? carry[m] : ks.at(3, m));
+    GSX_UNROLL for (int r = 4; r > St + 1; --r) acc = acc + stage_w<R>(r, St) * ks.at(r - 2, m);
+    if (St < 4) acc = acc + stage_w<R>(St + 1, St) * carry[m];
+    part[m] = acc;
+  }
+}
+
 // lam <- (d y_next / d y)^T lam, pb += (d y_next / d omega)^T lam at the step (y, h): the
-// reverse of tsit5_step_vjp in gsx_core.h with the stage derivatives in five LDS slots. The
-// forward recompute stores k_1..k_5 in slots 0..4; the reverse rebuilds each stage input
-// z_s from y and slots 0..s-1 (still k's: z_s needs k_1..k_s), after which k_s in slot s - 1
-// is dead and takes h * zbar_s, so kbar_r = b_r h lam + sum_{s>r} a_sr (h zbar_s) reads slots
-// r..4 (s = 5 down to r + 1, the order of the generic step VJP). In FP32 the step's
-// contributions to lam and pb are summed separately and added to the compensated pairs once.
+// reverse of tsit5_step_vjp in gsx_core.h with the stage derivatives in four LDS slots. The
+// forward recompute stores k_1..k_4 in slots 0..3; k_5 only enters the stage-6 input
+// z_6 = y + h sum_{r<=5} a_6r k_r, which goes to per-thread memory beside the diagonal terms.
+// The reverse rebuilds each stage input z_s (s <= 5) from y and slots 0..s-2, after which
+// k_{s-1} in slot s - 2 is dead and takes h zbar_{s+1}, carried in registers from the VJP
+// before (not live across the coupling products). So kbar_r = b_r h lam + sum_{s>r} a_sr
+// (h zbar_s) reads h zbar_s from slot s - 2 and the newest, s = r + 1, from the carried
+// registers; the terms are summed from s = 6 down to r + 1, the order of the generic step
+// VJP. In FP32 the step's contributions to lam and pb are summed separately and added to the
+// compensated pairs once.
 template <class R, class Ops>
 __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<R>& lam,
                                               Vec<R>& pb, const Ops& op,
                                               const StageSlots<R>& ks) {
   // The recompute keeps each stage's diagonal term dg (per-thread memory: the stage index is
-  // dynamic), so the VJPs of stages 0..4 run only the transposed products: 384 instead of 544
-  // MFMAs per step, and 6 instead of 11 sin/cos evaluations of the lane's 8 phases.
-  R dgs[5 * kSlots];
+  // dynamic), so the VJPs of stages 1..5 run only the transposed products: 384 instead of 544
+  // MFMAs per step. dgs[5] holds z_6.
+  R dgs[6 * kSlots];
   _Pragma("unroll 1") for (int st = 0; st <= 4; ++st) {
     R z[kSlots], kk[kSlots];
     stage_input<R>(ks, y, h, st, z);
     FKuramotoWave<R>::f(z, om, kk, op, dgs + st * kSlots);
-    GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+    if (st < 4) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st, m) = kk[m];
+    } else {
+      // z_6 in stage_input's left-to-right order
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
+        R acc = stage_w<R>(5, 0) * ks.at(0, m);
+        acc = acc + stage_w<R>(5, 1) * ks.at(1, m);
+        acc = acc + stage_w<R>(5, 2) * ks.at(2, m);
+        acc = acc + stage_w<R>(5, 3) * ks.at(3, m);
+        acc = acc + stage_w<R>(5, 4) * kk[m];
+        dgs[5 * kSlots + m] = y[m] + h * acc;
+      }
+    }
   }
   constexpr bool comp = Prec<R>::kComp;
-  R hl[kSlots], dl[kSlots], dp[kSlots];
+  R hl[kSlots], dl[kSlots], dp[kSlots], carry[kSlots];
   GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
     hl[m] = h * lam.hi[m];
     dl[m] = comp ? R(0) : lam.hi[m];
     dp[m] = R(0);
+    carry[m] = R(0);
   }
   _Pragma("unroll 1") for (int st = 5; st >= 0; --st) {
     R kb[kSlots], z[kSlots], zb[kSlots];
+    if (st == 5) {
+      GSX_UNROLL for (int m = 0; m < kSlots; ++m) z[m] = dgs[5 * kSlots + m];
+    } else {
+      stage_input<R>(ks, y, h, st, z);
+    }
     const R bw = stage_w<R>(6, st);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = bw * hl[m];
     if (st < 5) {
       R part[kSlots];
-      const R w5 = stage_w<R>(5, st);
-      GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = w5 * ks.at(4, m);
-      _Pragma("unroll 1") for (int r = 4; r > st; --r) {
-        const R w = stage_w<R>(r, st);
-        GSX_UNROLL for (int m = 0; m < kSlots; ++m) part[m] = part[m] + w * ks.at(r - 1, m);
+      switch (st) {
+        case 0: stage_cot_ct<R, 0>(ks, carry, part); break;
+        case 1: stage_cot_ct<R, 1>(ks, carry, part); break;
+        case 2: stage_cot_ct<R, 2>(ks, carry, part); break;
+        case 3: stage_cot_ct<R, 3>(ks, carry, part); break;
+        default: stage_cot_ct<R, 4>(ks, carry, part);
       }
       GSX_UNROLL for (int m = 0; m < kSlots; ++m) kb[m] = kb[m] + part[m];
+      // z_st is formed, so k_{st-1} in slot st - 1 is dead: it takes h zbar_{st+1}
+      if (st >= 1) {
+        GSX_UNROLL for (int m = 0; m < kSlots; ++m) ks.at(st - 1, m) = carry[m];
+      }
     }
-    stage_input<R>(ks, y, h, st, z);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) zb[m] = R(0);
     FKuramotoWave<R>::vjp(z, kb, zb, comp ? dp : pb.hi, op, st < 5 ? dgs + st * kSlots : nullptr);
     GSX_UNROLL for (int m = 0; m < kSlots; ++m) {
       dl[m] += zb[m];
-      if (st > 0) ks.at(st - 1, m) = h * zb[m];
+      carry[m] = h * zb[m];
     }
   }
   if constexpr (comp) {
@@ -789,25 +816,15 @@ __device__ __forceinline__ void kura_step_vjp(const R* y, R h, const R* om, Vec<
 // iterations take, for every save time bracketed by step s, the VJP of the re-step
 // (mu = (d ys[q] / d y_s)^T ysbar[q] into the mu buffer; omega_bar accumulates); the remaining
 // iterations are the reverse sweep over the recorded states, adding mu at each bracketing step.
-// In FP64 its LDS holds only the five stage slots (80 KiB per 256-lane block) and the MFMA
-// operands come from K through the vector cache, so two blocks, two wavefronts per SIMD, fit; in
-// FP32 the slots take half that and the operand tables stay in LDS (reading K through the cache
-// measured slower there).
+// Its LDS holds the four stage slots and both operand tables: 80 KiB per 256-lane block in
+// FP64, so two blocks, two wavefronts per SIMD, fit.
 template <class R>
 __global__ __launch_bounds__(kKBlock) __attribute__((amdgpu_waves_per_eu(2)))
 void kura_adj_kernel(KuraAdjArgs a) {
-  constexpr bool kLdsTables = Prec<R>::kComp;
-  __shared__ R ks_all[5 * kSlots * kKBlock];
-  __shared__ R op_tab[kLdsTables ? 2 * kOpTable : 1];
-  if constexpr (kLdsTables) fill_operands<R>(op_tab, a.K);
-  using Ops = typename std::conditional<kLdsTables, LdsOps<R>, CachedOps<R>>::type;
-  const Ops op = [&]() {
-    if constexpr (kLdsTables) {
-      return LdsOps<R>(op_tab);
-    } else {
-      return CachedOps<R>(a.K);
-    }
-  }();
+  __shared__ R ks_all[4 * kSlots * kKBlock];
+  __shared__ R op_tab[2 * kOpTable];
+  fill_operands<R>(op_tab, a.K);
+  const LdsOps<R, true> op(op_tab);
   const StageSlots<R> ks(ks_all + threadIdx.x);
   const int64_t n = a.n, ns = a.nsave;
   const LaneId id = lane_id(n);
