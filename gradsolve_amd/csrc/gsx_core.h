@@ -225,53 +225,56 @@ GSX_HD double norm_div(double a, double b) {
 #endif
 }
 
-// sin(x) and cos(x) together (the Kuramoto field's phases). For |x| < 2^20: k = rint(x * 2/pi)
-// and r = x - k*pi/2 by a three-part split of pi/2 (hi + mid + lo, each product inside a fused
-// multiply-add; the first subtraction is exact because k*hi and x are multiples of 2^-52 in
-// that range), then the Taylor series of sin through r^17 and of cos through r^16 on
-// |r| <= pi/4 (the first omitted terms are below 2^-60 relative), and the quadrant k mod 4
-// swaps and negates them. Within 2 ulp of the correctly rounded values. Larger |x| and
-// non-finite x go to the math library. Every rounding step is an explicit operation, so host
-// and device builds return the same bits.
-// The reduction and polynomials of gsx_sincos without its range branch, for callers that keep
-// a block of them branch-free (the Kuramoto field interleaves them with its MFMA products) and
-// redo the evaluation with gsx_sincos when some |x| >= 2^20. For such x (and non-finite x) the
-// results are meaningless but the evaluation is well defined: k is clamped before its
-// conversion to an integer.
+// sin(x) and cos(x) together (the Kuramoto field's phases). For |x| < 2^20: the nearest
+// integer k to x * 2/pi (added to 1.5 * 2^52 in one fused multiply-add, which rounds the sum to
+// an integer; the low bits of that double are k mod 4) and r = x - k*pi/2 by a three-part split
+// of pi/2 (hi + mid + lo, each product inside a fused multiply-add; the first subtraction is
+// exact because k*hi and x are multiples of 2^-52 in that range), then polynomials on
+// |r| <= pi/4: sin r = r + r^3 S(r^2) and cos r = 1 - r^2/2 + r^4 C(r^2) with S and C of degree
+// 5 (Chebyshev fits to (sin r - r)/r^3 and (cos r - 1 + r^2/2)/r^4 over [0, (pi/4)^2], computed
+// in 50-digit arithmetic and rounded to double; the fits' own errors are below 2^-55 and 2^-59
+// of the results), and the quadrant k mod 4 swaps them and flips their signs. Within 2 ulp of
+// the correctly rounded values. Larger |x| and non-finite x go to the math library. Every
+// rounding step is an explicit operation, so host and device builds return the same bits.
+// gsx_sincos_core is the evaluation without the range branch, for callers that keep a block of
+// them branch-free (the Kuramoto field interleaves them with its MFMA products) and redo the
+// evaluation with gsx_sincos when some |x| >= 2^20. For such x (and non-finite x) the results
+// are meaningless but well defined.
 GSX_HD void gsx_sincos_core(double x, double* s_out, double* c_out) {
   constexpr double kTwoOverPi = 0.6366197723675814;
+  constexpr double kShift = 6755399441055744.0;  // 1.5 * 2^52
   constexpr double kPio2Hi = 1.5707963267948966;
   constexpr double kPio2Mid = 6.123233995736766e-17;
   constexpr double kPio2Lo = -1.4973849048591698e-33;
-  const double k = std::rint(x * kTwoOverPi);
+  const double kd = std::fma(x, kTwoOverPi, kShift);
+  const double k = kd - kShift;
+  const uint32_t q = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, kd)) & 3u;
   double r = std::fma(-k, kPio2Hi, x);
   r = std::fma(-k, kPio2Mid, r);
   r = std::fma(-k, kPio2Lo, r);
   const double z = r * r;
-  double ps = 1.0 / 355687428096000.0;  // 1/17!
-  ps = std::fma(z, ps, -1.0 / 1307674368000.0);
-  ps = std::fma(z, ps, 1.0 / 6227020800.0);
-  ps = std::fma(z, ps, -1.0 / 39916800.0);
-  ps = std::fma(z, ps, 1.0 / 362880.0);
-  ps = std::fma(z, ps, -1.0 / 5040.0);
-  ps = std::fma(z, ps, 1.0 / 120.0);
-  ps = std::fma(z, ps, -1.0 / 6.0);
+  double ps = 1.5918129294866608e-10;
+  ps = std::fma(z, ps, -2.5051131845003624e-08);
+  ps = std::fma(z, ps, 2.755731610255244e-06);
+  ps = std::fma(z, ps, -0.00019841269836758574);
+  ps = std::fma(z, ps, 0.008333333333330948);
+  ps = std::fma(z, ps, -0.16666666666666666);
   const double sr = std::fma(r * z, ps, r);
-  double pc = 1.0 / 20922789888000.0;  // 1/16!
-  pc = std::fma(z, pc, -1.0 / 87178291200.0);
-  pc = std::fma(z, pc, 1.0 / 479001600.0);
-  pc = std::fma(z, pc, -1.0 / 3628800.0);
-  pc = std::fma(z, pc, 1.0 / 40320.0);
-  pc = std::fma(z, pc, -1.0 / 720.0);
-  pc = std::fma(z, pc, 1.0 / 24.0);
+  double pc = -1.1382632425521717e-11;
+  pc = std::fma(z, pc, 2.08761462684032e-09);
+  pc = std::fma(z, pc, -2.7557317271729793e-07);
+  pc = std::fma(z, pc, 2.480158729876569e-05);
+  pc = std::fma(z, pc, -0.0013888888888887398);
+  pc = std::fma(z, pc, 0.041666666666666664);
   pc = std::fma(z, pc, -0.5);
   const double cr = std::fma(z, pc, 1.0);
-  const double kc = std::fmin(std::fmax(k, -2097152.0), 2097152.0);  // NaN -> a bound
-  const int q = static_cast<int>(static_cast<int64_t>(kc) & 3);
-  const double sa = (q & 1) ? cr : sr;
-  const double ca = (q & 1) ? sr : cr;
-  *s_out = (q & 2) ? -sa : sa;
-  *c_out = ((q + 1) & 2) ? -ca : ca;
+  const double sa = (q & 1u) ? cr : sr;
+  const double ca = (q & 1u) ? sr : cr;
+  // sin(r + q pi/2) = +-sa (negative for q = 2, 3), cos = +-ca (negative for q = 1, 2)
+  const uint64_t sflip = static_cast<uint64_t>(q & 2u) << 62;
+  const uint64_t cflip = static_cast<uint64_t>((q + 1u) & 2u) << 62;
+  *s_out = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, sa) ^ sflip);
+  *c_out = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, ca) ^ cflip);
 }
 
 GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
@@ -283,42 +286,43 @@ GSX_HD void gsx_sincos(double x, double* s_out, double* c_out) {
   gsx_sincos_core(x, s_out, c_out);
 }
 
-// Single-precision counterpart (the FP32 Kuramoto variant): for |x| < 2^14 the same three-part
-// reduction with an FP32 split of pi/2 (hi = 0x1.921fb6p+0, mid = -0x1.777a5cp-25,
-// lo = -0x1.ee59dap-50; the first subtraction is exact because k*hi and x are multiples of
-// 2^-23 there), then the Taylor series of sin through r^11 and of cos through r^12 on
-// |r| <= pi/4 (the first omitted terms are below 2^-36 relative). Larger |x| and non-finite x
-// go to the math library (gsx_sincosf); gsx_sincosf_core is the evaluation without that
+// Single-precision counterpart (the FP32 Kuramoto variant): for |x| < 2^14 the same reduction
+// with 1.5 * 2^23 as the rounding constant and an FP32 split of pi/2 (hi = 0x1.921fb6p+0,
+// mid = -0x1.777a5cp-25, lo = -0x1.ee59dap-50; the first subtraction is exact because k*hi and
+// x are multiples of 2^-23 there), then sin r = r + r^3 S(r^2) with S of degree 3 and
+// cos r = 1 - r^2/2 + r^4 C(r^2) with C of degree 2 (Chebyshev fits as above, rounded to
+// single precision; fit errors below 2^-36 and 2^-30 of the results). Larger |x| and non-finite
+// x go to the math library (gsx_sincosf); gsx_sincosf_core is the evaluation without that
 // branch, as gsx_sincos_core.
 GSX_HD void gsx_sincosf_core(float x, float* s_out, float* c_out) {
   constexpr float kTwoOverPi = 0.636619746685028f;
+  constexpr float kShift = 12582912.0f;  // 1.5 * 2^23
   constexpr float kPio2Hi = 0x1.921fb6p+0f;
   constexpr float kPio2Mid = -0x1.777a5cp-25f;
   constexpr float kPio2Lo = -0x1.ee59dap-50f;
-  const float k = std::rint(x * kTwoOverPi);
+  const float kd = std::fma(x, kTwoOverPi, kShift);
+  const float k = kd - kShift;
+  const uint32_t q = __builtin_bit_cast(uint32_t, kd) & 3u;
   float r = std::fma(-k, kPio2Hi, x);
   r = std::fma(-k, kPio2Mid, r);
   r = std::fma(-k, kPio2Lo, r);
   const float z = r * r;
-  float ps = -1.0f / 39916800.0f;  // -1/11!
-  ps = std::fma(z, ps, 1.0f / 362880.0f);
-  ps = std::fma(z, ps, -1.0f / 5040.0f);
-  ps = std::fma(z, ps, 1.0f / 120.0f);
-  ps = std::fma(z, ps, -1.0f / 6.0f);
+  float ps = 0x1.6dbe08p-19f;
+  ps = std::fma(z, ps, -0x1.a013a8p-13f);
+  ps = std::fma(z, ps, 0x1.11110ep-7f);
+  ps = std::fma(z, ps, -0x1.555556p-3f);
   const float sr = std::fma(r * z, ps, r);
-  float pc = 1.0f / 479001600.0f;  // 1/12!
-  pc = std::fma(z, pc, -1.0f / 3628800.0f);
-  pc = std::fma(z, pc, 1.0f / 40320.0f);
-  pc = std::fma(z, pc, -1.0f / 720.0f);
-  pc = std::fma(z, pc, 1.0f / 24.0f);
+  float pc = 0x1.9bd89cp-16f;
+  pc = std::fma(z, pc, -0x1.6c12d2p-10f);
+  pc = std::fma(z, pc, 0x1.555554p-5f);
   pc = std::fma(z, pc, -0.5f);
   const float cr = std::fma(z, pc, 1.0f);
-  const float kc = std::fmin(std::fmax(k, -32768.0f), 32768.0f);  // NaN -> a bound
-  const int q = static_cast<int>(static_cast<int64_t>(kc) & 3);
-  const float sa = (q & 1) ? cr : sr;
-  const float ca = (q & 1) ? sr : cr;
-  *s_out = (q & 2) ? -sa : sa;
-  *c_out = ((q + 1) & 2) ? -ca : ca;
+  const float sa = (q & 1u) ? cr : sr;
+  const float ca = (q & 1u) ? sr : cr;
+  const uint32_t sflip = (q & 2u) << 30;
+  const uint32_t cflip = ((q + 1u) & 2u) << 30;
+  *s_out = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, sa) ^ sflip);
+  *c_out = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, ca) ^ cflip);
 }
 
 GSX_HD void gsx_sincosf(float x, float* s_out, float* c_out) {

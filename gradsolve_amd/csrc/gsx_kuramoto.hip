@@ -889,7 +889,19 @@ void kura_adj_kernel(KuraAdjArgs a) {
     } else {
       v = lam;
     }
+    // The reverse sweep reads the recorded states backwards: one word per 64 bytes of the next
+    // step's state rows and tape entries is read now (volatile, so it stays here), which brings
+    // those lines into the cache while this step is differentiated; the next iteration's loads
+    // then do not wait on HBM. The words are only kept alive, never used.
+    uint32_t pf_row = 0, pf_h = 0;
+    if (!ph1 && s >= 1) {
+      const int64_t sn = static_cast<int64_t>(s - 1) * n + j;
+      const int g = static_cast<int>(__lane_id()) >> 4;
+      pf_row = *reinterpret_cast<const volatile uint32_t*>(a.states + sn * kKD + 8 * g);
+      pf_h = *reinterpret_cast<const volatile uint32_t*>(a.tape + sn);
+    }
     kura_step_vjp<R>(y, static_cast<R>(h), om, v, pb, op, ks);
+    asm volatile("" ::"v"(pf_row), "v"(pf_h));
     if (ph1) {
       if (act) {
         double w[kSlots];
