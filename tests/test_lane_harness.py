@@ -159,8 +159,8 @@ def test_squared_norm_factor_and_mean():
 
 
 def test_sincos_within_two_ulp():
-    """The device sincos (three-part pi/2 reduction, Taylor polynomials; host build of the
-    same source) against the C library's sin and cos: within 2 ulp over |x| < 2^20, exact
+    """The device sincos (three-part pi/2 reduction, degree-5 fitted polynomials; host build of
+    the same source) against the C library's sin and cos: within 2 ulp over |x| < 2^20, exact
     library values beyond."""
     import math
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native_cpu")], check=True)
