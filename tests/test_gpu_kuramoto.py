@@ -232,3 +232,49 @@ def test_fp32_mfma_variant_error_against_fp64(gpu):
     cl = gs.grad_closure(P.Kuramoto(e.aux), e.y0[:8], e.params[:8], wrt="params", saveat=TS,
                          rtol=1e-8, atol=1e-8, precision="float32", device="cuda")
     assert cl.route.actual == "hip_kuramoto" and cl.rec.flags != 0
+
+
+def test_phases_outside_the_fast_sincos_range(gpu):
+    """Trajectories whose phases exceed 2^20 (one wavefront of three, all 32 phases shifted by
+    2^21) take the range-checked sin/cos (a wavefront-uniform redo of the field's products) in
+    the forward, the saves and both VJP forms of the adjoint; the other wavefronts keep the
+    fast path. Everything is compared with the oracle (C library sin/cos) as above; the large
+    phases loosen the step-size control (scale atol + rtol |theta| ~ 2e-2), so counts are few."""
+    n = 48
+    print(f"case kuramoto large phases n={n}", flush=True)
+    e, _, _, om = _setup(n, 6, gpu)
+    y0n = e.y0.copy()
+    y0n[16:32] += 2.0 ** 21
+    K = torch.as_tensor(e.aux, device=gpu)
+    y0 = torch.as_tensor(np.ascontiguousarray(y0n.T), device=gpu)
+    ts = np.linspace(0.0, 10.0, 9)
+    rec = _record(e, K, y0, om, gpu, ts=ts)
+    assert int((rec.status != 0).sum().item()) == 0
+    na = rec.n_acc.cpu().numpy().astype(np.int64)
+    nr = rec.n_rej.cpu().numpy().astype(np.int64)
+    _, _, ona, onr, _ = O.tsit5_record(O.KURAMOTO, y0n, e.params, 0.0, 10.0, rtol=1e-8,
+                                       atol=1e-8, aux=e.aux)
+    same = (na == ona) & (nr == onr)
+    print(f"accepted steps: shifted {na[16:32].mean():.1f}, others {na[:16].mean():.1f}; "
+          f"counts equal to the oracle {same.mean():.4f}", flush=True)
+    assert same.mean() > 0.99
+    tape = rec.padded_tape().cpu().numpy()
+    ryf, rys = O.tsit5_replay_saveat(O.KURAMOTO, y0n, e.params, 0.0, tape, ts, aux=e.aux)
+    yf = rec.y_final.t().cpu().numpy()
+    ys = KU.saveat(rec).cpu().numpy()
+    # relative to the row's largest entry: 1e-10 of 2^21 is 2e-4 rad, so the shifted rows are
+    # compared on the phase differences from their first phase instead
+    dfin = _rowrel(yf - yf[:, :1], ryf - ryf[:, :1])
+    dsav = _rowrel(ys - ys[:, :, :1], rys - rys[:, :, :1])
+    print(f"final phase differences vs oracle replay: max {dfin.max():.2e}; saves "
+          f"{dsav.max():.2e}", flush=True)
+    assert dfin.max() < STATE_REL and dsav.max() < STATE_REL
+    ysbar = np.random.default_rng(7).standard_normal(ys.shape) / ys.size
+    g0, gom = KU.adjoint(rec, ysbar=torch.as_tensor(ysbar, device=gpu))
+    og0, ogom = O.tsit5_adjoint_saveat(O.KURAMOTO, y0n, e.params, 0.0, tape, ts, ysbar,
+                                       aux=e.aux)
+    r0 = _rowrel(g0.t().cpu().numpy(), og0)
+    rp = _rowrel(gom.t().cpu().numpy(), ogom)
+    print(f"adjoint vs oracle: y0bar max {r0.max():.2e}, omega_bar max {rp.max():.2e}",
+          flush=True)
+    assert r0.max() < GRAD_REL and rp.max() < GRAD_REL
