@@ -172,7 +172,7 @@ def main():
     roofline = {
         "bound": "fp64-valu",
         "compute": "fp64 vector ALU (v_fma_f64); MI355X FP64 vector peak 78.6 TFLOP/s",
-        "kernel": dom,
+        "kernel": kernels[dom].get("launch", dom),
         "achieved": round(achieved, 3),
         "peak": WC.PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
