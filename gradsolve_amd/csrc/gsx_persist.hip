@@ -195,9 +195,14 @@ void persist_fwd_kernel(PersistFwdArgs pa) {
   pa.lane_count[lane] = tcount;
 }
 
+// Waves per SIMD the adjoint asks for: four, except where its segment states in LDS
+// (CMAX x 3 doubles x 64 lanes per block) leave room for fewer.
+constexpr int persist_adj_waves(int cmax) { return cmax <= 8 ? 4 : cmax == 16 ? 2 : 1; }
+
 // One lane walks its trajectories backwards, one segment record per iteration.
 template <class F, int CMAX, bool FP32>
-__global__ __launch_bounds__(kPAdjBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(kPAdjBlock)
+__attribute__((amdgpu_waves_per_eu(persist_adj_waves(CMAX))))
 void persist_adj_kernel(PersistAdjArgs pa) {
   static_assert(F::autonomous, "the persistent engine is built for autonomous fields");
   constexpr int D = F::D, P = F::P, DCK = ckpt_rows<F>();
